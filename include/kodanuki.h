@@ -1,0 +1,328 @@
+/*
+ * kodanuki.h — the C ABI of the MI355X render hot path.
+ *
+ * One shared library, libkodanuki_hip.so, exports every function below.
+ * Signatures hold plain C types only: pointers, sizes, PODs. Every call
+ * returns 0 on success or a nonzero KDN_E* code; the message of the last
+ * failure is read back with kdn_last_error().
+ *
+ * Host-owned buffers are passed as plain pointers. Entry points whose
+ * names end in _device take device pointers (HBM) and a HIP stream handle
+ * (NULL = the null stream) so that a caller that keeps its inputs resident
+ * hands them over without a copy.
+ *
+ * The reference is a single process C++26 renderer whose hot path is
+ * reached through the plugin hooks of source/core/plugin.h. Each entry
+ * point names the reference interface it stands in for.
+ */
+#ifndef KODANUKI_H
+#define KODANUKI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes -------------------------------------------------------- */
+#define KDN_OK 0
+#define KDN_E_ARGUMENT 1   /* a bad argument or a size beyond a stated limit */
+#define KDN_E_SCENE 2      /* the .sdf text does not load */
+#define KDN_E_DEVICE 3     /* a HIP call failed */
+#define KDN_E_STATE 4      /* a call out of order (no scene, no octree) */
+#define KDN_E_MEMORY 5     /* a device allocation failed */
+
+/* ---- the flat instruction tape ------------------------------------------ */
+/*
+ * One step of a program: the reference Step (source/sdf/program.h:56-63)
+ * with u16 register indices (the reference's own per-cell limit, KeptSteps
+ * entries are u16, program.h:104-128) and an FP32 value. 16 bytes.
+ *
+ * flags, for a Constant: 0 when value is the scene's double exactly,
+ * 1 when the double lies just above value (its enclosure is
+ * [value, nextup(value)]), 2 when it lies just below.
+ */
+typedef struct kdn_step {
+    uint8_t op;
+    uint8_t flags;
+    uint16_t a, b, c;
+    float value;
+    uint32_t leaf;
+} kdn_step;
+
+/* Opcodes, numbered as the reference Opcode enum (program.h:20-54). */
+enum {
+    KDN_CONSTANT = 0, KDN_ARGUMENT, KDN_POINT_X, KDN_POINT_Y, KDN_POINT_Z,
+    KDN_NORMAL_X, KDN_NORMAL_Y, KDN_NORMAL_Z, KDN_ADD, KDN_SUBTRACT,
+    KDN_MULTIPLY, KDN_DIVIDE, KDN_NEGATE, KDN_SQUARE_ROOT, KDN_ABSOLUTE,
+    KDN_FLOOR, KDN_SINE, KDN_COSINE, KDN_SIGN, KDN_ARC_TANGENT, KDN_POWER,
+    KDN_MINIMUM, KDN_MAXIMUM, KDN_SMOOTH_MINIMUM, KDN_POSITIVE_SQUARE,
+    KDN_LESS, KDN_LESS_EQUAL, KDN_EQUAL, KDN_NOT_EQUAL, KDN_AND, KDN_OR,
+    KDN_NOT, KDN_SELECT, KDN_OPCODE_COUNT
+};
+
+/* An object program inside a tape: steps [first, first + count), result
+ * relative to first. */
+typedef struct kdn_object {
+    uint32_t first;
+    uint32_t count;
+    uint32_t result;
+    uint32_t material;   /* the material of its first leaf (informational) */
+} kdn_object;
+
+#define KDN_NO_LEAF 0xFFFFFFFFu
+#define KDN_NO_MATERIAL 0xFFFFFFFFu
+
+/* ---- rays and hits -------------------------------------------------------- */
+/* The reference Ray (source/core/batch.h:125-155), 36 bytes. */
+typedef struct kdn_ray {
+    float origin[3];
+    float direction[3];
+    float far_;
+    uint32_t inside;
+    uint32_t within;
+} kdn_ray;
+
+#define KDN_HIT_MISSED 0
+#define KDN_HIT_SURFACE 1
+#define KDN_HIT_SOLID 2
+
+/* The reference Hit (source/core/batch.h:157-205) less its tree-only
+ * fields, 48 bytes. */
+typedef struct kdn_hit {
+    uint32_t kind;
+    float distance;
+    float point[3];
+    float normal[3];
+    uint32_t leaf;
+    uint32_t other;
+    float share;
+    uint32_t cell;
+} kdn_hit;
+
+/* ---- the loaded scene (host) -------------------------------------------- */
+typedef struct kdn_scene kdn_scene;
+
+#define KDN_MATTE 0
+#define KDN_PLASTIC 1
+#define KDN_METAL 2
+#define KDN_GLASS 3
+
+/* A material as the scene says it (source/sdf/scene.h:337-402, the
+ * fields the render hot path reads). */
+typedef struct kdn_material {
+    uint32_t kind;
+    float albedo[3];
+    float roughness;
+    float refraction;
+    float dispersion;
+    float glow[3];
+    float strength;
+} kdn_material;
+
+#define KDN_LIGHT_POINT 0
+#define KDN_LIGHT_QUAD 1
+
+/* A lamp (source/sdf/scene.h:481-528, quad and point kinds). */
+typedef struct kdn_light {
+    uint32_t kind;
+    float at[3];
+    float along[3];
+    float across[3];
+    float power[3];
+} kdn_light;
+
+/* A leaf as placed (source/sdf/scene.h:237-288): its primitive, the
+ * primitive's numbers, the world→local map and its material. */
+#define KDN_PRIM_SPHERE 0
+#define KDN_PRIM_BOX 1
+#define KDN_PRIM_TORUS 2
+#define KDN_PRIM_PLANE 3
+#define KDN_PRIM_CYLINDER 4
+#define KDN_PRIM_CAPSULE 5
+#define KDN_PRIM_ELLIPSOID 6
+typedef struct kdn_leaf {
+    uint32_t primitive;
+    uint32_t material;
+    double args[8];
+    double linear[9];
+    double origin[3];
+    double stretch;
+} kdn_leaf;
+
+/* A CSG node (source/sdf/scene.h:290-310): join 0 Leaf, 1 Merge, 2 Cut,
+ * 3 Clip; its children are the child_count entries of the children list
+ * from first_child on, as kdn_scene_nodes returns them. */
+typedef struct kdn_node {
+    uint32_t join;
+    double radius;
+    uint32_t leaf;
+    uint32_t first_child;
+    uint32_t child_count;
+} kdn_node;
+
+typedef struct kdn_camera {
+    float eye[3];
+    float target[3];
+    float up[3];
+    float fov;
+} kdn_camera;
+
+typedef struct kdn_render_settings {
+    int32_t width, height, seed, samples, surfaces;
+    float far_;
+    float sky_zenith[3];
+    float sky_horizon[3];
+    float sky_ground[3];
+} kdn_render_settings;
+
+typedef struct kdn_scene_info {
+    uint32_t steps;       /* steps over every object program */
+    uint32_t objects;
+    uint32_t leaves;
+    uint32_t nodes;
+    uint32_t top_nodes;   /* object roots */
+    uint32_t children;    /* entries of the children list */
+    uint32_t materials;
+    uint32_t lights;
+} kdn_scene_info;
+
+/*
+ * Load .sdf text and compile its object programs: the reference
+ * loadScene (source/syntax/evaluator.h:369) followed by objectPrograms
+ * (source/sdf/scene.cpp:290-296, scene.h:897 union split). base_dir is where
+ * `import` paths are read from (NULL: only the built-in standard.sdf).
+ * On failure err holds "line:column: message".
+ */
+int kdn_scene_load(const char* text, const char* base_dir, kdn_scene** out, char* err, size_t err_size);
+void kdn_scene_free(kdn_scene* scene);
+int kdn_scene_describe(const kdn_scene* scene, kdn_scene_info* info);
+/* Copy out the compiled tape: steps[info.steps], objects[info.objects]. */
+int kdn_scene_tape(const kdn_scene* scene, kdn_step* steps, kdn_object* objects);
+/* The same tape with its constants in double, for FP64 checks. */
+int kdn_scene_tape_values(const kdn_scene* scene, double* values);
+int kdn_scene_leaves(const kdn_scene* scene, kdn_leaf* leaves);
+int kdn_scene_nodes(const kdn_scene* scene, kdn_node* nodes, uint32_t* top_nodes, uint32_t* children);
+int kdn_scene_materials(const kdn_scene* scene, kdn_material* materials);
+int kdn_scene_lights(const kdn_scene* scene, kdn_light* lights);
+int kdn_scene_settings(const kdn_scene* scene, kdn_camera* camera, kdn_render_settings* render);
+/* The box the octree is built over: lower[3] and size[3], each size a
+ * power of two and each corner a multiple of the finest cell, so that
+ * every cell box is exact in FP32. It stands in for sceneRegion
+ * (source/plugins/geometry/tree/build.cpp:405, an interval search for the
+ * occupied bounds inside reachRegion, sphere_trace.cpp:39-46). */
+int kdn_scene_region(const kdn_scene* scene, int depth_cap, float* lower3, float* size3);
+
+/* ---- the device context -------------------------------------------------- */
+typedef struct kdn_ctx kdn_ctx;
+
+/* Open a context on a HIP device: stands in for Backend openWorkers
+ * (source/core/plugin.h:333-357). */
+int kdn_open(int device, kdn_ctx** out);
+void kdn_close(kdn_ctx* ctx);
+/* Copy the message of the last failure (NUL-terminated, truncated). */
+int kdn_last_error(const kdn_ctx* ctx, char* buf, size_t size);
+/* Last failure of a call that has no context (kdn_open, scene calls). */
+const char* kdn_global_error(void);
+
+/* Put the scene's tape, materials and lights in HBM: the Backend
+ * uploadBlock hook (source/core/plugin.h:356, core/block.h:21-49). */
+int kdn_upload_scene(kdn_ctx* ctx, const kdn_scene* scene);
+
+/* ---- the interval-cut octree ---------------------------------------------- */
+/*
+ * Build the octree level by level on the GPU: per cell, interval
+ * evaluation of every kept program over the cell box (sdf/numbers.cpp
+ * Interval ops, FP32 with outward ulp padding), the cut rewrite and prune
+ * (plugins/geometry/tree/cut.cpp:117-348), the cell decision cutCell
+ * (cut.cpp:446-476) and compaction of surviving cells. Stands in for
+ * Geometry buildWorld → buildTree (core/plugin.h:168, tree/build.cpp:985)
+ * with the split rule "Surface and level < depth_cap".
+ */
+typedef struct kdn_octree_stats {
+    uint32_t levels;                  /* depth_cap + 1 */
+    uint32_t nodes[16];               /* nodes per level */
+    uint32_t surface[16];             /* Surface nodes per level */
+    uint32_t filled[16];
+    uint32_t entries;                 /* (object, program) entries over all nodes */
+    uint64_t pool_steps;              /* steps in the program pool */
+    double build_ms;
+} kdn_octree_stats;
+
+int kdn_build_octree(kdn_ctx* ctx, const float* lower3, const float* size3, int depth_cap, kdn_octree_stats* stats);
+
+/* One octree node as downloaded: level, integer coordinates at its level,
+ * kind (0 Empty, 1 Filled, 2 Surface leaf, 3 Branch), first child index in
+ * the next level (Branch), and its (object, program) entries. */
+typedef struct kdn_node_record {
+    uint32_t level;
+    uint32_t x, y, z;
+    uint32_t kind;
+    uint32_t first_child;
+    uint32_t entry_first;
+    uint32_t entry_count;
+} kdn_node_record;
+
+typedef struct kdn_entry_record {
+    uint32_t object;
+    uint32_t program;     /* offset of the program header in the pool */
+} kdn_entry_record;
+
+/* Sizes of the built octree: nodes, entries, pool words. */
+int kdn_octree_sizes(kdn_ctx* ctx, uint64_t* nodes, uint64_t* entries, uint64_t* pool_steps);
+/* Download the whole octree. The pool holds each program as a header step
+ * (op = 255, a|b<<16 = length, c = result) followed by its steps. */
+int kdn_octree_download(kdn_ctx* ctx, kdn_node_record* nodes, kdn_entry_record* entries, kdn_step* pool);
+
+/* ---- extend: rays → hits -------------------------------------------------- */
+/*
+ * Trace rays through the octree: sphere tracing over each cell's cut
+ * programs, FP32, with hit leaf by the reference source walk. Stands in
+ * for Execution extendRays (core/plugin.h:183,
+ * plugins/execution/single/plugin.cpp:27-41) over Geometry
+ * sphere_trace (plugins/geometry/sphere_trace/sphere_trace.cpp:216-228).
+ */
+int kdn_extend(kdn_ctx* ctx, const kdn_ray* rays, uint32_t count, kdn_hit* hits);
+int kdn_extend_device(kdn_ctx* ctx, const void* rays, uint32_t count, void* hits, void* stream);
+
+/* ---- the wavefront integrator --------------------------------------------- */
+/*
+ * One pixel pass: `spp` samples of every pixel, from sample index
+ * `first_sample`, through the wavefront integrator (ray generation,
+ * extend, material-sorted shading, NEE shadow rays, MIS, hero-wavelength
+ * throughput, roulette), folded into FP64 accumulators. Stands in for the
+ * Backend hook the reference lacks (a whole pixel pass per call) over the
+ * Transport steps extendRays → arriveAtHits → connectLights →
+ * scatterWalks → fold (core/worker.cpp:31-39,
+ * plugins/transport/paths/plugin.cpp:107-155).
+ */
+typedef struct kdn_pass_desc {
+    int32_t width, height;
+    uint32_t first_sample;
+    uint32_t spp;
+    uint32_t seed;
+    uint32_t surfaces;      /* the most surfaces a path stands on */
+    uint32_t min_depth;     /* roulette starts at this depth (reference 4) */
+    uint32_t flags;         /* bit 0: keep the accumulators (progressive) */
+} kdn_pass_desc;
+
+typedef struct kdn_pass_stats {
+    double ms;
+    uint64_t segments;       /* extend rays traced */
+    uint64_t shadow_rays;
+    uint32_t bounces;        /* wavefront iterations */
+} kdn_pass_stats;
+
+int kdn_render_pass(kdn_ctx* ctx, const kdn_pass_desc* desc, kdn_pass_stats* stats);
+/* Read the FP64 accumulators: per pixel sum of colour (3 doubles), sum of
+ * squares of the luminance, and the sample count. */
+int kdn_read_accumulation(kdn_ctx* ctx, double* rgb_sum, double* sq_sum, uint32_t* count);
+/* The mean image as linear RGB floats (width*height*3). */
+int kdn_read_image(kdn_ctx* ctx, float* rgb);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

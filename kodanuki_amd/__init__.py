@@ -1,0 +1,349 @@
+"""kodanuki_amd — the MI355X render hot path behind Kodanuki's plugin hooks.
+
+This module is the Python face of libkodanuki_hip.so (include/kodanuki.h):
+ctypes mirrors of the C structs and thin wrappers that raise on a nonzero
+return code. It loads the in-tree library and fails loudly when it is not
+built; there is no CPU fallback.
+
+The objects mirror the reference's vocabulary: a Scene is what loadScene
+leaves behind (source/syntax/evaluator.h:369) with its object programs
+compiled (source/sdf/scene.cpp:290-296); a Device holds the scene, its
+octree and the integrator state in HBM, and answers extend (rays → hits)
+and pixel passes (source/core/plugin.h:168-357).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkodanuki_hip.so")
+
+KDN_HIT_MISSED, KDN_HIT_SURFACE, KDN_HIT_SOLID = 0, 1, 2
+NO_LEAF = 0xFFFFFFFF
+KIND_EMPTY, KIND_FILLED, KIND_SURFACE, KIND_BRANCH = 0, 1, 2, 3
+
+
+class Step(C.Structure):
+    _fields_ = [("op", C.c_uint8), ("flags", C.c_uint8), ("a", C.c_uint16), ("b", C.c_uint16),
+                ("c", C.c_uint16), ("value", C.c_float), ("leaf", C.c_uint32)]
+
+
+class Object(C.Structure):
+    _fields_ = [("first", C.c_uint32), ("count", C.c_uint32), ("result", C.c_uint32), ("material", C.c_uint32)]
+
+
+class Ray(C.Structure):
+    _fields_ = [("origin", C.c_float * 3), ("direction", C.c_float * 3), ("far_", C.c_float),
+                ("inside", C.c_uint32), ("within", C.c_uint32)]
+
+
+class Hit(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("distance", C.c_float), ("point", C.c_float * 3), ("normal", C.c_float * 3),
+                ("leaf", C.c_uint32), ("other", C.c_uint32), ("share", C.c_float), ("cell", C.c_uint32)]
+
+
+class Material(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("albedo", C.c_float * 3), ("roughness", C.c_float), ("refraction", C.c_float),
+                ("dispersion", C.c_float), ("glow", C.c_float * 3), ("strength", C.c_float)]
+
+
+class Light(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("at", C.c_float * 3), ("along", C.c_float * 3), ("across", C.c_float * 3),
+                ("power", C.c_float * 3)]
+
+
+class Leaf(C.Structure):
+    _fields_ = [("primitive", C.c_uint32), ("material", C.c_uint32), ("args", C.c_double * 8), ("linear", C.c_double * 9),
+                ("origin", C.c_double * 3), ("stretch", C.c_double)]
+
+
+class Node(C.Structure):
+    _fields_ = [("join", C.c_uint32), ("radius", C.c_double), ("leaf", C.c_uint32), ("first_child", C.c_uint32),
+                ("child_count", C.c_uint32)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("target", C.c_float * 3), ("up", C.c_float * 3), ("fov", C.c_float)]
+
+
+class RenderSettings(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("seed", C.c_int32), ("samples", C.c_int32),
+                ("surfaces", C.c_int32), ("far_", C.c_float), ("sky_zenith", C.c_float * 3),
+                ("sky_horizon", C.c_float * 3), ("sky_ground", C.c_float * 3)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("steps", C.c_uint32), ("objects", C.c_uint32), ("leaves", C.c_uint32), ("nodes", C.c_uint32),
+                ("top_nodes", C.c_uint32), ("children", C.c_uint32), ("materials", C.c_uint32), ("lights", C.c_uint32)]
+
+
+class OctreeStats(C.Structure):
+    _fields_ = [("levels", C.c_uint32), ("nodes", C.c_uint32 * 16), ("surface", C.c_uint32 * 16),
+                ("filled", C.c_uint32 * 16), ("entries", C.c_uint32), ("pool_steps", C.c_uint64), ("build_ms", C.c_double)]
+
+
+class NodeRecord(C.Structure):
+    _fields_ = [("level", C.c_uint32), ("x", C.c_uint32), ("y", C.c_uint32), ("z", C.c_uint32), ("kind", C.c_uint32),
+                ("first_child", C.c_uint32), ("entry_first", C.c_uint32), ("entry_count", C.c_uint32)]
+
+
+class EntryRecord(C.Structure):
+    _fields_ = [("object", C.c_uint32), ("program", C.c_uint32)]
+
+
+class PassDesc(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("first_sample", C.c_uint32), ("spp", C.c_uint32),
+                ("seed", C.c_uint32), ("surfaces", C.c_uint32), ("min_depth", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class PassStats(C.Structure):
+    _fields_ = [("ms", C.c_double), ("segments", C.c_uint64), ("shadow_rays", C.c_uint64), ("bounces", C.c_uint32)]
+
+
+# numpy views of the structs
+STEP_DTYPE = np.dtype([("op", "u1"), ("flags", "u1"), ("a", "<u2"), ("b", "<u2"), ("c", "<u2"), ("value", "<f4"), ("leaf", "<u4")])
+RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("direction", "<f4", 3), ("far_", "<f4"), ("inside", "<u4"), ("within", "<u4")])
+HIT_DTYPE = np.dtype([("kind", "<u4"), ("distance", "<f4"), ("point", "<f4", 3), ("normal", "<f4", 3), ("leaf", "<u4"),
+                      ("other", "<u4"), ("share", "<f4"), ("cell", "<u4")])
+NODE_DTYPE = np.dtype([("level", "<u4"), ("x", "<u4"), ("y", "<u4"), ("z", "<u4"), ("kind", "<u4"), ("first_child", "<u4"),
+                       ("entry_first", "<u4"), ("entry_count", "<u4")])
+ENTRY_DTYPE = np.dtype([("object", "<u4"), ("program", "<u4")])
+OBJECT_DTYPE = np.dtype([("first", "<u4"), ("count", "<u4"), ("result", "<u4"), ("material", "<u4")])
+assert STEP_DTYPE.itemsize == 16 and RAY_DTYPE.itemsize == 36 and HIT_DTYPE.itemsize == 48
+
+# Every symbol include/kodanuki.h declares.
+EXPORTS = [
+    "kdn_scene_load", "kdn_scene_free", "kdn_scene_describe", "kdn_scene_tape", "kdn_scene_tape_values",
+    "kdn_scene_leaves", "kdn_scene_nodes", "kdn_scene_materials", "kdn_scene_lights", "kdn_scene_settings",
+    "kdn_scene_region", "kdn_open", "kdn_close", "kdn_last_error", "kdn_global_error", "kdn_upload_scene",
+    "kdn_build_octree", "kdn_octree_sizes", "kdn_octree_download", "kdn_extend", "kdn_extend_device",
+    "kdn_render_pass", "kdn_read_accumulation", "kdn_read_image",
+]
+
+_lib = None
+
+
+class KodanukiError(RuntimeError):
+    pass
+
+
+def library():
+    """Load libkodanuki_hip.so from the package folder (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise KodanukiError(f"{LIB_PATH} is not built: run __graft_entry__.build() (hipcc, gfx950)")
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    sig = {
+        "kdn_scene_load": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(P), C.c_char_p, C.c_size_t]),
+        "kdn_scene_free": (None, [P]),
+        "kdn_scene_describe": (C.c_int, [P, C.POINTER(SceneInfo)]),
+        "kdn_scene_tape": (C.c_int, [P, P, P]),
+        "kdn_scene_tape_values": (C.c_int, [P, P]),
+        "kdn_scene_leaves": (C.c_int, [P, P]),
+        "kdn_scene_nodes": (C.c_int, [P, P, P, P]),
+        "kdn_scene_materials": (C.c_int, [P, P]),
+        "kdn_scene_lights": (C.c_int, [P, P]),
+        "kdn_scene_settings": (C.c_int, [P, C.POINTER(Camera), C.POINTER(RenderSettings)]),
+        "kdn_scene_region": (C.c_int, [P, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "kdn_open": (C.c_int, [C.c_int, C.POINTER(P)]),
+        "kdn_close": (None, [P]),
+        "kdn_last_error": (C.c_int, [P, C.c_char_p, C.c_size_t]),
+        "kdn_global_error": (C.c_char_p, []),
+        "kdn_upload_scene": (C.c_int, [P, P]),
+        "kdn_build_octree": (C.c_int, [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int, C.POINTER(OctreeStats)]),
+        "kdn_octree_sizes": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+        "kdn_octree_download": (C.c_int, [P, P, P, P]),
+        "kdn_extend": (C.c_int, [P, P, C.c_uint32, P]),
+        "kdn_extend_device": (C.c_int, [P, P, C.c_uint32, P, P]),
+        "kdn_render_pass": (C.c_int, [P, C.POINTER(PassDesc), C.POINTER(PassStats)]),
+        "kdn_read_accumulation": (C.c_int, [P, P, P, P]),
+        "kdn_read_image": (C.c_int, [P, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc, ctx=None):
+    if rc != 0:
+        lib = library()
+        if ctx is not None:
+            buf = C.create_string_buffer(1024)
+            lib.kdn_last_error(ctx, buf, 1024)
+            msg = buf.value.decode()
+        else:
+            msg = lib.kdn_global_error().decode()
+        raise KodanukiError(f"kodanuki error {rc}: {msg}")
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Scene:
+    """A loaded and compiled .sdf scene (host memory)."""
+
+    def __init__(self, text, base_dir=None):
+        lib = library()
+        self._h = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        rc = lib.kdn_scene_load(text.encode(), base_dir.encode() if base_dir else None, C.byref(self._h), err, 1024)
+        if rc != 0:
+            raise KodanukiError(f"scene does not load: {err.value.decode()}")
+        self.info = SceneInfo()
+        _check(lib.kdn_scene_describe(self._h, C.byref(self.info)))
+
+    @classmethod
+    def from_file(cls, path):
+        with open(path) as f:
+            return cls(f.read(), os.path.dirname(os.path.abspath(path)))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            library().kdn_scene_free(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def tape(self):
+        """(steps, objects) as numpy structured arrays."""
+        steps = np.zeros(self.info.steps, STEP_DTYPE)
+        objects = np.zeros(self.info.objects, OBJECT_DTYPE)
+        _check(library().kdn_scene_tape(self._h, _ptr(steps), _ptr(objects)))
+        return steps, objects
+
+    def tape_values(self):
+        v = np.zeros(self.info.steps, np.float64)
+        _check(library().kdn_scene_tape_values(self._h, _ptr(v)))
+        return v
+
+    def leaves(self):
+        arr = (Leaf * max(1, self.info.leaves))()
+        _check(library().kdn_scene_leaves(self._h, C.cast(arr, C.c_void_p)))
+        return list(arr)[: self.info.leaves]
+
+    def nodes(self):
+        nodes = (Node * max(1, self.info.nodes))()
+        tops = np.zeros(max(1, self.info.top_nodes), np.uint32)
+        children = np.zeros(max(1, self.info.children), np.uint32)
+        _check(library().kdn_scene_nodes(self._h, C.cast(nodes, C.c_void_p), _ptr(tops), _ptr(children)))
+        return list(nodes)[: self.info.nodes], tops[: self.info.top_nodes], children[: self.info.children]
+
+    def materials(self):
+        arr = (Material * max(1, self.info.materials))()
+        if self.info.materials:
+            _check(library().kdn_scene_materials(self._h, C.cast(arr, C.c_void_p)))
+        return list(arr)[: self.info.materials]
+
+    def lights(self):
+        arr = (Light * max(1, self.info.lights))()
+        if self.info.lights:
+            _check(library().kdn_scene_lights(self._h, C.cast(arr, C.c_void_p)))
+        return list(arr)[: self.info.lights]
+
+    def settings(self):
+        cam, rs = Camera(), RenderSettings()
+        _check(library().kdn_scene_settings(self._h, C.byref(cam), C.byref(rs)))
+        return cam, rs
+
+    def region(self, depth):
+        """(lower[3], size[3]) of the octree's box for a depth cap."""
+        lower = (C.c_float * 3)()
+        size = (C.c_float * 3)()
+        _check(library().kdn_scene_region(self._h, depth, lower, size))
+        return np.array(lower[:], np.float32), np.array(size[:], np.float32)
+
+
+class Device:
+    """A context on one HIP device: scene, octree and integrator in HBM."""
+
+    def __init__(self, device=0):
+        lib = library()
+        self._h = C.c_void_p()
+        _check(lib.kdn_open(device, C.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().kdn_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def upload(self, scene):
+        _check(library().kdn_upload_scene(self._h, scene.handle), self._h)
+
+    def build_octree(self, lower, size, depth):
+        stats = OctreeStats()
+        lo = (C.c_float * 3)(*[float(x) for x in lower])
+        sz = (C.c_float * 3)(*[float(x) for x in size])
+        _check(library().kdn_build_octree(self._h, lo, sz, depth, C.byref(stats)), self._h)
+        return stats
+
+    def octree(self):
+        """(nodes, entries, pool) as numpy structured arrays."""
+        n, e, p = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        lib = library()
+        _check(lib.kdn_octree_sizes(self._h, C.byref(n), C.byref(e), C.byref(p)), self._h)
+        nodes = np.zeros(n.value, NODE_DTYPE)
+        entries = np.zeros(e.value, ENTRY_DTYPE)
+        pool = np.zeros(p.value, STEP_DTYPE)
+        _check(lib.kdn_octree_download(self._h, _ptr(nodes), _ptr(entries), _ptr(pool)), self._h)
+        return nodes, entries, pool
+
+    def extend(self, rays):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        _check(library().kdn_extend(self._h, _ptr(rays), len(rays), _ptr(hits)), self._h)
+        return hits
+
+    def extend_device(self, rays_ptr, count, hits_ptr, stream=None):
+        _check(library().kdn_extend_device(self._h, C.c_void_p(rays_ptr), count, C.c_void_p(hits_ptr),
+                                           C.c_void_p(stream) if stream else None), self._h)
+
+    def render_pass(self, width, height, spp, first_sample=0, seed=1, surfaces=7, min_depth=4, keep=False):
+        d = PassDesc(width, height, first_sample, spp, seed, surfaces, min_depth, 1 if keep else 0)
+        st = PassStats()
+        _check(library().kdn_render_pass(self._h, C.byref(d), C.byref(st)), self._h)
+        return st
+
+    def accumulation(self, width, height):
+        rgb = np.zeros((height, width, 3), np.float64)
+        sq = np.zeros((height, width), np.float64)
+        n = np.zeros((height, width), np.uint32)
+        _check(library().kdn_read_accumulation(self._h, _ptr(rgb), _ptr(sq), _ptr(n)), self._h)
+        return rgb, sq, n
+
+    def image(self, width, height):
+        rgb = np.zeros((height, width, 3), np.float32)
+        _check(library().kdn_read_image(self._h, _ptr(rgb)), self._h)
+        return rgb
+
+
+def make_rays(origins, directions, far=np.inf, inside=0, within=NO_LEAF):
+    r = np.zeros(len(origins), RAY_DTYPE)
+    r["origin"] = origins
+    r["direction"] = directions
+    r["far_"] = far
+    r["inside"] = inside
+    r["within"] = within
+    return r

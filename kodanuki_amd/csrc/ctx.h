@@ -1,0 +1,106 @@
+// The device context behind kdn_ctx: every buffer the hot path keeps
+// resident in HBM, and the launchers the C ABI calls.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "kodanuki.h"
+
+namespace kdn {
+
+// An octree node as kept on the device, 32 bytes: the downloaded
+// kdn_node_record layout.
+struct NodeDev {
+    uint32_t level, x, y, z, kind, first_child, entry_first, entry_count;
+};
+
+enum NodeKind : uint32_t { KindEmpty = 0, KindFilled = 1, KindSurface = 2, KindBranch = 3 };
+
+struct EntryDev {
+    uint32_t object, program;
+};
+
+// A device buffer that grows by doubling and keeps its contents.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct SceneDev {
+    kdn_step* steps = nullptr;       // the scene tape
+    kdn_object* objects = nullptr;
+    uint32_t n_steps = 0, n_objects = 0;
+    kdn_material* materials = nullptr;
+    uint32_t n_materials = 0;
+    kdn_light* lights = nullptr;
+    uint32_t n_lights = 0;
+    uint32_t* leaf_material = nullptr;  // per leaf
+    uint32_t n_leaves = 0;
+    kdn_camera camera{};
+    kdn_render_settings render{};
+};
+
+struct OctreeDev {
+    bool built = false;
+    float lower[3] = {0, 0, 0};
+    float size[3] = {0, 0, 0};
+    int depth = 0;
+    std::vector<uint32_t> level_first;   // first node of each level in nodes
+    std::vector<uint32_t> level_count;
+    DevBuf nodes;        // NodeDev, levels one after another
+    DevBuf entries;      // EntryDev
+    DevBuf pool;         // kdn_step programs with a header step each
+    uint64_t n_nodes = 0, n_entries = 0, n_pool = 0;
+    uint32_t* level_first_dev = nullptr;  // device copy of level_first
+};
+
+struct PathState;
+
+struct Ctx {
+    int device = 0;
+    std::string error;
+    hipStream_t stream = nullptr;
+    SceneDev scene;
+    OctreeDev tree;
+    // integrator state
+    int width = 0, height = 0;
+    double* accum = nullptr;        // rgb sums, FP64, per pixel
+    double* accum_sq = nullptr;     // luminance squares, FP64, per pixel
+    uint32_t* accum_n = nullptr;    // samples per pixel
+    PathState* paths = nullptr;
+    size_t path_capacity = 0;
+    DevBuf scratch[8];              // reusable scratch for launches
+    float* wavelength_cuts = nullptr;  // 257 cuts + xyz tables
+    float* cie = nullptr;
+};
+
+bool ensure(DevBuf& b, size_t bytes, std::string& err, bool keep);
+void releaseAll(Ctx& c);
+
+// octree.hip
+bool buildOctree(Ctx& c, const float lower[3], const float size[3], int depth, kdn_octree_stats* stats);
+
+// trace.hip
+bool extendRays(Ctx& c, const kdn_ray* rays, uint32_t n, kdn_hit* hits, hipStream_t stream);
+
+// integrator.hip
+bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats);
+bool uploadTables(Ctx& c);
+
+// scan.hip: exclusive prefix sum of n u32 values into out, returns the total
+bool scanU32(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* total, DevBuf& tmp, hipStream_t s, std::string& err);
+
+#define KDN_HIP(call)                                                                 \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            c.error = std::string(#call) + ": " + hipGetErrorString(e_);              \
+            return false;                                                             \
+        }                                                                             \
+    } while (0)
+
+}  // namespace kdn

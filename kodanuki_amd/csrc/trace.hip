@@ -1,0 +1,42 @@
+// The extend step: rays in, hits out, one lane per ray.
+#include <hip/hip_runtime.h>
+
+#include "trace_dev.h"
+
+namespace kdn {
+
+__global__ void __launch_bounds__(256) k_extend(TreeView tv, const kdn_ray* rays, uint32_t n, kdn_hit* hits) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Dl buf[kMaxRegs];
+    kdn_hit hit;
+    trace_ray(tv, rays[i], buf, hit);
+    hits[i] = hit;
+}
+
+TreeView treeView(const Ctx& c) {
+    TreeView tv;
+    tv.nodes = (const NodeDev*)c.tree.nodes.ptr;
+    tv.entries = (const EntryDev*)c.tree.entries.ptr;
+    tv.pool = (const kdn_step*)c.tree.pool.ptr;
+    tv.leafMaterial = c.scene.leaf_material;
+    for (int k = 0; k < 3; ++k) {
+        tv.lower[k] = c.tree.lower[k];
+        tv.size[k] = c.tree.size[k];
+    }
+    tv.depth = c.tree.depth;
+    return tv;
+}
+
+bool extendRays(Ctx& c, const kdn_ray* rays, uint32_t n, kdn_hit* hits, hipStream_t stream) {
+    if (!c.tree.built) {
+        c.error = "extend before the octree is built";
+        return false;
+    }
+    if (n == 0) return true;
+    hipLaunchKernelGGL(k_extend, dim3((n + 255) / 256), dim3(256), 0, stream, treeView(c), rays, n, hits);
+    KDN_HIP(hipGetLastError());
+    return true;
+}
+
+}  // namespace kdn

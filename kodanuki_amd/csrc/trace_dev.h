@@ -1,0 +1,323 @@
+// Sphere tracing through the interval-cut octree, one ray per lane.
+//
+// The ray walks the octree's leaves front to back: it descends from the
+// root by integer cell coordinates (exact, no point location in floats),
+// skips Empty leaves to their exit, and inside a Surface leaf marches by
+// the least value of the leaf's cut programs, which equal the whole
+// programs inside the leaf (cut.cpp's invariant), so each step is a safe
+// sphere-tracing step. Leaving a leaf it steps to the face neighbour
+// across its exit face.
+//
+// Landing, tie-break, inside rays, normals and the hit leaf follow the
+// reference sphere_trace plugin (plugins/geometry/sphere_trace/
+// sphere_trace.cpp:103-228) with FP32 tolerances.
+#pragma once
+
+#include "ctx.h"
+#include "tape_dev.h"
+
+namespace kdn {
+
+constexpr int kMaxRegs = 384;         // registers of a leaf program (checked at build)
+constexpr int kMaxIterations = 4096;  // cells plus march steps of one ray
+constexpr float kTolAbs = 1e-5f;      // landing tolerance: kTolAbs + kTolRel * t
+constexpr float kTolRel = 2e-6f;
+constexpr float kTie = 1e-6f;         // values within kTie (1 + max|p|) tie (sphere_trace.cpp:103-113)
+constexpr float kGraze = 0.02f;       // sphere_trace.cpp:16
+
+struct TreeView {
+    const NodeDev* nodes;
+    const EntryDev* entries;
+    const kdn_step* pool;
+    const uint32_t* leafMaterial;
+    float lower[3];
+    float size[3];
+    int depth;
+};
+
+struct Regs {
+    Dl* d;
+    KDN_D float& operator[](uint32_t i) const { return d[i].v; }
+};
+struct DRegs {
+    Dl* d;
+    KDN_D Dl& operator[](uint32_t i) const { return d[i]; }
+};
+
+KDN_D float eval_program(const kdn_step* pool, uint32_t at, const Regs& r, float px, float py, float pz) {
+    const kdn_step h = pool[at];
+    const uint32_t L = (uint32_t)h.a | ((uint32_t)h.b << 16);
+    const kdn_step* prog = pool + at + 1;
+    for (uint32_t i = 0; i < L; ++i) r[i] = point_step(prog[i], r, px, py, pz);
+    return r[h.c];
+}
+
+KDN_D void eval_dual(const kdn_step* pool, uint32_t at, const DRegs& r, float px, float py, float pz, Dl* out) {
+    const kdn_step h = pool[at];
+    const uint32_t L = (uint32_t)h.a | ((uint32_t)h.b << 16);
+    const kdn_step* prog = pool + at + 1;
+    for (uint32_t i = 0; i < L; ++i) r[i] = dual_step(prog[i], r, px, py, pz);
+    *out = r[h.c];
+}
+
+struct LeafField {
+    float least;      // least value over the leaf's programs
+    float nearest;    // least |value|
+    uint32_t arg;     // entry of the least value (first on ties)
+};
+
+KDN_D LeafField leaf_field(const TreeView& tv, const NodeDev& n, const Regs& r, float px, float py, float pz) {
+    LeafField f{kInf, kInf, 0};
+    for (uint32_t j = 0; j < n.entry_count; ++j) {
+        const float v = eval_program(tv.pool, tv.entries[n.entry_first + j].program, r, px, py, pz);
+        if (v < f.least) {
+            f.least = v;
+            f.arg = j;
+        }
+        f.nearest = fmin_(f.nearest, __builtin_fabsf(v));
+    }
+    return f;
+}
+
+KDN_D float vmax3(float a, float b, float c) { return fmax_(fmax_(a, b), c); }
+
+// Normal from the gradient, faced as sphereNormal (sphere_trace.cpp:133-149).
+KDN_D void faced_normal(const float g[3], const float d[3], bool inside, float n[3]) {
+    const float len = __builtin_sqrtf(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    if (!(len > 0)) {
+        n[0] = -d[0]; n[1] = -d[1]; n[2] = -d[2];
+        return;
+    }
+    float u[3] = {g[0] / len, g[1] / len, g[2] / len};
+    const float along = u[0] * d[0] + u[1] * d[1] + u[2] * d[2];
+    if (inside) {
+        const float s = along < 0 ? -1.0f : 1.0f;
+        n[0] = s * u[0]; n[1] = s * u[1]; n[2] = s * u[2];
+        return;
+    }
+    if (along > 0) {
+        float w[3] = {u[0] - d[0] * (along + kGraze), u[1] - d[1] * (along + kGraze), u[2] - d[2] * (along + kGraze)};
+        const float l = __builtin_sqrtf(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+        n[0] = w[0] / l; n[1] = w[1] / l; n[2] = w[2] / l;
+        return;
+    }
+    n[0] = u[0]; n[1] = u[1]; n[2] = u[2];
+}
+
+// The hit at t from entry `winner` of leaf n.
+KDN_D void land(const TreeView& tv, const NodeDev& n, uint32_t nodeIndex, uint32_t winner, const float o[3], const float d[3], float t,
+                bool inside, Dl* buf, kdn_hit& hit) {
+    const float px = o[0] + d[0] * t, py = o[1] + d[1] * t, pz = o[2] + d[2] * t;
+    const uint32_t at = tv.entries[n.entry_first + winner].program;
+    Dl value;
+    eval_dual(tv.pool, at, DRegs{buf}, px, py, pz, &value);
+    hit.kind = KDN_HIT_SURFACE;
+    hit.distance = t;
+    hit.point[0] = px; hit.point[1] = py; hit.point[2] = pz;
+    faced_normal(value.g, d, inside, hit.normal);
+    const kdn_step h = tv.pool[at];
+    source_of(tv.pool + at + 1, h.c, Regs{buf}, &hit.leaf, &hit.other, &hit.share);
+    hit.cell = nodeIndex;
+}
+
+// The winner of a landing: the least value, ties going to the least
+// value a little further along (sphereWinner, sphere_trace.cpp:103-113).
+KDN_D uint32_t winner_at(const TreeView& tv, const NodeDev& n, const Regs& r, const float o[3], const float d[3], float t,
+                         float inset, const LeafField& f) {
+    if (n.entry_count == 1) return 0;
+    const float px = o[0] + d[0] * t, py = o[1] + d[1] * t, pz = o[2] + d[2] * t;
+    const float band = kTie * (1 + vmax3(__builtin_fabsf(px), __builtin_fabsf(py), __builtin_fabsf(pz)));
+    const float qx = o[0] + d[0] * (t + inset), qy = o[1] + d[1] * (t + inset), qz = o[2] + d[2] * (t + inset);
+    uint32_t winner = f.arg;
+    float winnerOn = eval_program(tv.pool, tv.entries[n.entry_first + f.arg].program, r, qx, qy, qz);
+    for (uint32_t j = 0; j < n.entry_count; ++j) {
+        const uint32_t at = tv.entries[n.entry_first + j].program;
+        const float v = eval_program(tv.pool, at, r, px, py, pz);
+        if (!(v - f.least <= band)) continue;
+        const float on = eval_program(tv.pool, at, r, qx, qy, qz);
+        if (on < winnerOn) {
+            winner = j;
+            winnerOn = on;
+        }
+    }
+    return winner;
+}
+
+KDN_D void missed(kdn_hit& hit) {
+    hit.kind = KDN_HIT_MISSED;
+    hit.distance = kInf;
+    hit.point[0] = hit.point[1] = hit.point[2] = 0;
+    hit.normal[0] = hit.normal[1] = hit.normal[2] = 0;
+    hit.leaf = KDN_NO_LEAF;
+    hit.other = KDN_NO_LEAF;
+    hit.share = 0;
+    hit.cell = KDN_NO_LEAF;
+}
+
+KDN_D float slope_of(float part) {
+    const float m = 1e-12f;
+    return 1.0f / (__builtin_fabsf(part) < m ? (part < 0 ? -m : m) : part);
+}
+
+KDN_D void trace_ray(const TreeView& tv, const kdn_ray& ray, Dl* buf, kdn_hit& hit) {
+    const float o[3] = {ray.origin[0], ray.origin[1], ray.origin[2]};
+    const float d[3] = {ray.direction[0], ray.direction[1], ray.direction[2]};
+    const bool inside = ray.inside != 0;
+    // the span inside the region (sphereSpan, sphere_trace.cpp:51-69)
+    float t0 = 0.0f, t1 = ray.far_;
+    float inv[3];
+    for (int k = 0; k < 3; ++k) {
+        inv[k] = slope_of(d[k]);
+        const float a = (tv.lower[k] - o[k]) * inv[k];
+        const float b = (tv.lower[k] + tv.size[k] - o[k]) * inv[k];
+        t0 = fmax_(t0, fmin_(a, b));
+        t1 = fmin_(t1, fmax_(a, b));
+    }
+    missed(hit);
+    if (!(t0 <= t1)) return;
+    const int D = tv.depth;
+    const uint32_t N = 1u << D;
+    float finest[3];
+    uint32_t ci[3];
+    for (int k = 0; k < 3; ++k) {
+        finest[k] = __builtin_ldexpf(tv.size[k], -D);
+        const float p = o[k] + d[k] * t0;
+        const float q = __builtin_floorf((p - tv.lower[k]) / finest[k]);
+        ci[k] = q < 0 ? 0u : (q >= (float)N ? N - 1 : (uint32_t)q);
+    }
+    const Regs r{buf};
+    float t = t0;
+    int iterations = 0;
+    while (iterations < kMaxIterations) {
+        // descend by integer coordinates
+        uint32_t index = 0;
+        NodeDev n = tv.nodes[0];
+        int lvl = 0;
+        while (n.kind == KindBranch) {
+            ++lvl;
+            const int sh = D - lvl;
+            const uint32_t oct = ((ci[0] >> sh) & 1u) | (((ci[1] >> sh) & 1u) << 1) | (((ci[2] >> sh) & 1u) << 2);
+            index = n.first_child + oct;
+            n = tv.nodes[index];
+        }
+        const uint32_t nc[3] = {n.x, n.y, n.z};
+        float lo[3], hi[3];
+        float tExit = kInf;
+        int axis = 0;
+        for (int k = 0; k < 3; ++k) {
+            const float cell = __builtin_ldexpf(tv.size[k], -lvl);
+            lo[k] = tv.lower[k] + (float)nc[k] * cell;
+            hi[k] = lo[k] + cell;
+            const float bound = d[k] > 0 ? hi[k] : lo[k];
+            const float te = d[k] == 0 ? kInf : (bound - o[k]) * inv[k];
+            if (te < tExit) {
+                tExit = te;
+                axis = k;
+            }
+        }
+        const float tEnd = fmin_(tExit, t1);
+        if (n.kind == KindSurface) {
+            while (t <= tEnd && iterations < kMaxIterations) {
+                ++iterations;
+                const float px = o[0] + d[0] * t, py = o[1] + d[1] * t, pz = o[2] + d[2] * t;
+                const LeafField f = leaf_field(tv, n, r, px, py, pz);
+                const float tol = kTolAbs + kTolRel * t;
+                const float inset = 2 * tol;
+                float step;
+                if (!inside) {
+                    if (f.least < tol) {
+                        const uint32_t w = winner_at(tv, n, r, o, d, t, inset, f);
+                        land(tv, n, index, w, o, d, t, false, buf, hit);
+                        return;
+                    }
+                    step = f.least;
+                } else {
+                    if (f.nearest < tol) {
+                        const float qx = o[0] + d[0] * (t + inset), qy = o[1] + d[1] * (t + inset), qz = o[2] + d[2] * (t + inset);
+                        const LeafField past = leaf_field(tv, n, r, qx, qy, qz);
+                        if (past.least > 0) {
+                            const LeafField here = leaf_field(tv, n, r, px, py, pz);
+                            const uint32_t w = winner_at(tv, n, r, o, d, t, inset, here);
+                            land(tv, n, index, w, o, d, t, true, buf, hit);
+                            return;
+                        }
+                        // another solid past the crossing: a hit where it is of
+                        // another material (sphereCrossing, sphere_trace.cpp:176-187)
+                        kdn_hit other;
+                        land(tv, n, index, past.arg, o, d, t + inset, true, buf, other);
+                        if (other.leaf != KDN_NO_LEAF && tv.leafMaterial[other.leaf] != ray.within) {
+                            other.distance = t;
+                            other.point[0] = px; other.point[1] = py; other.point[2] = pz;
+                            hit = other;
+                            return;
+                        }
+                        step = inset;
+                    } else {
+                        step = f.nearest;
+                    }
+                }
+                // The leaf's programs bound the distance to surfaces inside
+                // it only (objects Empty here were dropped), so a step stops
+                // at the leaf's exit.
+                const float next = t + step;
+                if (next >= tExit) {
+                    t = tExit;
+                    break;
+                }
+                t = next;
+            }
+        } else if (n.kind == KindFilled && !inside) {
+            // a ray from outside enters a filled leaf: the side it meets
+            hit.kind = KDN_HIT_SOLID;
+            hit.distance = t;
+            hit.point[0] = o[0] + d[0] * t; hit.point[1] = o[1] + d[1] * t; hit.point[2] = o[2] + d[2] * t;
+            hit.normal[0] = hit.normal[1] = hit.normal[2] = 0;
+            hit.normal[axis] = d[axis] > 0 ? -1.0f : 1.0f;
+            hit.cell = index;
+            if (n.entry_count > 0) {
+                const uint32_t at = tv.entries[n.entry_first].program;
+                const float v = eval_program(tv.pool, at, r, hit.point[0], hit.point[1], hit.point[2]);
+                (void)v;
+                float sh;
+                source_of(tv.pool + at + 1, tv.pool[at].c, r, &hit.leaf, &hit.other, &sh);
+                hit.share = sh;
+            }
+            return;
+        } else if (n.kind == KindEmpty && inside) {
+            // a ray from inside stands in empty space: it left the solid at t
+            hit.kind = KDN_HIT_SURFACE;
+            hit.distance = t;
+            hit.point[0] = o[0] + d[0] * t; hit.point[1] = o[1] + d[1] * t; hit.point[2] = o[2] + d[2] * t;
+            hit.normal[0] = hit.normal[1] = hit.normal[2] = 0;
+            hit.normal[axis] = d[axis] > 0 ? 1.0f : -1.0f;
+            hit.cell = index;
+            return;
+        }
+        ++iterations;
+        if (tExit >= t1) return;
+        t = fmax_(t, tExit);
+        // the face neighbour across the exit face
+        const int sh = D - lvl;
+        uint32_t next[3];
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t first = nc[k] << sh, last = ((nc[k] + 1) << sh) - 1;
+            if (k == axis) {
+                if (d[k] > 0) {
+                    next[k] = last + 1;
+                } else {
+                    if (first == 0) return;
+                    next[k] = first - 1;
+                }
+            } else {
+                const float p = o[k] + d[k] * tExit;
+                const float q = __builtin_floorf((p - tv.lower[k]) / finest[k]);
+                const uint32_t qi = q < (float)first ? first : (q > (float)last ? last : (uint32_t)q);
+                next[k] = qi;
+            }
+        }
+        if (next[axis] >= N) return;
+        ci[0] = next[0]; ci[1] = next[1]; ci[2] = next[2];
+    }
+}
+
+}  // namespace kdn

@@ -1,0 +1,174 @@
+"""Seeded synthetic scenes, written as .sdf text for the scene loader.
+
+Random numbers are SplitMix64 (the reference's mixedBits,
+source/math/random.cpp:8-13) over a counter, so a scene is the same for a
+seed on every machine. The shapes are the standard.sdf primitives
+(sphere, box, torus, plane); the statements are those the loader reads.
+
+Workloads:
+- plain(32):   32 primitives, smooth-merged (k = 0.05) over a ground plane,
+               matte albedos U[0.2, 0.8], one quad light, 256x256.
+- production(512): 512 primitives in groups nested by cut/clip/merge to
+               depth 4, 40% matte / 30% rough metal / 30% dispersive glass,
+               four quad lights and a gradient sky, 1920x1080.
+- glass_heavy: the plain scene with 60% glass.
+"""
+import math
+
+MASK = (1 << 64) - 1
+
+
+def mixed_bits(v):
+    v = (v + 0x9E3779B97F4A7C15) & MASK
+    v = ((v ^ (v >> 30)) * 0xBF58476D1CE4E5B9) & MASK
+    v = ((v ^ (v >> 27)) * 0x94D049BB133111EB) & MASK
+    return v ^ (v >> 31)
+
+
+class Rng:
+    def __init__(self, seed):
+        self.stream = mixed_bits(seed)
+        self.n = 0
+
+    def unit(self):
+        self.n += 1
+        return (mixed_bits((self.stream + self.n) & MASK) >> 11) * 2.0 ** -53
+
+    def uniform(self, lo, hi):
+        return lo + (hi - lo) * self.unit()
+
+    def pick(self, weights):
+        u = self.unit() * sum(weights)
+        for i, w in enumerate(weights):
+            if u < w:
+                return i
+            u -= w
+        return len(weights) - 1
+
+
+def _v(x):
+    return "[" + ", ".join(f"{c:.6g}" for c in x) + "]"
+
+
+def _primitive(rng, scale=1.0):
+    kind = rng.pick([1, 1, 1])
+    if kind == 0:
+        r = rng.uniform(0.1, 0.5) * scale
+        return f"sphere({r:.6g})", r
+    if kind == 1:
+        size = [rng.uniform(0.2, 0.8) * scale for _ in range(3)]
+        r = min(0.05 * scale, min(size) / 4)
+        return f"box({_v(size)}, {r:.6g})", max(size) / 2
+    big = rng.uniform(0.15, 0.4) * scale
+    small = rng.uniform(0.04, 0.12) * scale
+    return f"torus({big:.6g}, {small:.6g})", big + small
+
+
+def _material(rng, name, kinds):
+    k = rng.pick(kinds)
+    if k == 0:
+        a = [rng.uniform(0.2, 0.8) for _ in range(3)]
+        return f"let {name} = matte({_v(a)});"
+    if k == 1:
+        a = [rng.uniform(0.5, 0.95) for _ in range(3)]
+        alpha = rng.uniform(0.05, 0.5)
+        return f"let {name} = metal({_v(a)}, roughness = {math.sqrt(alpha):.6g});"
+    A = rng.uniform(1.4, 1.7)
+    B = rng.uniform(0.003, 0.01)
+    return f"let {name} = glass(refraction = {A:.6g}, dispersion = {B:.6g});"
+
+
+def plain(seed=0, count=32, glass=0.0, width=256, height=256, samples=16, surfaces=7):
+    """The plain scene: `count` primitives smooth-merged over a plane."""
+    rng = Rng(seed)
+    out = ['import "standard.sdf";',
+           f"render {{ width = {width}; height = {height}; seed = {seed + 1}; samples = {samples}; surfaces = {surfaces}; far = 4; }}",
+           "camera { eye = [0, -6.5, 3.2]; target = [0, 0, 0.45]; up = [0, 0, 1]; fov = 0.75; }",
+           "light quad { at = [-0.75, -1.25, 4]; along = [0, 1.5, 0]; across = [1.5, 0, 0]; power = [40, 38, 34]; }"]
+    kinds = [1.0 - glass, 0.0, glass] if glass > 0 else [1.0, 0.0, 0.0]
+    for i in range(count):
+        out.append(_material(rng, f"m{i}", kinds))
+    out.append("let floor = matte([0.55, 0.55, 0.5]);")
+    out.append("merge(0.05) {")
+    side = max(1, int(math.ceil(math.sqrt(count))))
+    spacing = 3.6 / side
+    for i in range(count):
+        shape, reach = _primitive(rng)
+        gx, gy = i % side, i // side
+        x = -1.8 + spacing * (gx + 0.5) + rng.uniform(-0.15, 0.15)
+        y = -1.8 + spacing * (gy + 0.5) + rng.uniform(-0.15, 0.15)
+        z = reach * rng.uniform(0.6, 1.0)
+        ang = rng.uniform(0, math.pi)
+        out.append(f"\tpaint(m{i}) at({_v([x, y, z])}) turn(z, {ang:.6g}) {shape};")
+    out.append("}")
+    out.append("paint(floor) plane([0, 0, 1], 0);")
+    return "\n".join(out) + "\n"
+
+
+def glass_heavy(seed=0, **kw):
+    """The plain scene with 60% glass (caustics)."""
+    return plain(seed, glass=0.6, **kw)
+
+
+def _nest(rng, depth, leaves, materials):
+    """A CSG nest of `leaves` primitives to `depth` levels of joins."""
+    if depth == 0 or leaves == 1:
+        shape, _ = _primitive(rng, 0.8)
+        m = materials[int(rng.unit() * len(materials))]
+        off = [rng.uniform(-0.25, 0.25), rng.uniform(-0.25, 0.25), rng.uniform(-0.1, 0.2)]
+        return [f"paint({m}) at({_v(off)}) {shape};"]
+    join = ["merge", "cut", "clip"][rng.pick([0.5, 0.3, 0.2])]
+    radius = [0, 0.02, 0.05][rng.pick([0.4, 0.4, 0.2])]
+    head = f"{join}({radius:.6g}) {{" if radius > 0 else f"{join} {{"
+    left = leaves // 2
+    body = _nest(rng, depth - 1, left, materials) + _nest(rng, depth - 1, leaves - left, materials)
+    return [head] + ["\t" + line for line in body] + ["}"]
+
+
+def production(seed=0, count=512, width=1920, height=1080, samples=64, surfaces=13):
+    """The production scene: groups of 4 primitives nested to depth 4."""
+    rng = Rng(seed + 1000)
+    out = ['import "standard.sdf";',
+           f"render {{ width = {width}; height = {height}; seed = {seed + 1}; samples = {samples}; surfaces = {surfaces}; far = 8; }}",
+           "camera { eye = [0, -13, 6.5]; target = [0, 0, 0.3]; up = [0, 0, 1]; fov = 0.8; }",
+           "sky gradient { zenith = [0.25, 0.35, 0.6]; horizon = [0.6, 0.6, 0.62]; ground = [0.08, 0.08, 0.08]; }"]
+    for k, (x, y) in enumerate([(-4, -4), (3, -4), (-4, 3), (3, 3)]):
+        p = 30 + 10 * k
+        out.append(f"light quad {{ at = {_v([x, y, 5])}; along = [0, 1.2, 0]; across = [1.2, 0, 0]; power = {_v([p, p * 0.95, p * 0.9])}; }}")
+    nm = 48
+    for i in range(nm):
+        out.append(_material(rng, f"m{i}", [0.4, 0.3, 0.3]))
+    out.append("let floor = matte([0.5, 0.5, 0.48]);")
+    groups = count // 4
+    side = int(math.ceil(math.sqrt(groups)))
+    spacing = 12.0 / side
+    mats = [f"m{i}" for i in range(nm)]
+    for g in range(groups):
+        gx, gy = g % side, g // side
+        x = -6 + spacing * (gx + 0.5)
+        y = -6 + spacing * (gy + 0.5)
+        body = _nest(rng, 4, 4, mats)
+        out.append(f"at({_v([x, y, 0.35])}) turn(z, {rng.uniform(0, math.pi):.6g}) " + body[0])
+        out.extend(body[1:])
+    out.append("paint(floor) plane([0, 0, 1], 0);")
+    return "\n".join(out) + "\n"
+
+
+def tiny(seed=0):
+    """A handful of shapes for quick checks: a sphere, a box and a torus."""
+    return """import "standard.sdf";
+render { width = 64; height = 48; seed = 1; samples = 4; surfaces = 4; far = 3; }
+camera { eye = [0, -5, 2]; target = [0, 0, 0.4]; up = [0, 0, 1]; fov = 0.8; }
+light quad { at = [-0.5, -0.5, 3]; along = [0, 1, 0]; across = [1, 0, 0]; power = [20, 20, 20]; }
+let red = matte([0.7, 0.2, 0.2]);
+let steel = metal([0.9, 0.9, 0.9], roughness = 0.4);
+let pane = glass(refraction = 1.5, dispersion = 0.005);
+let grey = matte([0.5, 0.5, 0.5]);
+merge(0.1) {
+	paint(red) at([-0.7, 0, 0.4]) sphere(0.4);
+	paint(steel) at([0.1, 0.2, 0.3]) turn(z, 0.5) box([0.5, 0.6, 0.5], 0.05);
+}
+paint(pane) at([0.9, -0.3, 0.25]) torus(0.3, 0.1);
+cut(0.02) { paint(grey) at([0, 0.9, 0.3]) box([0.6, 0.6, 0.6], 0.02); at([0.2, 0.8, 0.5]) sphere(0.3); }
+paint(grey) plane([0, 0, 1], 0);
+"""

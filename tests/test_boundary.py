@@ -1,0 +1,62 @@
+"""The C-ABI boundary: the library loads, exports what the header
+declares, and reports errors as codes with messages (the reference's
+syntax layer returns "line:column: message", source/syntax/evaluator.h:342-369)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import kodanuki_amd as k
+from kodanuki_amd import scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "kodanuki.h")).read()
+    return sorted(set(re.findall(r"\b(kdn_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = k.library()
+    declared = header_symbols()
+    assert len(declared) >= 20
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert sorted(k.EXPORTS) == declared
+
+
+def test_scene_error_is_a_value_with_position():
+    with pytest.raises(k.KodanukiError) as e:
+        k.Scene("let a = ;")
+    assert re.search(r"\d+:\d+: ", str(e.value))
+    with pytest.raises(k.KodanukiError):
+        k.Scene('import "standard.sdf";\nspindle(1);')
+
+
+def test_device_calls_fail_loudly_without_a_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(k.KodanukiError):
+        k.Device(0)
+
+
+def test_region_is_exact_power_of_two_box():
+    s = k.Scene(scenes.plain(0))
+    for depth in (4, 6, 8):
+        lower, size = s.region(depth)
+        for kk in range(3):
+            m, e = np.frexp(size[kk])
+            assert m == 0.5
+            q = lower[kk] / np.ldexp(size[kk], -depth)
+            assert q == np.floor(q)
+
+
+def test_scene_settings_round_trip():
+    s = k.Scene(scenes.plain(3, width=320, height=200, samples=9))
+    cam, rs = s.settings()
+    assert (rs.width, rs.height, rs.samples) == (320, 200, 9)
+    assert s.info.lights == 1
+    assert s.info.leaves == 33

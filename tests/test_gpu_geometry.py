@@ -1,0 +1,123 @@
+"""GPU parity of the geometry hot path, through the C ABI.
+
+- the octree (node list, kept entries, cut programs) is the FP32 twin's to
+  the bit, at the plain scene's depth 6 and the production scene's depth 8;
+- hits (kind, distance, point, leaf, other, cell) are the twin's to the bit
+  on seeded camera and random ray sets, inside rays included;
+- hit distances fall within 1e-4 relative of the FP64 sphere trace of the
+  whole scene (plugins/geometry/sphere_trace), and the hit leaf is the
+  FP64 trace's away from seams and ties.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import kodanuki_amd as k
+from kodanuki_amd import scenes
+from tests import oracle_py as O
+from tests import raysets
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+CASES = {
+    "tiny": (scenes.tiny, 5),
+    "plain": (lambda: scenes.plain(0), 6),
+    "production": (lambda: scenes.production(0), 8),
+}
+
+
+@pytest.fixture(scope="module")
+def built(device):
+    out = {}
+    for name, (make, depth) in CASES.items():
+        s = k.Scene(make())
+        A = O.SceneArrays(s)
+        lower, size = s.region(depth)
+        note(f"[octree] {name} depth {depth} region {lower} {size}")
+        device.upload(s)
+        stats = device.build_octree(lower, size, depth)
+        nodes, entries, pool = device.octree()
+        tree, tn, te, tp = O.build_octree32(A, lower, size, depth)
+        out[name] = dict(scene=s, arrays=A, lower=lower, size=size, depth=depth, stats=stats,
+                         gpu=(nodes, entries, pool), twin=(tree, tn, te, tp))
+    yield out
+    for v in out.values():
+        O.lib().or_free_octree(v["twin"][0])
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_octree_bit_exact(built, name):
+    b = built[name]
+    nodes, entries, pool = b["gpu"]
+    _, tn, te, tp = b["twin"]
+    note(f"[octree] {name}: gpu nodes {len(nodes)} entries {len(entries)} pool {len(pool)} build {b['stats'].build_ms:.1f} ms")
+    assert len(nodes) == len(tn)
+    assert nodes.tobytes() == tn.tobytes()
+    assert len(entries) == len(te)
+    assert entries.tobytes() == te.tobytes()
+    assert len(pool) == len(tp)
+    assert pool.tobytes() == tp.tobytes()
+
+
+def _rays(b, count, seed):
+    cam = raysets.camera_rays(b["scene"], count // 2, seed)
+    rnd = raysets.random_rays(b["arrays"], b["lower"], b["size"], count - count // 2, seed + 1, inside_share=0.2)
+    return np.concatenate([cam, rnd])
+
+
+def _trace_both(device, b, rays):
+    s = b["scene"]
+    device.upload(s)
+    device.build_octree(b["lower"], b["size"], b["depth"])
+    hits = device.extend(rays)
+    twin = O.trace32(b["twin"][0], b["lower"], b["size"], b["depth"], b["arrays"].leaf_material, rays)
+    return hits, twin
+
+
+@pytest.mark.parametrize("name,count", [("tiny", 4096), ("plain", 16384), ("production", 8192)])
+def test_hits_bit_exact(device, built, name, count):
+    b = built[name]
+    rays = _rays(b, count, 42)
+    note(f"[extend] {name}: {len(rays)} rays")
+    hits, twin = _trace_both(device, b, rays)
+    for field in ("kind", "leaf", "other", "cell"):
+        bad = np.nonzero(hits[field] != twin[field])[0]
+        assert len(bad) == 0, (field, bad[:10], hits[bad[:3]], twin[bad[:3]])
+    for field in ("distance", "point", "normal", "share"):
+        assert hits[field].tobytes() == twin[field].tobytes(), field
+    hit_share = np.mean(hits["kind"] == k.KDN_HIT_SURFACE)
+    note(f"[extend] {name}: surface hits {hit_share:.3f}")
+    assert hit_share > 0.2
+
+
+@pytest.mark.parametrize("name,count", [("tiny", 1024), ("plain", 1024), ("production", 256)])
+def test_hits_against_fp64_sphere_trace(device, built, name, count):
+    b = built[name]
+    rays = _rays(b, count, 7)
+    rays = rays[rays["inside"] == 0]
+    hits, _ = _trace_both(device, b, rays)
+    ref, d64 = O.sphere_trace64(b["arrays"], b["lower"], b["size"], rays)
+    both = (hits["kind"] == k.KDN_HIT_SURFACE) & (ref["kind"] == k.KDN_HIT_SURFACE)
+    # the same rays hit and miss
+    agree = np.mean((hits["kind"] != k.KDN_HIT_MISSED) == (ref["kind"] != k.KDN_HIT_MISSED))
+    note(f"[fp64] {name}: hit/miss agreement {agree:.4f} over {len(rays)} rays")
+    assert agree > 0.995
+    rel = np.abs(hits["distance"][both].astype(np.float64) - d64[both]) / np.maximum(d64[both], 1e-3)
+    # grazing hits stop further from the surface along the ray; judge the
+    # bulk and the near-normal ones
+    d = rays["direction"][both].astype(np.float64)
+    n = ref["normal"][both].astype(np.float64)
+    facing = np.abs(np.sum(d * n, axis=1))
+    note(f"[fp64] {name}: median rel {np.median(rel):.2e} p99 {np.percentile(rel, 99):.2e}")
+    assert np.all(rel[facing > 0.2] < 1e-4)
+    assert np.mean(rel < 1e-4) > 0.99
+    same_leaf = np.mean(hits["leaf"][both] == ref["leaf"][both])
+    note(f"[fp64] {name}: leaf agreement {same_leaf:.4f}")
+    assert same_leaf > 0.99
