@@ -71,11 +71,15 @@ struct Ctx {
     double* accum = nullptr;        // rgb sums, FP64, per pixel
     double* accum_sq = nullptr;     // luminance squares, FP64, per pixel
     uint32_t* accum_n = nullptr;    // samples per pixel
-    PathState* paths = nullptr;
-    size_t path_capacity = 0;
+    PathState* paths = nullptr;     // host record of the path SoA in HBM
+    size_t max_paths = 0;           // paths in flight per chunk (0: 2^23)
     DevBuf scratch[8];              // reusable scratch for launches
-    float* wavelength_cuts = nullptr;  // 257 cuts + xyz tables
-    float* cie = nullptr;
+    void* colour = nullptr;         // ColourTables
+    void* materials_dev = nullptr;  // MaterialDev per material, the fallback last
+    uint32_t n_materials_dev = 0;
+    void* lamps_dev = nullptr;      // LampDev per quad lamp
+    uint32_t n_lamps = 0;
+    bool integrator_ready = false;
 };
 
 bool ensure(DevBuf& b, size_t bytes, std::string& err, bool keep);
@@ -89,7 +93,7 @@ bool extendRays(Ctx& c, const kdn_ray* rays, uint32_t n, kdn_hit* hits, hipStrea
 
 // integrator.hip
 bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats);
-bool uploadTables(Ctx& c);
+void freeIntegrator(Ctx& c);
 
 // scan.hip: exclusive prefix sum of n u32 values into out, returns the total
 bool scanU32(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* total, DevBuf& tmp, hipStream_t s, std::string& err);

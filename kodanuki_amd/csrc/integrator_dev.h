@@ -1,0 +1,390 @@
+// The plain spectral integrator on the device: counter-based numbers,
+// hero wavelengths, camera, materials, lamps and sky.
+//
+// Restates the reference transport (plugins/transport/paths) for the
+// plain configuration: numbers keyed (stream, sample, dimension)
+// (trace.cpp:47-74, path.cpp:15-46, random.cpp:19-21); four wavelength
+// lanes by the CIE-importance cuts (colour.cpp:236-255); RGB to spectrum
+// by smooth-step bases (spectrum.cpp:49-67, colour.cpp:88-103); matte,
+// GGX metal with energy compensation and exact dispersive glass
+// (material.cpp, microfacet.cpp); quad lamps met along rays and aimed at
+// (light.cpp); the gradient sky (light.cpp:582-591); power-heuristic MIS
+// between aiming and meeting; roulette from depth 4 with survival
+// min(0.95, max lane) (trace.cpp:1051-1055).
+#pragma once
+
+#include "tables.h"
+#include "trace_dev.h"
+
+namespace kdn {
+
+constexpr float kPi = 3.14159265358979f;
+constexpr int kLanes = 4;
+
+// ---- numbers --------------------------------------------------------------
+KDN_D uint64_t mixed_bits(uint64_t v) {
+    v += 0x9E3779B97F4A7C15ULL;
+    v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    v = (v ^ (v >> 27)) * 0x94D049BB133111EBULL;
+    return v ^ (v >> 31);
+}
+KDN_D double hashed(uint64_t stream, uint64_t index, uint64_t coordinate) {
+    const uint64_t bits = mixed_bits(stream + mixed_bits(index + mixed_bits(coordinate)));
+    return (double)(bits >> 11) * 0x1p-53;
+}
+// dimensionOf for the per-event draws LightU..Pick (path.cpp:22-37)
+enum Draw : int { PixelX = 0, PixelY = 1, LensU = 2, LensV = 3, LightU = 4, LightV = 5, DirectionU = 6, DirectionV = 7,
+                  Lobe = 8, Side = 9, Survival = 10, Pick = 11 };
+KDN_D uint64_t dimension(int event, int draw) {
+    if (draw < LightU) return (uint64_t)draw;
+    return (uint64_t)(4 + event * 8 + draw - LightU);
+}
+KDN_D float number(uint64_t stream, uint32_t sample, int event, int draw) {
+    return (float)hashed(stream, sample, dimension(event, draw));
+}
+
+// ---- wavelengths (colour.cpp:236-257, spectrum.cpp) ----------------------------
+struct Lanes {
+    float nm[kLanes];
+    float density[kLanes];
+};
+KDN_D float wavelength_share(double start, uint32_t index) {
+    const double turned = (double)index * 0.6180339887498949 + start;
+    return (float)(turned - __builtin_floor(turned));
+}
+KDN_D Lanes sampled_wavelengths(float u, const ColourTables* t) {
+    Lanes w;
+    for (int lane = 0; lane < kLanes; ++lane) {
+        const float raw = u + 0.25f * lane;
+        const float turned = raw >= 1 ? raw - 1 : raw;
+        const float place = turned * kWavelengthCuts;
+        const int index = (int)place < kWavelengthCuts - 1 ? (int)place : kWavelengthCuts - 1;
+        const float width = t->cuts[index + 1] - t->cuts[index];
+        w.nm[lane] = fmin_(t->cuts[index] + (place - index) * width, 699.99994f);
+        w.density[lane] = 1.0f / (kWavelengthCuts * width);
+    }
+    return w;
+}
+KDN_D float smooth_rise(float nm, float lower, float upper) {
+    float b = (nm - lower) / (upper - lower);
+    b = b < 0 ? 0.0f : (b > 1 ? 1.0f : b);
+    return b * b * (3 - 2 * b);
+}
+KDN_D void bases(float nm, float* r, float* g, float* b) {
+    *r = smooth_rise(nm, 575.0f, 605.0f);
+    *b = 1 - smooth_rise(nm, 445.0f, 530.0f);
+    *g = (1 - *b) * (1 - *r);
+}
+// spectrumOf (colour.cpp:88-103)
+KDN_D void spectrum_of(const float rgb[3], const Lanes& w, float out[kLanes]) {
+    int least = (rgb[0] <= rgb[1] && rgb[0] <= rgb[2]) ? 0 : ((rgb[1] <= rgb[0] && rgb[1] <= rgb[2]) ? 1 : 2);
+    const int first = least == 0 ? 1 : 0, second = least == 2 ? 1 : 2;
+    const int most = rgb[first] <= rgb[second] ? second : first;
+    const int mid = 3 - least - most;
+    const float base = rgb[least], low = rgb[mid] - base, high = rgb[most] - rgb[mid];
+    for (int lane = 0; lane < kLanes; ++lane) {
+        float bb[3];
+        bases(w.nm[lane], &bb[0], &bb[1], &bb[2]);
+        out[lane] = base + low * (1 - bb[least]) + high * bb[most];
+    }
+}
+KDN_D float table_at(const float* curve, float nm) {
+    const float slot = (nm - 380.0f) / 5.0f;
+    int below = (int)slot;
+    below = below < 0 ? 0 : (below > kTableSize - 2 ? kTableSize - 2 : below);
+    float blend = slot - below;
+    blend = blend < 0 ? 0.0f : (blend > 1 ? 1.0f : blend);
+    return curve[below] + blend * (curve[below + 1] - curve[below]);
+}
+// colourOf (colour.cpp:213-234): lanes → XYZ → linear sRGB over white
+KDN_D void colour_of(const Lanes& w, const float s[kLanes], const ColourTables* t, double rgb[3]) {
+    double x = 0, y = 0, z = 0;
+    for (int lane = 0; lane < kLanes; ++lane) {
+        if (w.density[lane] <= 0) continue;
+        const double weight = (double)s[lane] / w.density[lane];
+        x += table_at(t->x, w.nm[lane]) * weight;
+        y += table_at(t->y, w.nm[lane]) * weight;
+        z += table_at(t->z, w.nm[lane]) * weight;
+    }
+    x /= kLanes;
+    y /= kLanes;
+    z /= kLanes;
+    rgb[0] = (3.2406 * x - 1.5372 * y - 0.4986 * z) / t->white[0];
+    rgb[1] = (-0.9689 * x + 1.8758 * y + 0.0415 * z) / t->white[1];
+    rgb[2] = (0.0557 * x - 0.2040 * y + 1.0570 * z) / t->white[2];
+}
+
+// ---- vectors -------------------------------------------------------------------
+struct V3 {
+    float x, y, z;
+};
+KDN_D V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+KDN_D V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+KDN_D V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+KDN_D V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+KDN_D V3 operator-(V3 a) { return v3(-a.x, -a.y, -a.z); }
+KDN_D float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+KDN_D V3 cross(V3 a, V3 b) { return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+KDN_D float length(V3 a) { return __builtin_sqrtf(dot(a, a)); }
+KDN_D V3 normalized(V3 a) { return a * (1.0f / length(a)); }
+KDN_D V3 mirrored(V3 out, V3 n) { return n * (2 * dot(out, n)) - out; }
+
+// An orthonormal basis around n (Duff et al. 2017, branchless).
+struct Basis {
+    V3 t, b, n;
+};
+KDN_D Basis basis_around(V3 n) {
+    const float sign = n.z >= 0 ? 1.0f : -1.0f;
+    const float a = -1.0f / (sign + n.z);
+    const float b = n.x * n.y * a;
+    Basis o;
+    o.t = v3(1 + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    o.b = v3(b, sign + n.y * n.y * a, -n.y);
+    o.n = n;
+    return o;
+}
+KDN_D V3 to_world(const Basis& B, V3 v) { return B.t * v.x + B.b * v.y + B.n * v.z; }
+KDN_D V3 to_local(const Basis& B, V3 v) { return v3(dot(v, B.t), dot(v, B.b), dot(v, B.n)); }
+
+KDN_D V3 cosine_hemisphere(const Basis& B, float u, float v) {
+    const float reach = __builtin_sqrtf(u);
+    const float angle = 2 * kPi * v;
+    const float height = __builtin_sqrtf(fmax_(1 - u, 0.0f));
+    return to_world(B, v3(reach * __builtin_cosf(angle), reach * __builtin_sinf(angle), height));
+}
+
+// ---- microfacets (microfacet.cpp) --------------------------------------------
+KDN_D float facet_distribution(float cosine, float width) {
+    if (cosine <= 0) return 0.0f;
+    const float sq = width * width;
+    const float lower = cosine * cosine * (sq - 1) + 1;
+    return sq / (kPi * lower * lower);
+}
+KDN_D float smith_visibility(float leaving, float arriving, float width) {
+    const float sq = width * width;
+    const float first = arriving * __builtin_sqrtf(sq + (1 - sq) * leaving * leaving);
+    const float second = leaving * __builtin_sqrtf(sq + (1 - sq) * arriving * arriving);
+    const float total = first + second;
+    return total > 0 ? 0.5f / total : 0.0f;
+}
+KDN_D float smith_masking(float cosine, float width) {
+    const float sq = width * width;
+    const float under = cosine + __builtin_sqrtf(sq + (1 - sq) * cosine * cosine);
+    return under > 0 ? 2 * cosine / under : 0.0f;
+}
+KDN_D V3 halfway_of(V3 out, V3 in, V3 n) {
+    const V3 total = out + in;
+    const float size = length(total);
+    return size > 1e-12f ? total * (1.0f / size) : n;
+}
+KDN_D V3 raised_facet(V3 s, float u, float v) {
+    const float flat = s.x * s.x + s.y * s.y;
+    const V3 first = flat > 1e-12f ? v3(-s.y, s.x, 0) * (1.0f / __builtin_sqrtf(flat)) : v3(1, 0, 0);
+    const V3 second = cross(s, first);
+    const float reach = __builtin_sqrtf(u);
+    const float angle = 2 * kPi * v;
+    const float across = reach * __builtin_cosf(angle);
+    const float lean = 0.5f * (1 + s.z);
+    const float along = (1 - lean) * __builtin_sqrtf(fmax_(1 - across * across, 0.0f)) + lean * reach * __builtin_sinf(angle);
+    const float height = __builtin_sqrtf(fmax_(1 - across * across - along * along, 0.0f));
+    return first * across + second * along + s * height;
+}
+// visibleFacet (microfacet.cpp:54-62)
+KDN_D V3 visible_facet(V3 n, V3 out, float width, float u, float v) {
+    const Basis B = basis_around(n);
+    const V3 local = to_local(B, out);
+    const V3 stretched = normalized(v3(width * local.x, width * local.y, local.z));
+    const V3 raised = raised_facet(stretched, u, v);
+    const V3 un = v3(width * raised.x, width * raised.y, fmax_(raised.z, 1e-12f));
+    return to_world(B, normalized(un));
+}
+KDN_D float dielectric_reflectance(float cosine, float ratio) {
+    const float leftover = 1 - ratio * ratio * (1 - cosine * cosine);
+    if (leftover <= 0) return 1.0f;
+    const float onward = __builtin_sqrtf(leftover);
+    const float perp = (ratio * cosine - onward) / (ratio * cosine + onward);
+    const float para = (ratio * onward - cosine) / (ratio * onward + cosine);
+    return 0.5f * (perp * perp + para * para);
+}
+
+// ---- materials ---------------------------------------------------------------
+KDN_D float escape_at(const MaterialDev& m, float cosine) {
+    const int last = kEnergyViews - 1;
+    const float place = (cosine < 0 ? 0.0f : (cosine > 1 ? 1.0f : cosine)) * kEnergyViews - 0.5f;
+    int lower = (int)__builtin_floorf(place);
+    lower = lower < 0 ? 0 : (lower > last ? last : lower);
+    const int upper = lower + 1 < last ? lower + 1 : last;
+    float share = place - lower;
+    share = share < 0 ? 0.0f : (share > 1 ? 1.0f : share);
+    return m.escape[lower] * (1 - share) + m.escape[upper] * share;
+}
+
+// f(out, in) per lane for matte and metal; in and out unit, n faces out.
+KDN_D void rough_value(const MaterialDev& m, const Lanes& w, V3 n, V3 out, V3 in, float f[kLanes]) {
+    const float leaving = dot(n, out), arriving = dot(n, in);
+    for (int l = 0; l < kLanes; ++l) f[l] = 0;
+    if (leaving <= 0 || arriving <= 0) return;
+    if (m.kind == KDN_MATTE || m.kind == KDN_PLASTIC) {
+        const float a[3] = {m.albedo[0] / kPi, m.albedo[1] / kPi, m.albedo[2] / kPi};
+        spectrum_of(a, w, f);
+        return;
+    }
+    // metal: Schlick facets plus the energy lost to multiple bounces
+    // (material.cpp:592-660)
+    const V3 h = halfway_of(out, in, n);
+    const float onto = fmax_(dot(out, h), 0.0f);
+    const float facets = facet_distribution(dot(n, h), m.width) * smith_visibility(leaving, arriving, m.width);
+    const float rest = __builtin_powf(fmin_(fmax_(1 - onto, 0.0f), 1.0f), 5.0f);
+    float schlick[3], avg[3];
+    for (int k = 0; k < 3; ++k) {
+        schlick[k] = m.albedo[k] + (1 - m.albedo[k]) * rest;
+        avg[k] = m.albedo[k] + (1 - m.albedo[k]) * (1.0f / 21);
+    }
+    float face[kLanes], flat[kLanes];
+    spectrum_of(schlick, w, face);
+    spectrum_of(avg, w, flat);
+    const float room = 1 - m.escape_avg;
+    const float lost = room <= 1e-6f ? 0.0f : (1 - escape_at(m, leaving)) * (1 - escape_at(m, arriving)) / (kPi * room);
+    for (int l = 0; l < kLanes; ++l) {
+        const float twice = flat[l] * flat[l] * m.escape_avg / (1 - flat[l] * (1 - m.escape_avg));
+        f[l] = face[l] * facets + twice * lost;
+    }
+}
+
+KDN_D float rough_density(const MaterialDev& m, V3 n, V3 out, V3 in) {
+    const float leaving = dot(n, out), arriving = dot(n, in);
+    if (leaving <= 0 || arriving <= 0) return 0.0f;
+    const float cosine = arriving / kPi;
+    if (m.kind != KDN_METAL) return cosine;
+    const V3 h = halfway_of(out, in, n);
+    float facets = 0;
+    if (dot(out, h) > 0) facets = facet_distribution(dot(n, h), m.width) * smith_masking(leaving, m.width) / (4 * leaving);
+    return m.facet_share * facets + (1 - m.facet_share) * cosine;
+}
+
+struct Scatter {
+    bool found;
+    bool through;    // passes into or out of the solid
+    bool exact;      // a delta lobe (no density)
+    V3 in;
+    float weight[kLanes];
+    float density;
+    bool collapse;   // the passage bends the lanes apart: keep the hero
+};
+
+// Per-lane indices of the two sides (material.cpp:329-339 with the
+// dispersion feature, dispersion.cpp:44-73).
+KDN_D void sides_of(const MaterialDev& m, const Lanes& w, bool inside, float travelling[kLanes], float beyond[kLanes], bool* parted) {
+    *parted = m.spread != 0;
+    for (int l = 0; l < kLanes; ++l) {
+        const float um = w.nm[l] * 1e-3f;
+        const float n = m.spread != 0 ? m.index + m.spread / (um * um) : m.index;
+        travelling[l] = inside ? n : 1.0f;
+        beyond[l] = inside ? 1.0f : n;
+    }
+}
+
+// exactGlassScatter (material.cpp:1296-1325)
+KDN_D Scatter glass_scatter(const MaterialDev& m, const Lanes& w, V3 n, V3 out, bool inside, float uSide) {
+    Scatter s;
+    s.found = false;
+    s.exact = true;
+    s.density = 0;
+    s.collapse = false;
+    const float leaving = dot(n, out);
+    if (leaving <= 0) return s;
+    float tr[kLanes], be[kLanes];
+    bool parted;
+    sides_of(m, w, inside, tr, be, &parted);
+    float R[kLanes];
+    for (int l = 0; l < kLanes; ++l) R[l] = dielectric_reflectance(leaving, tr[l] / be[l]);
+    if (!parted) R[1] = R[2] = R[3] = R[0];
+    const float chance = parted ? 0.25f * (R[0] + R[1] + R[2] + R[3]) : R[0];
+    const float ratio = tr[0] / be[0];
+    const float leftover = 1 - ratio * ratio * (1 - leaving * leaving);
+    const bool passed = leftover > 0;
+    s.found = true;
+    if (uSide < chance || !passed) {
+        s.in = mirrored(out, n);
+        s.through = false;
+        const float c = passed ? chance : 1.0f;
+        for (int l = 0; l < kLanes; ++l) s.weight[l] = R[l] / c;
+        return s;
+    }
+    s.in = (n * leaving - out) * ratio - n * __builtin_sqrtf(leftover);
+    s.through = true;
+    for (int l = 0; l < kLanes; ++l) s.weight[l] = (1 - R[l]) / (1 - chance);
+    s.collapse = parted && ratio != 1.0f;
+    return s;
+}
+
+KDN_D Scatter rough_scatter(const MaterialDev& m, const Lanes& w, V3 n, V3 out, float uLobe, float u, float v) {
+    Scatter s;
+    s.found = false;
+    s.exact = false;
+    s.through = false;
+    s.collapse = false;
+    const float leaving = dot(n, out);
+    if (leaving <= 0) return s;
+    V3 in;
+    if (m.kind != KDN_METAL) {
+        in = cosine_hemisphere(basis_around(n), u, v);
+    } else if (uLobe < m.facet_share) {
+        const V3 facet = visible_facet(n, out, m.width, u, v);
+        if (dot(out, facet) <= 0) return s;
+        in = mirrored(out, facet);
+    } else {
+        in = cosine_hemisphere(basis_around(n), u, v);
+    }
+    const float arriving = dot(n, in);
+    if (arriving <= 0) return s;
+    const float density = rough_density(m, n, out, in);
+    if (!(density > 0)) return s;
+    float f[kLanes];
+    rough_value(m, w, n, out, in, f);
+    s.found = true;
+    s.in = in;
+    s.density = density;
+    for (int l = 0; l < kLanes; ++l) s.weight[l] = f[l] * (arriving / density);
+    return s;
+}
+
+// ---- lamps and sky -----------------------------------------------------------------
+struct LampHit {
+    float t, cosine;
+};
+// Where a ray meets a quad lamp from its lit side (meetQuad, light.cpp:458-480).
+KDN_D bool meet_quad(const LampDev& L, V3 o, V3 d, float tMax, LampHit* h) {
+    const V3 facing = v3(L.facing[0], L.facing[1], L.facing[2]);
+    const float toward = -dot(facing, d);
+    if (!(toward > 0)) return false;
+    const V3 at = v3(L.at[0], L.at[1], L.at[2]);
+    const float t = dot(at - o, facing) / dot(d, facing);
+    if (!(t > 0) || !(t < tMax)) return false;
+    const V3 p = o + d * t - at;
+    const V3 a = v3(L.along[0], L.along[1], L.along[2]), c = v3(L.across[0], L.across[1], L.across[2]);
+    // solve p = a s + c r on the plane
+    const float aa = dot(a, a), ac = dot(a, c), cc = dot(c, c), pa = dot(p, a), pc = dot(p, c);
+    const float det = aa * cc - ac * ac;
+    const float sA = (pa * cc - pc * ac) / det, sC = (pc * aa - pa * ac) / det;
+    if (sA < 0 || sA > 1 || sC < 0 || sC > 1) return false;
+    h->t = t;
+    h->cosine = toward;
+    return true;
+}
+
+KDN_D void sky_radiance(const float* zenith, const float* horizon, const float* ground, V3 d, const Lanes& w, float out[kLanes]) {
+    const float height = d.z;
+    const float* far = height >= 0 ? zenith : ground;
+    const float share = __builtin_sqrtf(__builtin_fabsf(height));
+    float c[3];
+    for (int k = 0; k < 3; ++k) c[k] = horizon[k] * (1 - share) + far[k] * share;
+    spectrum_of(c, w, out);
+}
+
+KDN_D float power_weight(float a, float b) {
+    const float a2 = a * a, b2 = b * b;
+    return a2 + b2 > 0 ? a2 / (a2 + b2) : 0.0f;
+}
+
+}  // namespace kdn

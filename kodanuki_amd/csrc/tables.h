@@ -1,0 +1,53 @@
+// Host-built tables the integrator reads from HBM.
+#pragma once
+
+#include <cstdint>
+
+#include "kodanuki.h"
+
+namespace kdn {
+
+constexpr int kTableSize = 65;        // CIE table entries, 380..700 nm by 5 nm
+constexpr int kWavelengthCuts = 256;  // importance cuts of the wavelength draw
+constexpr int kEnergyViews = 32;      // escape table entries of a GGX finish
+
+// The colour-matching tables and the wavelength importance cuts, as the
+// reference builds them (source/math/colour.cpp:134-211).
+struct ColourTables {
+    float x[kTableSize], y[kTableSize], z[kTableSize];
+    float white[3];
+    float cuts[kWavelengthCuts + 1];
+};
+
+// A material as the integrator reads it: the scene's material with its
+// GGX width, facet share and escape table (material.cpp:175-209,
+// energy.cpp:100-127).
+struct MaterialDev {
+    uint32_t kind;
+    uint32_t exact;        // smooth glass: a true mirror and passage
+    float albedo[3];
+    float width;           // GGX alpha
+    float facet_share;     // the chance a rough draw follows the facets
+    float escape_avg;
+    float escape[kEnergyViews];
+    float index;           // glass: Cauchy A
+    float spread;          // glass: Cauchy B (um^2), 0 for none
+    float glow[3];         // glow * strength
+};
+
+// A quad lamp as the integrator reads it (light.cpp:32-110).
+struct LampDev {
+    float at[3], along[3], across[3];
+    float facing[3];
+    float radiance[3];
+    float area;
+    float pick;            // the chance NEE picks it
+    float pick_cdf;        // cumulative pick chance up to and with it
+    uint32_t kind;
+};
+
+void buildColourTables(ColourTables& t);
+void escapeTable(double width, float table[kEnergyViews], float* average);
+double facetWidthOf(double roughness);
+
+}  // namespace kdn

@@ -163,3 +163,49 @@ def sphere_trace64(arrays, lower, size, rays):
     d64 = np.zeros(len(rays), np.float64)
     lib().or_sphere_trace64(C.byref(s), ptr(rays), len(rays), ptr(hits), ptr(d64))
     return hits, d64
+
+
+class RenderScene(C.Structure):
+    _fields_ = [("geometry", Scene64), ("materials", P), ("material_count", C.c_uint32), ("lights", P), ("light_count", C.c_uint32),
+                ("camera", k.Camera), ("sky_zenith", C.c_float * 3), ("sky_horizon", C.c_float * 3), ("sky_ground", C.c_float * 3)]
+
+
+def render64(scene, arrays, lower, size, width, height, spp, first=0, seed=1, surfaces=7, min_depth=4, window=None, threads=None):
+    """The FP64 path tracer over a window of the image: rgb sums (h, w, 3)."""
+    L = lib()
+    L.or_render64.argtypes = [C.POINTER(RenderScene), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32,
+                              C.c_uint32, C.c_uint32, C.c_uint32, P, C.c_int]
+    x0, y0, x1, y1 = window or (0, 0, width, height)
+    mats = scene.materials()
+    grey = k.Material()
+    grey.kind = 0
+    grey.albedo[:] = [0.5, 0.5, 0.5]
+    grey.roughness = 1
+    grey.strength = 1
+    mats = mats + [grey]
+    marr = (k.Material * len(mats))(*mats)
+    lights = scene.lights()
+    larr = (k.Light * max(1, len(lights)))(*lights)
+    cam, rs = scene.settings()
+    R = RenderScene()
+    g = R.geometry
+    g.steps = ptr(arrays.steps).value
+    g.values = ptr(arrays.values).value
+    g.objects = ptr(arrays.objects).value
+    g.object_count = len(arrays.objects)
+    g.leaf_material = ptr(arrays.leaf_material).value
+    for i in range(3):
+        g.lower[i] = float(lower[i])
+        g.size[i] = float(size[i])
+    R.materials = C.cast(marr, P).value
+    R.material_count = len(mats)
+    R.lights = C.cast(larr, P).value
+    R.light_count = len(lights)
+    R.camera = cam
+    R.sky_zenith[:] = rs.sky_zenith[:]
+    R.sky_horizon[:] = rs.sky_horizon[:]
+    R.sky_ground[:] = rs.sky_ground[:]
+    rgb = np.zeros((y1 - y0, x1 - x0, 3), np.float64)
+    L.or_render64(C.byref(R), width, height, x0, y0, x1, y1, first, spp, seed, surfaces, min_depth, ptr(rgb),
+                  threads or min(16, os.cpu_count() or 1))
+    return rgb
