@@ -304,7 +304,10 @@ typedef struct kdn_pass_desc {
     uint32_t seed;
     uint32_t surfaces;      /* the most surfaces a path stands on */
     uint32_t min_depth;     /* roulette starts at this depth (reference 4) */
-    uint32_t flags;         /* bit 0: keep the accumulators (progressive) */
+    uint32_t flags;         /* bit 0: keep the accumulators (progressive);
+                               bit 1: time the kernels with HIP events;
+                               bit 2: count the march's FP32 operations
+                                      (an instrumented, slower pass) */
 } kdn_pass_desc;
 
 typedef struct kdn_pass_stats {
@@ -312,11 +315,18 @@ typedef struct kdn_pass_stats {
     uint64_t segments;       /* extend rays traced */
     uint64_t shadow_rays;
     uint32_t bounces;        /* wavefront iterations */
+    uint32_t trace_launches; /* extend launches (shadow launches apart) */
+    double trace_ms;         /* extend kernels, HIP events (flags bit 1) */
+    double shadow_ms;        /* shadow-ray kernels */
+    double shade_ms;         /* shading kernels */
+    double other_ms;         /* generate and fold */
+    double trace_flops;      /* FP32 operations of the extend march (bit 2) */
+    double shadow_flops;
 } kdn_pass_stats;
 
 int kdn_render_pass(kdn_ctx* ctx, const kdn_pass_desc* desc, kdn_pass_stats* stats);
-/* Read the FP64 accumulators: per pixel sum of colour (3 doubles), sum of
- * squares of the luminance, and the sample count. */
+/* Read the FP64 accumulators: per pixel the sum of linear RGB (3 doubles),
+ * the sum of its squares per channel (3 doubles) and the sample count. */
 int kdn_read_accumulation(kdn_ctx* ctx, double* rgb_sum, double* sq_sum, uint32_t* count);
 /* The mean image as linear RGB floats (width*height*3). */
 int kdn_read_image(kdn_ctx* ctx, float* rgb);

@@ -98,7 +98,12 @@ class PassDesc(C.Structure):
 
 
 class PassStats(C.Structure):
-    _fields_ = [("ms", C.c_double), ("segments", C.c_uint64), ("shadow_rays", C.c_uint64), ("bounces", C.c_uint32)]
+    _fields_ = [("ms", C.c_double), ("segments", C.c_uint64), ("shadow_rays", C.c_uint64), ("bounces", C.c_uint32),
+                ("trace_launches", C.c_uint32), ("trace_ms", C.c_double), ("shadow_ms", C.c_double), ("shade_ms", C.c_double),
+                ("other_ms", C.c_double), ("trace_flops", C.c_double), ("shadow_flops", C.c_double)]
+
+
+PASS_KEEP, PASS_TIME, PASS_COUNT = 1, 2, 4
 
 
 # numpy views of the structs
@@ -320,15 +325,15 @@ class Device:
         _check(library().kdn_extend_device(self._h, C.c_void_p(rays_ptr), count, C.c_void_p(hits_ptr),
                                            C.c_void_p(stream) if stream else None), self._h)
 
-    def render_pass(self, width, height, spp, first_sample=0, seed=1, surfaces=7, min_depth=4, keep=False):
-        d = PassDesc(width, height, first_sample, spp, seed, surfaces, min_depth, 1 if keep else 0)
+    def render_pass(self, width, height, spp, first_sample=0, seed=1, surfaces=7, min_depth=4, keep=False, flags=0):
+        d = PassDesc(width, height, first_sample, spp, seed, surfaces, min_depth, (PASS_KEEP if keep else 0) | flags)
         st = PassStats()
         _check(library().kdn_render_pass(self._h, C.byref(d), C.byref(st)), self._h)
         return st
 
     def accumulation(self, width, height):
         rgb = np.zeros((height, width, 3), np.float64)
-        sq = np.zeros((height, width), np.float64)
+        sq = np.zeros((height, width, 3), np.float64)
         n = np.zeros((height, width), np.uint32)
         _check(library().kdn_read_accumulation(self._h, _ptr(rgb), _ptr(sq), _ptr(n)), self._h)
         return rgb, sq, n

@@ -625,7 +625,7 @@ int kdn_read_accumulation(kdn_ctx* ctx, double* rgb_sum, double* sq_sum, uint32_
     const size_t px = (size_t)c.width * c.height;
     hipStreamSynchronize(c.stream);
     if ((rgb_sum && hipMemcpy(rgb_sum, c.accum, px * 3 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) ||
-        (sq_sum && hipMemcpy(sq_sum, c.accum_sq, px * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) ||
+        (sq_sum && hipMemcpy(sq_sum, c.accum_sq, px * 3 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) ||
         (count && hipMemcpy(count, c.accum_n, px * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)) {
         c.error = "kdn_read_accumulation: copy failed";
         return ctxFail(ctx, KDN_E_DEVICE);

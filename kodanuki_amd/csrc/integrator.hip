@@ -137,6 +137,23 @@ __global__ void __launch_bounds__(256) k_trace_queue(TreeView tv, const uint32_t
     hits[i] = h;
 }
 
+// The same march, adding the FP32 operations of every evaluation to *work
+// (the instrumented pass the roofline's algorithmic work comes from).
+__global__ void __launch_bounds__(256) k_trace_queue_count(TreeView tv, const uint32_t* queue, const uint32_t* count,
+                                                           const kdn_ray* rays, kdn_hit* hits, double* work) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= *count) return;
+    const uint32_t i = queue[q];
+    Dl buf[kMaxRegs];
+    kdn_hit h;
+    double w = 0;
+    TreeView counted = tv;
+    counted.work = &w;
+    trace_ray(counted, rays[i], buf, h);
+    hits[i] = h;
+    atomicAdd(work, w);
+}
+
 KDN_D float lift_at(V3 x) {
     return 8e-5f + 8e-6f * fmax_(fmax_(__builtin_fabsf(x.x), __builtin_fabsf(x.y)), __builtin_fabsf(x.z));
 }
@@ -312,7 +329,11 @@ __global__ void __launch_bounds__(256) k_shadow(TreeView tv, PathState s) {
     const kdn_ray r = s.shadow[i];
     Dl buf[kMaxRegs];
     kdn_hit h;
-    trace_ray(tv, r, buf, h);
+    TreeView t = tv;
+    double w = 0;
+    if (tv.work) t.work = &w;
+    trace_ray(t, r, buf, h);
+    if (tv.work) atomicAdd(tv.work, w);
     if (h.kind == KDN_HIT_MISSED || !(h.distance < r.far_)) {
         const size_t N = s.capacity;
         for (int l = 0; l < kLanes; ++l) s.rad[l * N + i] += s.srad[l * N + i];
@@ -502,15 +523,44 @@ bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
     P.seed_bits = mixed_bits_host(d.seed);
     const TreeView tv = treeView(c);
 
+    const bool timing = (d.flags & 2) != 0, counting = (d.flags & 4) != 0;
+    // kernel timing: an event pair per launch, read after each bounce's sync
+    hipEvent_t ev[2];
+    if (timing) {
+        KDN_HIP(hipEventCreate(&ev[0]));
+        KDN_HIP(hipEventCreate(&ev[1]));
+    }
+    double traceMs = 0, shadeMs = 0, shadowMs = 0, otherMs = 0;
+    auto timed = [&](double* acc, auto&& launch) -> bool {
+        if (timing) KDN_HIP(hipEventRecord(ev[0], st));
+        launch();
+        if (timing) {
+            KDN_HIP(hipEventRecord(ev[1], st));
+            KDN_HIP(hipEventSynchronize(ev[1]));
+            float ms = 0;
+            KDN_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            *acc += ms;
+        }
+        return true;
+    };
+    double* work = nullptr;
+    if (counting) {
+        if (!ensure(c.scratch[7], 64, c.error, false)) return false;
+        work = (double*)c.scratch[7].ptr;
+        KDN_HIP(hipMemsetAsync(work, 0, 16, st));
+    }
+    TreeView tvShadow = tv;
+    tvShadow.work = counting ? work + 1 : nullptr;
+
     uint64_t segments = 0, shadows = 0;
-    uint32_t bounces = 0;
+    uint32_t bounces = 0, launches = 0;
     for (uint32_t first = 0; first < d.spp; first += chunk) {
         const uint32_t n = std::min(chunk, d.spp - first);
         P.first_sample = d.first_sample + first;
         P.chunk = n;
         const uint32_t slots = plane * n;
         KDN_HIP(hipMemsetAsync(s->counters, 0, 64, st));
-        hipLaunchKernelGGL(k_generate, dim3((slots + 255) / 256), dim3(256), 0, st, P, slots);
+        timed(&otherMs, [&] { hipLaunchKernelGGL(k_generate, dim3((slots + 255) / 256), dim3(256), 0, st, P, slots); });
         uint32_t in = 0, out = 1;
         for (uint32_t bounce = 0; bounce < d.surfaces; ++bounce) {
             uint32_t count = 0;
@@ -519,27 +569,51 @@ bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
             if (count == 0) break;
             segments += count;
             ++bounces;
+            ++launches;
             const uint32_t blocks = (count + 255) / 256;
-            hipLaunchKernelGGL(k_trace_queue, dim3(blocks), dim3(256), 0, st, tv, s->queue[in], &s->counters[in], s->rays, s->hits);
+            timed(&traceMs, [&] {
+                if (counting) {
+                    hipLaunchKernelGGL(k_trace_queue_count, dim3(blocks), dim3(256), 0, st, tv, s->queue[in], &s->counters[in], s->rays,
+                                       s->hits, work);
+                } else {
+                    hipLaunchKernelGGL(k_trace_queue, dim3(blocks), dim3(256), 0, st, tv, s->queue[in], &s->counters[in], s->rays, s->hits);
+                }
+            });
             KDN_HIP(hipMemsetAsync(&s->counters[out], 0, 4, st));
             KDN_HIP(hipMemsetAsync(&s->counters[2], 0, 4, st));
-            hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, st, P, in, out);
+            timed(&shadeMs, [&] { hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, st, P, in, out); });
             uint32_t sc = 0;
             KDN_HIP(hipMemcpyAsync(&sc, &s->counters[2], 4, hipMemcpyDeviceToHost, st));
             KDN_HIP(hipStreamSynchronize(st));
             shadows += sc;
-            if (sc) hipLaunchKernelGGL(k_shadow, dim3((sc + 255) / 256), dim3(256), 0, st, tv, *s);
+            if (sc) timed(&shadowMs, [&] { hipLaunchKernelGGL(k_shadow, dim3((sc + 255) / 256), dim3(256), 0, st, tvShadow, *s); });
             std::swap(in, out);
         }
-        hipLaunchKernelGGL(k_fold, dim3((plane + 255) / 256), dim3(256), 0, st, P, c.accum, c.accum_sq, c.accum_n);
+        timed(&otherMs, [&] {
+            hipLaunchKernelGGL(k_fold, dim3((plane + 255) / 256), dim3(256), 0, st, P, c.accum, c.accum_sq, c.accum_n);
+        });
     }
     KDN_HIP(hipStreamSynchronize(st));
     KDN_HIP(hipGetLastError());
+    double counted[2] = {0, 0};
+    if (counting) KDN_HIP(hipMemcpy(counted, work, 16, hipMemcpyDeviceToHost));
+    if (timing) {
+        KDN_HIP(hipEventDestroy(ev[0]));
+        KDN_HIP(hipEventDestroy(ev[1]));
+    }
     if (stats) {
+        *stats = kdn_pass_stats{};
         stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
         stats->segments = segments;
         stats->shadow_rays = shadows;
         stats->bounces = bounces;
+        stats->trace_launches = launches;
+        stats->trace_ms = traceMs;
+        stats->shade_ms = shadeMs;
+        stats->shadow_ms = shadowMs;
+        stats->other_ms = otherMs;
+        stats->trace_flops = counted[0];
+        stats->shadow_flops = counted[1];
     }
     return true;
 }

@@ -233,7 +233,7 @@ __global__ void __launch_bounds__(256) k_emit(const ActEntry* act, uint32_t nIte
     const Strided<kdn_step> ST{const_cast<kdn_step*>(stScratch) + base, octants};
     const Strided<uint32_t> MAP{alScratch + base, octants};
     // compactedProgram (program.cpp:358-380): marks become new indices
-    uint32_t kept = 0;
+    uint32_t kept = 0, flops = 0;
     kdn_step* dst = pool + at + 1;
     for (uint32_t i = 0; i < L; ++i) {
         if (!MAP[i]) continue;
@@ -243,6 +243,7 @@ __global__ void __launch_bounds__(256) k_emit(const ActEntry* act, uint32_t nIte
         s.b = count > 1 ? (uint16_t)MAP[s.b] : 0;
         s.c = count > 2 ? (uint16_t)MAP[s.c] : 0;
         dst[kept] = s;
+        flops += step_flops(s.op);
         MAP[i] = kept++;
     }
     kdn_step h;
@@ -251,7 +252,7 @@ __global__ void __launch_bounds__(256) k_emit(const ActEntry* act, uint32_t nIte
     h.a = (uint16_t)(kept & 0xFFFF);
     h.b = (uint16_t)(kept >> 16);
     h.c = (uint16_t)MAP[it.result];
-    h.value = 0;
+    h.value = (float)flops;  // the program's point-evaluation work
     h.leaf = entry.object;
     pool[at] = h;
 }
@@ -292,10 +293,14 @@ __global__ void k_root_entries(uint32_t nObjects, const kdn_object* objects, uin
     h.a = (uint16_t)(ob.count & 0xFFFF);
     h.b = (uint16_t)(ob.count >> 16);
     h.c = (uint16_t)ob.result;
-    h.value = 0;
+    uint32_t flops = 0;
+    for (uint32_t k = 0; k < ob.count; ++k) {
+        pool[at + 1 + k] = steps[ob.first + k];
+        flops += step_flops(steps[ob.first + k].op);
+    }
+    h.value = (float)flops;
     h.leaf = i;
     pool[at] = h;
-    for (uint32_t k = 0; k < ob.count; ++k) pool[at + 1 + k] = steps[ob.first + k];
     entries[i] = EntryDev{i, (uint32_t)at};
     act[i] = ActEntry{0, i};
     lengths[i] = ob.count;

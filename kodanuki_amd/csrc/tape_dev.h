@@ -175,6 +175,20 @@ KDN_D int operand_count(uint8_t op) {
 }
 KDN_D bool is_literal(uint8_t op) { return op == KDN_CONSTANT || op == KDN_ARGUMENT; }
 
+// FP32 operations one point evaluation of a step takes: the algorithmic
+// work the march's roofline counts (loads of constants and coordinates
+// are free, a smooth minimum is ten operations, a transcendental twenty).
+KDN_D uint32_t step_flops(uint8_t op) {
+    switch (op) {
+        case KDN_CONSTANT: case KDN_ARGUMENT: case KDN_POINT_X: case KDN_POINT_Y: case KDN_POINT_Z:
+        case KDN_NORMAL_X: case KDN_NORMAL_Y: case KDN_NORMAL_Z: return 0;
+        case KDN_SMOOTH_MINIMUM: return 10;
+        case KDN_POSITIVE_SQUARE: return 2;
+        case KDN_SINE: case KDN_COSINE: case KDN_ARC_TANGENT: case KDN_POWER: return 20;
+        default: return 1;
+    }
+}
+
 template <class IV>
 KDN_D uint32_t separated(const kdn_step& s, const IV& iv_, bool lowerWins) {
     const Iv a = iv_[s.a], b = iv_[s.b];

@@ -25,6 +25,7 @@ TreeView treeView(const Ctx& c) {
         tv.size[k] = c.tree.size[k];
     }
     tv.depth = c.tree.depth;
+    tv.work = nullptr;
     return tv;
 }
 

@@ -33,6 +33,7 @@ struct TreeView {
     float lower[3];
     float size[3];
     int depth;
+    double* work;  // when set, the FP32 operations of every evaluation are added here
 };
 
 struct Regs {
@@ -44,16 +45,20 @@ struct DRegs {
     KDN_D Dl& operator[](uint32_t i) const { return d[i]; }
 };
 
-KDN_D float eval_program(const kdn_step* pool, uint32_t at, const Regs& r, float px, float py, float pz) {
+KDN_D float eval_program(const TreeView& tv, uint32_t at, const Regs& r, float px, float py, float pz) {
+    const kdn_step* pool = tv.pool;
     const kdn_step h = pool[at];
+    if (tv.work) *tv.work += h.value;
     const uint32_t L = (uint32_t)h.a | ((uint32_t)h.b << 16);
     const kdn_step* prog = pool + at + 1;
     for (uint32_t i = 0; i < L; ++i) r[i] = point_step(prog[i], r, px, py, pz);
     return r[h.c];
 }
 
-KDN_D void eval_dual(const kdn_step* pool, uint32_t at, const DRegs& r, float px, float py, float pz, Dl* out) {
+KDN_D void eval_dual(const TreeView& tv, uint32_t at, const DRegs& r, float px, float py, float pz, Dl* out) {
+    const kdn_step* pool = tv.pool;
     const kdn_step h = pool[at];
+    if (tv.work) *tv.work += 4 * h.value;
     const uint32_t L = (uint32_t)h.a | ((uint32_t)h.b << 16);
     const kdn_step* prog = pool + at + 1;
     for (uint32_t i = 0; i < L; ++i) r[i] = dual_step(prog[i], r, px, py, pz);
@@ -69,7 +74,7 @@ struct LeafField {
 KDN_D LeafField leaf_field(const TreeView& tv, const NodeDev& n, const Regs& r, float px, float py, float pz) {
     LeafField f{kInf, kInf, 0};
     for (uint32_t j = 0; j < n.entry_count; ++j) {
-        const float v = eval_program(tv.pool, tv.entries[n.entry_first + j].program, r, px, py, pz);
+        const float v = eval_program(tv, tv.entries[n.entry_first + j].program, r, px, py, pz);
         if (v < f.least) {
             f.least = v;
             f.arg = j;
@@ -104,16 +109,27 @@ KDN_D void faced_normal(const float g[3], const float d[3], bool inside, float n
     n[0] = u[0]; n[1] = u[1]; n[2] = u[2];
 }
 
-// The hit at t from entry `winner` of leaf n.
+// The hit at t from entry `winner` of leaf n. The march stops within its
+// tolerance of the surface; one Newton step along the ray, t - v / (g.d),
+// bounded by the tolerance and taken only where the ray is not grazing,
+// brings the distance onto the surface.
 KDN_D void land(const TreeView& tv, const NodeDev& n, uint32_t nodeIndex, uint32_t winner, const float o[3], const float d[3], float t,
                 bool inside, Dl* buf, kdn_hit& hit) {
     const float px = o[0] + d[0] * t, py = o[1] + d[1] * t, pz = o[2] + d[2] * t;
     const uint32_t at = tv.entries[n.entry_first + winner].program;
     Dl value;
-    eval_dual(tv.pool, at, DRegs{buf}, px, py, pz, &value);
+    eval_dual(tv, at, DRegs{buf}, px, py, pz, &value);
+    const float slope = value.g[0] * d[0] + value.g[1] * d[1] + value.g[2] * d[2];
+    const float glen = __builtin_sqrtf(value.g[0] * value.g[0] + value.g[1] * value.g[1] + value.g[2] * value.g[2]);
+    const float bound = 2 * (kTolAbs + kTolRel * t);
+    float tr = t;
+    if (__builtin_fabsf(slope) > 0.1f * glen) {
+        const float dt = -value.v / slope;
+        tr = t + fmax_(-bound, fmin_(bound, dt));
+    }
     hit.kind = KDN_HIT_SURFACE;
-    hit.distance = t;
-    hit.point[0] = px; hit.point[1] = py; hit.point[2] = pz;
+    hit.distance = tr;
+    hit.point[0] = o[0] + d[0] * tr; hit.point[1] = o[1] + d[1] * tr; hit.point[2] = o[2] + d[2] * tr;
     faced_normal(value.g, d, inside, hit.normal);
     const kdn_step h = tv.pool[at];
     source_of(tv.pool + at + 1, h.c, Regs{buf}, &hit.leaf, &hit.other, &hit.share);
@@ -129,12 +145,12 @@ KDN_D uint32_t winner_at(const TreeView& tv, const NodeDev& n, const Regs& r, co
     const float band = kTie * (1 + vmax3(__builtin_fabsf(px), __builtin_fabsf(py), __builtin_fabsf(pz)));
     const float qx = o[0] + d[0] * (t + inset), qy = o[1] + d[1] * (t + inset), qz = o[2] + d[2] * (t + inset);
     uint32_t winner = f.arg;
-    float winnerOn = eval_program(tv.pool, tv.entries[n.entry_first + f.arg].program, r, qx, qy, qz);
+    float winnerOn = eval_program(tv, tv.entries[n.entry_first + f.arg].program, r, qx, qy, qz);
     for (uint32_t j = 0; j < n.entry_count; ++j) {
         const uint32_t at = tv.entries[n.entry_first + j].program;
-        const float v = eval_program(tv.pool, at, r, px, py, pz);
+        const float v = eval_program(tv, at, r, px, py, pz);
         if (!(v - f.least <= band)) continue;
-        const float on = eval_program(tv.pool, at, r, qx, qy, qz);
+        const float on = eval_program(tv, at, r, qx, qy, qz);
         if (on < winnerOn) {
             winner = j;
             winnerOn = on;
@@ -276,7 +292,7 @@ KDN_D void trace_ray(const TreeView& tv, const kdn_ray& ray, Dl* buf, kdn_hit& h
             hit.cell = index;
             if (n.entry_count > 0) {
                 const uint32_t at = tv.entries[n.entry_first].program;
-                const float v = eval_program(tv.pool, at, r, hit.point[0], hit.point[1], hit.point[2]);
+                const float v = eval_program(tv, at, r, hit.point[0], hit.point[1], hit.point[2]);
                 (void)v;
                 float sh;
                 source_of(tv.pool + at + 1, tv.pool[at].c, r, &hit.leaf, &hit.other, &sh);

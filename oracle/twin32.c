@@ -764,6 +764,21 @@ static uint64_t push_program(or_octree* t, const kdn_step* steps, uint32_t count
     h.b = (uint16_t)(count >> 16);
     h.c = (uint16_t)result;
     h.leaf = leaf;
+    /* the header's value: the FP32 operations of one point evaluation
+     * (constants and coordinates free, smooth minimum 10, positive square
+     * 2, transcendentals 20, every other step 1) */
+    uint32_t flops = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        switch (steps[i].op) {
+            case KDN_CONSTANT: case KDN_ARGUMENT: case KDN_POINT_X: case KDN_POINT_Y: case KDN_POINT_Z:
+            case KDN_NORMAL_X: case KDN_NORMAL_Y: case KDN_NORMAL_Z: break;
+            case KDN_SMOOTH_MINIMUM: flops += 10; break;
+            case KDN_POSITIVE_SQUARE: flops += 2; break;
+            case KDN_SINE: case KDN_COSINE: case KDN_ARC_TANGENT: case KDN_POWER: flops += 20; break;
+            default: flops += 1;
+        }
+    }
+    h.value = (float)flops;
     t->pool[at] = h;
     memcpy(t->pool + at + 1, steps, sizeof(kdn_step) * count);
     t->n_pool += count + 1;
@@ -1118,11 +1133,20 @@ static void land(const View* v, const kdn_node_record* n, uint32_t index, uint32
     const float px = o[0] + d[0] * t, py = o[1] + d[1] * t, pz = o[2] + d[2] * t;
     const uint32_t at = v->t->entries[n->entry_first + winner].program;
     const Dl value = eval_dual_prog(v, at, px, py, pz);
+    /* one Newton step along the ray onto the surface, bounded */
+    const float slope = value.g[0] * d[0] + value.g[1] * d[1] + value.g[2] * d[2];
+    const float glen = sqrtf(value.g[0] * value.g[0] + value.g[1] * value.g[1] + value.g[2] * value.g[2]);
+    const float bound = 2 * (kTolAbs + kTolRel * t);
+    float tr = t;
+    if (fabsf(slope) > 0.1f * glen) {
+        const float dt = -value.v / slope;
+        tr = t + fmax_(-bound, fmin_(bound, dt));
+    }
     hit->kind = KDN_HIT_SURFACE;
-    hit->distance = t;
-    hit->point[0] = px;
-    hit->point[1] = py;
-    hit->point[2] = pz;
+    hit->distance = tr;
+    hit->point[0] = o[0] + d[0] * tr;
+    hit->point[1] = o[1] + d[1] * tr;
+    hit->point[2] = o[2] + d[2] * tr;
     faced_normal(value.g, d, inside, hit->normal);
     source_of(&v->t->pool[at + 1], v->t->pool[at].c, v->buf, &hit->leaf, &hit->other, &hit->share);
     hit->cell = index;

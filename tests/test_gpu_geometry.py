@@ -109,13 +109,15 @@ def test_hits_against_fp64_sphere_trace(device, built, name, count):
     agree = np.mean((hits["kind"] != k.KDN_HIT_MISSED) == (ref["kind"] != k.KDN_HIT_MISSED))
     note(f"[fp64] {name}: hit/miss agreement {agree:.4f} over {len(rays)} rays")
     assert agree > 0.995
-    rel = np.abs(hits["distance"][both].astype(np.float64) - d64[both]) / np.maximum(d64[both], 1e-3)
-    # grazing hits stop further from the surface along the ray; judge the
-    # bulk and the near-normal ones
+    # relative to the distance, with a floor of 0.05 scene units below which
+    # no FP32 march resolves 1e-4 relative (the FP64 oracle itself stops up
+    # to 1e-6 short of the surface)
+    rel = np.abs(hits["distance"][both].astype(np.float64) - d64[both]) / np.maximum(d64[both], 0.05)
     d = rays["direction"][both].astype(np.float64)
     n = ref["normal"][both].astype(np.float64)
     facing = np.abs(np.sum(d * n, axis=1))
-    note(f"[fp64] {name}: median rel {np.median(rel):.2e} p99 {np.percentile(rel, 99):.2e}")
+    note(f"[fp64] {name}: median rel {np.median(rel):.2e} p99 {np.percentile(rel, 99):.2e} max {rel.max():.2e} "
+         f"max facing>0.2 {rel[facing > 0.2].max():.2e}")
     assert np.all(rel[facing > 0.2] < 1e-4)
     assert np.mean(rel < 1e-4) > 0.99
     same_leaf = np.mean(hits["leaf"][both] == ref["leaf"][both])
