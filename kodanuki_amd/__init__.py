@@ -140,7 +140,12 @@ def library():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise KodanukiError(f"{LIB_PATH} is not built: run __graft_entry__.build() (hipcc, gfx950)")
-    lib = C.CDLL(LIB_PATH)
+    return _bind(C.CDLL(LIB_PATH))
+
+
+def _bind(lib):
+    """Declare the C ABI's signatures on a loaded library and keep it."""
+    global _lib
     P = C.c_void_p
     sig = {
         "kdn_scene_load": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(P), C.c_char_p, C.c_size_t]),

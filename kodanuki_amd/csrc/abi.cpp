@@ -436,60 +436,8 @@ int kdn_upload_scene(kdn_ctx* ctx, const kdn_scene* s) {
         return ctxFail(ctx, KDN_E_MEMORY);
     }
     // the integrator's materials (the fallback last) and lamps
-    std::vector<kdn::MaterialDev> md;
-    for (const kdn_material& m : mats) {
-        kdn::MaterialDev x{};
-        x.kind = m.kind;
-        for (int k = 0; k < 3; ++k) {
-            x.albedo[k] = m.albedo[k];
-            x.glow[k] = m.glow[k] * m.strength;
-        }
-        const double width = kdn::facetWidthOf(m.kind == KDN_MATTE ? 1.0 : m.roughness);
-        x.width = (float)width;
-        if (m.kind == KDN_METAL) {
-            kdn::escapeTable(width, x.escape, &x.escape_avg);
-        } else {
-            for (float& e : x.escape) e = 1.0f;
-            x.escape_avg = 1.0f;
-        }
-        x.facet_share = std::min(std::max(x.escape_avg, 0.25f), 1.0f);
-        x.exact = m.kind == KDN_GLASS && m.roughness <= 0 ? 1u : 0u;
-        x.index = m.refraction;
-        x.spread = m.dispersion;
-        md.push_back(x);
-    }
-    std::vector<kdn::LampDev> lamps;
-    double totalPower = 0;
-    for (const kdn_light& l : lights) {
-        if (l.kind != KDN_LIGHT_QUAD) continue;
-        kdn::LampDev x{};
-        x.kind = l.kind;
-        double f[3] = {(double)l.along[1] * l.across[2] - (double)l.along[2] * l.across[1],
-                       (double)l.along[2] * l.across[0] - (double)l.along[0] * l.across[2],
-                       (double)l.along[0] * l.across[1] - (double)l.along[1] * l.across[0]};
-        const double area = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
-        if (!(area > 1e-12)) continue;
-        for (int k = 0; k < 3; ++k) {
-            x.at[k] = l.at[k];
-            x.along[k] = l.along[k];
-            x.across[k] = l.across[k];
-            x.facing[k] = (float)(f[k] / area);
-            x.radiance[k] = (float)(l.power[k] / area);  // radianceOf (light.cpp:32-36)
-        }
-        x.area = (float)area;
-        const double emitted = std::max((l.power[0] + l.power[1] + l.power[2]) / 3.0, 0.0) * M_PI;  // emittedOf
-        x.pick = (float)emitted;
-        totalPower += emitted;
-        lamps.push_back(x);
-    }
-    double cdf = 0;
-    for (kdn::LampDev& x : lamps) {
-        const double p = totalPower > 0 ? x.pick / totalPower : 1.0 / lamps.size();
-        x.pick = (float)p;
-        cdf += p;
-        x.pick_cdf = (float)cdf;
-    }
-    if (!lamps.empty()) lamps.back().pick_cdf = 1.0f;
+    const std::vector<kdn::MaterialDev> md = kdn::materialTable(mats.data(), mats.size());
+    const std::vector<kdn::LampDev> lamps = kdn::lampTable(lights.data(), lights.size());
     if (c.materials_dev) hipFree(c.materials_dev);
     if (c.lamps_dev) hipFree(c.lamps_dev);
     c.materials_dev = c.lamps_dev = nullptr;

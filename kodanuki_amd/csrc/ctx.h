@@ -9,20 +9,9 @@
 #include <vector>
 
 #include "kodanuki.h"
+#include "layout.h"
 
 namespace kdn {
-
-// An octree node as kept on the device, 32 bytes: the downloaded
-// kdn_node_record layout.
-struct NodeDev {
-    uint32_t level, x, y, z, kind, first_child, entry_first, entry_count;
-};
-
-enum NodeKind : uint32_t { KindEmpty = 0, KindFilled = 1, KindSurface = 2, KindBranch = 3 };
-
-struct EntryDev {
-    uint32_t object, program;
-};
 
 // A device buffer that grows by doubling and keeps its contents.
 struct DevBuf {

@@ -1,0 +1,345 @@
+// The integrator's per-item kernel bodies: one path slot, queue entry or
+// pixel each. The HIP kernels call them one lane per item; a CPU build of
+// this header (bounds-checked) runs the same code item by item.
+#pragma once
+
+#include "integrator_dev.h"
+
+namespace kdn {
+
+struct PathState {
+    kdn_ray* rays;
+    kdn_hit* hits;
+    kdn_ray* shadow;
+    float* srad;       // 4 lanes, lane-major
+    float* thr;        // 4 lanes
+    float* rad;        // 4 lanes
+    float* nm;         // 4 lanes
+    float* dens;       // 4 lanes
+    float* pdf;
+    uint32_t* meta;    // depth | flags << 8
+    uint32_t* pixel;
+    uint32_t* sample;
+    uint64_t* stream;
+    uint32_t* queue[3];
+    uint32_t* counters;
+    size_t capacity;
+};
+
+enum : uint32_t { FlagExact = 1u << 8, FlagDispersed = 1u << 9 };
+
+struct PassParams {
+    int width, height;
+    uint32_t pixels;
+    uint32_t first_sample, chunk;
+    uint64_t seed_bits;
+    uint32_t surfaces, min_depth;
+    float eye[3], forward[3], right[3], up[3];
+    float reach, aspect;
+    float sky_zenith[3], sky_horizon[3], sky_ground[3];
+    uint32_t n_lamps, n_materials, fallback_material;
+    const LampDev* lamps;
+    const MaterialDev* materials;
+    const uint32_t* leaf_material;
+    const ColourTables* colour;
+    PathState s;
+};
+
+KDN_D float lane(const float* a, uint32_t l, size_t i, size_t n) { return a[l * n + i]; }
+
+// Pixel of path slot i: the chunk's sample planes one after another, each
+// plane in 8x8 tiles so that a wavefront's camera rays are neighbours.
+KDN_D void pixel_of(const PassParams& P, uint32_t p, uint32_t* x, uint32_t* y) {
+    const uint32_t tilesX = (P.width + 7) / 8;
+    const uint32_t tile = p / 64, in = p % 64;
+    const uint32_t tx = tile % tilesX, ty = tile / tilesX;
+    *x = tx * 8 + (in % 8);
+    *y = ty * 8 + (in / 8);
+}
+
+// generate: the path of slot i (one per pixel and sample of the chunk)
+KDN_D void generate_one(PassParams& P, uint32_t i) {
+    PathState& s = P.s;
+    const size_t N = s.capacity;
+    const uint32_t tilesX = (P.width + 7) / 8, tilesY = (P.height + 7) / 8;
+    const uint32_t plane = tilesX * tilesY * 64;
+    const uint32_t sIdx = i / plane, p = i % plane;
+    uint32_t x, y;
+    pixel_of(P, p, &x, &y);
+    const bool live = x < (uint32_t)P.width && y < (uint32_t)P.height;
+    s.pixel[i] = live ? y * P.width + x : 0xFFFFFFFFu;
+    if (!live) {
+        for (int l = 0; l < kLanes; ++l) s.rad[l * N + i] = 0;
+        return;
+    }
+    const uint32_t sample = P.first_sample + sIdx;
+    // pixelNumbers and pixelWavelengths (trace.cpp:65-84)
+    const uint64_t place = ((uint64_t)x << 32) | (uint64_t)y;
+    const uint64_t stream = P.seed_bits ^ mixed_bits(place);
+    const double start = hashed(stream, ~0ull, 0);
+    const Lanes w = sampled_wavelengths(wavelength_share(start, sample), P.colour);
+    // filmDirection (core/camera.cpp:32-38) at the jittered film point
+    const double fx = x + hashed(stream, sample, PixelX), fy = y + hashed(stream, sample, PixelY);
+    const float u = (float)(2 * fx / P.width - 1), v = (float)(1 - 2 * fy / P.height);
+    V3 d = v3(P.forward[0], P.forward[1], P.forward[2]) + v3(P.right[0], P.right[1], P.right[2]) * (u * P.reach * P.aspect) +
+           v3(P.up[0], P.up[1], P.up[2]) * (v * P.reach);
+    d = normalized(d);
+    kdn_ray r;
+    r.origin[0] = P.eye[0];
+    r.origin[1] = P.eye[1];
+    r.origin[2] = P.eye[2];
+    r.direction[0] = d.x;
+    r.direction[1] = d.y;
+    r.direction[2] = d.z;
+    r.far_ = kInf;
+    r.inside = 0;
+    r.within = KDN_NO_MATERIAL;
+    s.rays[i] = r;
+    for (int l = 0; l < kLanes; ++l) {
+        s.thr[l * N + i] = 1.0f;
+        s.rad[l * N + i] = 0.0f;
+        s.nm[l * N + i] = w.nm[l];
+        s.dens[l * N + i] = w.density[l];
+    }
+    s.pdf[i] = 0;
+    s.meta[i] = FlagExact;
+    s.sample[i] = sample;
+    s.stream[i] = stream;
+    const uint32_t at = atomicAdd(&s.counters[0], 1u);
+    s.queue[0][at] = i;
+}
+
+// extend: trace path i's ray; with work set, count the march's operations
+KDN_D void trace_item(const TreeView& tv, uint32_t i, const kdn_ray* rays, kdn_hit* hits, Dl* buf, double* work) {
+    kdn_hit h;
+    if (work) {
+        double w = 0;
+        TreeView counted = tv;
+        counted.work = &w;
+        trace_ray(counted, rays[i], buf, h);
+        atomicAdd(work, w);
+    } else {
+        trace_ray(tv, rays[i], buf, h);
+    }
+    hits[i] = h;
+}
+
+KDN_D float lift_at(V3 x) {
+    return 8e-5f + 8e-6f * fmax_(fmax_(__builtin_fabsf(x.x), __builtin_fabsf(x.y)), __builtin_fabsf(x.z));
+}
+
+// shade: queue entry q of the input queue
+KDN_D void shade_one(PassParams& P, uint32_t q, uint32_t inQueue, uint32_t outQueue) {
+    PathState& s = P.s;
+    const size_t N = s.capacity;
+    const uint32_t i = s.queue[inQueue][q];
+    const kdn_hit hit = s.hits[i];
+    const kdn_ray ray = s.rays[i];
+    const uint64_t stream = s.stream[i];
+    const uint32_t sample = s.sample[i];
+    uint32_t meta = s.meta[i];
+    const int depth = (int)(meta & 0xFF);
+    Lanes w;
+    float thr[kLanes], rad[kLanes];
+    for (int l = 0; l < kLanes; ++l) {
+        w.nm[l] = s.nm[l * N + i];
+        w.density[l] = s.dens[l * N + i];
+        thr[l] = s.thr[l * N + i];
+        rad[l] = s.rad[l * N + i];
+    }
+    const V3 o = v3(ray.origin[0], ray.origin[1], ray.origin[2]);
+    const V3 d = v3(ray.direction[0], ray.direction[1], ray.direction[2]);
+    const bool inside = ray.inside != 0;
+    const float tHit = hit.kind == KDN_HIT_MISSED ? kInf : hit.distance;
+    const bool exactBefore = (meta & FlagExact) != 0;
+    const float pdfBefore = s.pdf[i];
+
+    // lamps met along the ray (metAlong, trace.cpp:352-374), MIS against aiming
+    if (!inside) {
+        for (uint32_t j = 0; j < P.n_lamps; ++j) {
+            const LampDev& L = P.lamps[j];
+            LampHit lh;
+            if (!meet_quad(L, o, d, tHit, &lh)) continue;
+            float Le[kLanes];
+            spectrum_of(L.radiance, w, Le);
+            const float pdfL = L.pick * lh.t * lh.t / (lh.cosine * L.area);
+            const float wt = exactBefore ? 1.0f : power_weight(pdfBefore, pdfL);
+            for (int l = 0; l < kLanes; ++l) rad[l] += thr[l] * Le[l] * wt;
+        }
+    }
+    bool alive = false;
+    if (hit.kind == KDN_HIT_MISSED) {
+        if (!inside) {
+            float sky[kLanes];
+            sky_radiance(P.sky_zenith, P.sky_horizon, P.sky_ground, d, w, sky);
+            for (int l = 0; l < kLanes; ++l) rad[l] += thr[l] * sky[l];
+        }
+    } else if (hit.kind == KDN_HIT_SURFACE) {
+        const uint32_t mi = hit.leaf != KDN_NO_LEAF ? P.leaf_material[hit.leaf] : P.fallback_material;
+        const MaterialDev& m = P.materials[mi < P.n_materials ? mi : P.fallback_material];
+        const V3 x = v3(hit.point[0], hit.point[1], hit.point[2]);
+        const V3 out = -d;
+        V3 n = v3(hit.normal[0], hit.normal[1], hit.normal[2]);
+        if (dot(n, out) < 0) n = -n;
+        if (m.glow[0] > 0 || m.glow[1] > 0 || m.glow[2] > 0) {
+            float g[kLanes];
+            spectrum_of(m.glow, w, g);
+            for (int l = 0; l < kLanes; ++l) rad[l] += thr[l] * g[l];
+        }
+        const float lift = lift_at(x);
+        const bool exact = m.kind == KDN_GLASS && m.exact;
+        // aim at one lamp (aimedFrom, trace.cpp:896-947), MIS against meeting
+        if (!exact && P.n_lamps > 0) {
+            const float uPick = number(stream, sample, depth, Pick);
+            uint32_t j = 0;
+            while (j + 1 < P.n_lamps && uPick >= P.lamps[j].pick_cdf) ++j;
+            const LampDev& L = P.lamps[j];
+            const float u = number(stream, sample, depth, LightU), v = number(stream, sample, depth, LightV);
+            const V3 y = v3(L.at[0], L.at[1], L.at[2]) + v3(L.along[0], L.along[1], L.along[2]) * u + v3(L.across[0], L.across[1], L.across[2]) * v;
+            const V3 toward = y - x;
+            const float dist = length(toward);
+            const V3 in = toward * (1.0f / dist);
+            const float cosL = -dot(v3(L.facing[0], L.facing[1], L.facing[2]), in);
+            const float cosS = dot(n, in);
+            if (cosL > 0 && cosS > 0 && dist > 2 * lift) {
+                float f[kLanes], Le[kLanes];
+                rough_value(m, w, n, out, in, f);
+                spectrum_of(L.radiance, w, Le);
+                const float pdfL = L.pick * dist * dist / (cosL * L.area);
+                const float wt = power_weight(pdfL, rough_density(m, n, out, in));
+                float c[kLanes];
+                float any = 0;
+                for (int l = 0; l < kLanes; ++l) {
+                    c[l] = thr[l] * f[l] * cosS * Le[l] / pdfL * wt;
+                    any += c[l];
+                }
+                if (any > 0) {
+                    kdn_ray sr;
+                    const V3 so = x + n * lift;
+                    sr.origin[0] = so.x;
+                    sr.origin[1] = so.y;
+                    sr.origin[2] = so.z;
+                    sr.direction[0] = in.x;
+                    sr.direction[1] = in.y;
+                    sr.direction[2] = in.z;
+                    sr.far_ = dist - lift;
+                    sr.inside = 0;
+                    sr.within = KDN_NO_MATERIAL;
+                    s.shadow[i] = sr;
+                    for (int l = 0; l < kLanes; ++l) s.srad[l * N + i] = c[l];
+                    const uint32_t at = atomicAdd(&s.counters[2], 1u);
+                    s.queue[2][at] = i;
+                }
+            }
+        }
+        // scatter (scatteredFrom, trace.cpp:1020-1074)
+        if (depth + 1 < (int)P.surfaces) {
+            const float uLobe = number(stream, sample, depth, Lobe), uSide = number(stream, sample, depth, Side);
+            const float du = number(stream, sample, depth, DirectionU), dv = number(stream, sample, depth, DirectionV);
+            const Scatter sc = m.kind == KDN_GLASS ? glass_scatter(m, w, n, out, inside, uSide) : rough_scatter(m, w, n, out, uLobe, du, dv);
+            if (sc.found) {
+                const float side = dot(sc.in, n);
+                const bool below = sc.through ? side >= 0 : side <= 0;
+                if (!below) {
+                    float carried[kLanes];
+                    for (int l = 0; l < kLanes; ++l) carried[l] = thr[l] * sc.weight[l];
+                    if (sc.collapse && !(meta & FlagDispersed)) {
+                        carried[0] *= kLanes;
+                        carried[1] = carried[2] = carried[3] = 0;
+                        meta |= FlagDispersed;
+                    }
+                    bool survives = true;
+                    if (depth >= (int)P.min_depth) {
+                        const float survival = fmin_(0.95f, fmax_(fmax_(carried[0], carried[1]), fmax_(carried[2], carried[3])));
+                        if (!(number(stream, sample, depth, Survival) < survival)) {
+                            survives = false;
+                        } else {
+                            for (int l = 0; l < kLanes; ++l) carried[l] /= survival;
+                        }
+                    }
+                    if (survives) {
+                        const bool nowInside = sc.through ? !inside : inside;
+                        const V3 no = x + n * (sc.through ? -lift : lift);
+                        kdn_ray nr;
+                        nr.origin[0] = no.x;
+                        nr.origin[1] = no.y;
+                        nr.origin[2] = no.z;
+                        nr.direction[0] = sc.in.x;
+                        nr.direction[1] = sc.in.y;
+                        nr.direction[2] = sc.in.z;
+                        nr.far_ = kInf;
+                        nr.inside = nowInside ? 1u : 0u;
+                        nr.within = nowInside ? mi : KDN_NO_MATERIAL;
+                        s.rays[i] = nr;
+                        for (int l = 0; l < kLanes; ++l) thr[l] = carried[l];
+                        s.pdf[i] = sc.exact ? 0.0f : sc.density;
+                        meta = (meta & ~(0xFFu | FlagExact)) | (uint32_t)(depth + 1) | (sc.exact ? FlagExact : 0u);
+                        alive = true;
+                    }
+                }
+            }
+        }
+    }
+    for (int l = 0; l < kLanes; ++l) {
+        s.rad[l * N + i] = rad[l];
+        s.thr[l * N + i] = thr[l];
+    }
+    s.meta[i] = meta;
+    if (alive) {
+        const uint32_t at = atomicAdd(&s.counters[outQueue], 1u);
+        s.queue[outQueue][at] = i;
+    }
+}
+
+// shadow: entry q of the shadow queue
+KDN_D void shadow_one(const TreeView& tv, PathState& s, uint32_t q, Dl* buf) {
+    const uint32_t i = s.queue[2][q];
+    const kdn_ray r = s.shadow[i];
+    kdn_hit h;
+    TreeView t = tv;
+    double w = 0;
+    if (tv.work) t.work = &w;
+    trace_ray(t, r, buf, h);
+    if (tv.work) atomicAdd(tv.work, w);
+    if (h.kind == KDN_HIT_MISSED || !(h.distance < r.far_)) {
+        const size_t N = s.capacity;
+        for (int l = 0; l < kLanes; ++l) s.rad[l * N + i] += s.srad[l * N + i];
+    }
+}
+
+// Fold the chunk's samples of each pixel into the FP64 accumulators, in
+// sample order.
+// fold: pixel slot t of the first sample plane
+KDN_D void fold_one(const PassParams& P, uint32_t t, double* accum, double* accumSq, uint32_t* accumN) {
+    const uint32_t tilesX = (P.width + 7) / 8, tilesY = (P.height + 7) / 8;
+    const uint32_t plane = tilesX * tilesY * 64;
+    if (t >= plane) return;
+    const PathState& s = P.s;
+    const size_t N = s.capacity;
+    const uint32_t pixel = s.pixel[t];
+    if (pixel == 0xFFFFFFFFu) return;
+    double sum[3] = {0, 0, 0}, sq[3] = {0, 0, 0};
+    for (uint32_t k = 0; k < P.chunk; ++k) {
+        const size_t i = (size_t)k * plane + t;
+        Lanes w;
+        float rad[kLanes];
+        for (int l = 0; l < kLanes; ++l) {
+            w.nm[l] = s.nm[l * N + i];
+            w.density[l] = s.dens[l * N + i];
+            rad[l] = s.rad[l * N + i];
+        }
+        double rgb[3];
+        colour_of(w, rad, P.colour, rgb);
+        for (int c = 0; c < 3; ++c) {
+            sum[c] += rgb[c];
+            sq[c] += rgb[c] * rgb[c];
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        accum[3 * (size_t)pixel + c] += sum[c];
+        accumSq[3 * (size_t)pixel + c] += sq[c];
+    }
+    accumN[pixel] += P.chunk;
+}
+
+}  // namespace kdn

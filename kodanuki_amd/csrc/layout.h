@@ -1,0 +1,26 @@
+// Device data layouts shared by the kernels and the host: octree nodes and
+// entries. Free of HIP headers so that a CPU build of the kernels can use
+// them.
+#pragma once
+
+#include <cstdint>
+
+namespace kdn {
+
+// An octree node as kept on the device, 32 bytes: the downloaded
+// kdn_node_record layout.
+struct NodeDev {
+    uint32_t level, x, y, z, kind, first_child, entry_first, entry_count;
+};
+
+// Registers of the trace's per-lane register file: the longest program a
+// surface leaf may hold (the octree build fails beyond it).
+constexpr int kMaxRegs = 384;
+
+enum NodeKind : uint32_t { KindEmpty = 0, KindFilled = 1, KindSurface = 2, KindBranch = 3 };
+
+struct EntryDev {
+    uint32_t object, program;
+};
+
+}  // namespace kdn

@@ -135,4 +135,68 @@ void escapeTable(double width, float table[kEnergyViews], float* average) {
     *average = (float)avg;
 }
 
+std::vector<MaterialDev> materialTable(const kdn_material* mats, size_t count) {
+    std::vector<MaterialDev> md;
+    for (size_t i = 0; i < count; ++i) {
+        const kdn_material& m = mats[i];
+        MaterialDev x{};
+        x.kind = m.kind;
+        for (int k = 0; k < 3; ++k) {
+            x.albedo[k] = m.albedo[k];
+            x.glow[k] = m.glow[k] * m.strength;
+        }
+        const double width = facetWidthOf(m.kind == KDN_MATTE ? 1.0 : m.roughness);
+        x.width = (float)width;
+        if (m.kind == KDN_METAL) {
+            escapeTable(width, x.escape, &x.escape_avg);
+        } else {
+            for (float& e : x.escape) e = 1.0f;
+            x.escape_avg = 1.0f;
+        }
+        x.facet_share = std::min(std::max(x.escape_avg, 0.25f), 1.0f);  // material.cpp:196
+        x.exact = m.kind == KDN_GLASS && m.roughness <= 0 ? 1u : 0u;
+        x.index = m.refraction;
+        x.spread = m.dispersion;
+        md.push_back(x);
+    }
+    return md;
+}
+
+std::vector<LampDev> lampTable(const kdn_light* lights, size_t count) {
+    std::vector<LampDev> lamps;
+    double totalPower = 0;
+    for (size_t i = 0; i < count; ++i) {
+        const kdn_light& l = lights[i];
+        if (l.kind != KDN_LIGHT_QUAD) continue;
+        LampDev x{};
+        x.kind = l.kind;
+        const double f[3] = {(double)l.along[1] * l.across[2] - (double)l.along[2] * l.across[1],
+                             (double)l.along[2] * l.across[0] - (double)l.along[0] * l.across[2],
+                             (double)l.along[0] * l.across[1] - (double)l.along[1] * l.across[0]};
+        const double area = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+        if (!(area > 1e-12)) continue;
+        for (int k = 0; k < 3; ++k) {
+            x.at[k] = l.at[k];
+            x.along[k] = l.along[k];
+            x.across[k] = l.across[k];
+            x.facing[k] = (float)(f[k] / area);
+            x.radiance[k] = (float)(l.power[k] / area);  // radianceOf (light.cpp:32-36)
+        }
+        x.area = (float)area;
+        const double emitted = std::max((l.power[0] + l.power[1] + l.power[2]) / 3.0, 0.0) * kPi;  // emittedOf (light.cpp:45-48)
+        x.pick = (float)emitted;
+        totalPower += emitted;
+        lamps.push_back(x);
+    }
+    double cdf = 0;
+    for (LampDev& x : lamps) {
+        const double p = totalPower > 0 ? x.pick / totalPower : 1.0 / lamps.size();
+        x.pick = (float)p;
+        cdf += p;
+        x.pick_cdf = (float)cdf;
+    }
+    if (!lamps.empty()) lamps.back().pick_cdf = 1.0f;
+    return lamps;
+}
+
 }  // namespace kdn

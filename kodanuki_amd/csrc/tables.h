@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <vector>
 
 #include "kodanuki.h"
 
@@ -47,6 +48,9 @@ struct LampDev {
 };
 
 void buildColourTables(ColourTables& t);
+// The integrator's material and lamp tables from the scene's.
+std::vector<MaterialDev> materialTable(const kdn_material* materials, size_t count);
+std::vector<LampDev> lampTable(const kdn_light* lights, size_t count);
 void escapeTable(double width, float table[kEnergyViews], float* average);
 double facetWidthOf(double roughness);
 

@@ -1,6 +1,7 @@
 // The extend step: rays in, hits out, one lane per ray.
 #include <hip/hip_runtime.h>
 
+#include "ctx.h"
 #include "trace_dev.h"
 
 namespace kdn {

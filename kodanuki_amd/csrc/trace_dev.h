@@ -13,17 +13,22 @@
 // sphere_trace.cpp:103-228) with FP32 tolerances.
 #pragma once
 
-#include "ctx.h"
+#include "layout.h"
 #include "tape_dev.h"
 
 namespace kdn {
 
-constexpr int kMaxRegs = 384;         // registers of a leaf program (checked at build)
 constexpr int kMaxIterations = 4096;  // cells plus march steps of one ray
 constexpr float kTolAbs = 1e-5f;      // landing tolerance: kTolAbs + kTolRel * t
 constexpr float kTolRel = 2e-6f;
 constexpr float kTie = 1e-6f;         // values within kTie (1 + max|p|) tie (sphere_trace.cpp:103-113)
 constexpr float kGraze = 0.02f;       // sphere_trace.cpp:16
+// The landing's Newton step: taken where the ray meets the surface at a
+// cosine above kNewtonFacing, and no longer than kNewtonReach tolerances
+// (a march stops at most one tolerance above the surface, which is
+// 1/kNewtonFacing tolerances along such a ray).
+constexpr float kNewtonFacing = 0.01f;
+constexpr float kNewtonReach = 100.0f;
 
 struct TreeView {
     const NodeDev* nodes;
@@ -111,8 +116,8 @@ KDN_D void faced_normal(const float g[3], const float d[3], bool inside, float n
 
 // The hit at t from entry `winner` of leaf n. The march stops within its
 // tolerance of the surface; one Newton step along the ray, t - v / (g.d),
-// bounded by the tolerance and taken only where the ray is not grazing,
-// brings the distance onto the surface.
+// bounded and taken only where the ray is not grazing, brings the distance
+// onto the surface.
 KDN_D void land(const TreeView& tv, const NodeDev& n, uint32_t nodeIndex, uint32_t winner, const float o[3], const float d[3], float t,
                 bool inside, Dl* buf, kdn_hit& hit) {
     const float px = o[0] + d[0] * t, py = o[1] + d[1] * t, pz = o[2] + d[2] * t;
@@ -121,9 +126,9 @@ KDN_D void land(const TreeView& tv, const NodeDev& n, uint32_t nodeIndex, uint32
     eval_dual(tv, at, DRegs{buf}, px, py, pz, &value);
     const float slope = value.g[0] * d[0] + value.g[1] * d[1] + value.g[2] * d[2];
     const float glen = __builtin_sqrtf(value.g[0] * value.g[0] + value.g[1] * value.g[1] + value.g[2] * value.g[2]);
-    const float bound = 2 * (kTolAbs + kTolRel * t);
+    const float bound = kNewtonReach * (kTolAbs + kTolRel * t);
     float tr = t;
-    if (__builtin_fabsf(slope) > 0.1f * glen) {
+    if (__builtin_fabsf(slope) > kNewtonFacing * glen) {
         const float dt = -value.v / slope;
         tr = t + fmax_(-bound, fmin_(bound, dt));
     }
@@ -290,13 +295,18 @@ KDN_D void trace_ray(const TreeView& tv, const kdn_ray& ray, Dl* buf, kdn_hit& h
             hit.normal[0] = hit.normal[1] = hit.normal[2] = 0;
             hit.normal[axis] = d[axis] > 0 ? -1.0f : 1.0f;
             hit.cell = index;
+            // the solid's leaf from its program where the register file
+            // holds it (a filled cell's program may be longer than any
+            // surface leaf's; the build checks surface leaves only)
             if (n.entry_count > 0) {
                 const uint32_t at = tv.entries[n.entry_first].program;
-                const float v = eval_program(tv, at, r, hit.point[0], hit.point[1], hit.point[2]);
-                (void)v;
-                float sh;
-                source_of(tv.pool + at + 1, tv.pool[at].c, r, &hit.leaf, &hit.other, &sh);
-                hit.share = sh;
+                const kdn_step head = tv.pool[at];
+                if (((uint32_t)head.a | ((uint32_t)head.b << 16)) <= (uint32_t)kMaxRegs) {
+                    (void)eval_program(tv, at, r, hit.point[0], hit.point[1], hit.point[2]);
+                    float sh;
+                    source_of(tv.pool + at + 1, head.c, r, &hit.leaf, &hit.other, &sh);
+                    hit.share = sh;
+                }
             }
             return;
         } else if (n.kind == KindEmpty && inside) {

@@ -984,6 +984,8 @@ void or_free_octree(or_octree* t) {
 /* ---- the march (twin of trace_dev.h) ---- */
 #define MAX_ITER 4096
 static const float kTolAbs = 1e-5f, kTolRel = 2e-6f, kTie = 1e-6f, kGraze = 0.02f;
+static const float kNewtonFacing = 0.01f, kNewtonReach = 100.0f;
+static const uint32_t kMaxRegs = 384; /* the GPU trace's register file */
 
 typedef struct {
     const or_octree* t;
@@ -1136,9 +1138,9 @@ static void land(const View* v, const kdn_node_record* n, uint32_t index, uint32
     /* one Newton step along the ray onto the surface, bounded */
     const float slope = value.g[0] * d[0] + value.g[1] * d[1] + value.g[2] * d[2];
     const float glen = sqrtf(value.g[0] * value.g[0] + value.g[1] * value.g[1] + value.g[2] * value.g[2]);
-    const float bound = 2 * (kTolAbs + kTolRel * t);
+    const float bound = kNewtonReach * (kTolAbs + kTolRel * t);
     float tr = t;
-    if (fabsf(slope) > 0.1f * glen) {
+    if (fabsf(slope) > kNewtonFacing * glen) {
         const float dt = -value.v / slope;
         tr = t + fmax_(-bound, fmin_(bound, dt));
     }
@@ -1300,7 +1302,9 @@ static void trace_one(const View* v, const kdn_ray* ray, kdn_hit* hit) {
             hit->normal[0] = hit->normal[1] = hit->normal[2] = 0;
             hit->normal[axis] = d[axis] > 0 ? -1.0f : 1.0f;
             hit->cell = index;
-            if (n.entry_count > 0) {
+            /* the GPU's register file holds kMaxRegs values: a longer
+             * program of a filled cell leaves the solid's leaf unnamed */
+            if (n.entry_count > 0 && header_length(&v->t->pool[v->t->entries[n.entry_first].program]) <= kMaxRegs) {
                 const uint32_t at = v->t->entries[n.entry_first].program;
                 (void)eval_prog(v, at, hit->point[0], hit->point[1], hit->point[2]);
                 float sh;

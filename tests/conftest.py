@@ -8,8 +8,22 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 
+# KDN_EMULATION names the bounds-checked CPU build of the library
+# (tests/cpu_build): the GPU tests then run their calls on the CPU under
+# AddressSanitizer, with the bounds-checked oracle, to find out-of-bounds
+# accesses of kernels and launches. Never set on a GPU.
+EMULATION = os.environ.get("KDN_EMULATION")
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X)")
+    if EMULATION:
+        import ctypes
+
+        import kodanuki_amd as k
+        from tests import oracle_py
+        k._bind(ctypes.CDLL(os.path.abspath(EMULATION)))
+        oracle_py.PATH = os.path.join(os.path.dirname(os.path.abspath(EMULATION)), "liboracle_asan.so")
 
 
 def has_gpu():
@@ -24,7 +38,7 @@ def has_gpu():
 def device():
     """One device context for the whole GPU session."""
     import kodanuki_amd as k
-    if not has_gpu():
+    if not (EMULATION or has_gpu()):
         pytest.skip("no GPU")
     dev = k.Device(0)
     yield dev

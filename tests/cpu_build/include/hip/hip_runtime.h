@@ -1,0 +1,164 @@
+// A CPU stand-in for the parts of the HIP runtime the library uses, so that
+// the library's own sources (kernels and host orchestration alike) build
+// with g++ under AddressSanitizer and run on the CPU. Test infrastructure:
+// only the bounds-checking build in this folder puts it on the include path.
+//
+// Device memory is host heap memory filled with 0xCD bytes when allocated,
+// so that a read of memory no kernel or copy wrote gives large indices and
+// NaN-like floats instead of the zeros fresh device memory often holds.
+// A launch runs every thread of every block one after another, which is a
+// legal schedule for kernels without barriers or wave-level exchange; the
+// prefix sum (the one kernel family with both) is replaced by emu_scan.cpp.
+#pragma once
+
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+
+#define __device__
+#define __host__
+#define __global__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+
+struct dim3 {
+    uint32_t x, y, z;
+    constexpr dim3(uint32_t a = 1, uint32_t b = 1, uint32_t c = 1) : x(a), y(b), z(c) {}
+};
+inline thread_local dim3 blockIdx{0, 0, 0}, threadIdx{0, 0, 0}, blockDim{1, 1, 1}, gridDim{1, 1, 1};
+
+enum hipError_t { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorOutOfMemory = 2, hipErrorNoDevice = 100 };
+enum hipMemcpyKind { hipMemcpyHostToHost = 0, hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice, hipMemcpyDefault };
+struct EmuStream {
+    int unused;
+};
+struct EmuEvent {
+    std::chrono::steady_clock::time_point at;
+};
+typedef EmuStream* hipStream_t;
+typedef EmuEvent* hipEvent_t;
+
+inline const char* hipGetErrorString(hipError_t e) { return e == hipSuccess ? "no error" : "emulated HIP error"; }
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipSetDevice(int) { return hipSuccess; }
+inline hipError_t hipGetDeviceCount(int* n) {
+    *n = 1;
+    return hipSuccess;
+}
+inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
+
+template <class T>
+inline hipError_t hipMalloc(T** p, size_t bytes) {
+    void* m = std::malloc(bytes ? bytes : 1);
+    if (!m) return hipErrorOutOfMemory;
+    std::memset(m, 0xCD, bytes);
+    *p = (T*)m;
+    return hipSuccess;
+}
+inline hipError_t hipFree(void* p) {
+    std::free(p);
+    return hipSuccess;
+}
+inline hipError_t hipMemcpy(void* dst, const void* src, size_t bytes, hipMemcpyKind) {
+    if (bytes) std::memcpy(dst, src, bytes);
+    return hipSuccess;
+}
+inline hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKind k, hipStream_t = nullptr) {
+    return hipMemcpy(dst, src, bytes, k);
+}
+inline hipError_t hipMemsetAsync(void* dst, int v, size_t bytes, hipStream_t = nullptr) {
+    if (bytes) std::memset(dst, v, bytes);
+    return hipSuccess;
+}
+inline hipError_t hipMemset(void* dst, int v, size_t bytes) { return hipMemsetAsync(dst, v, bytes); }
+
+inline hipError_t hipStreamCreate(hipStream_t* s) {
+    *s = new EmuStream{0};
+    return hipSuccess;
+}
+inline hipError_t hipStreamDestroy(hipStream_t s) {
+    delete s;
+    return hipSuccess;
+}
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+
+inline hipError_t hipEventCreate(hipEvent_t* e) {
+    *e = new EmuEvent{};
+    return hipSuccess;
+}
+inline hipError_t hipEventDestroy(hipEvent_t e) {
+    delete e;
+    return hipSuccess;
+}
+inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t = nullptr) {
+    e->at = std::chrono::steady_clock::now();
+    return hipSuccess;
+}
+inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) {
+    *ms = std::chrono::duration<float, std::milli>(b->at - a->at).count();
+    return hipSuccess;
+}
+
+// KDN_EMU_ORDER=reverse runs the threads of a launch last to first: a
+// kernel whose output depends on the order of its threads (beyond the
+// order of atomic appends) then gives different results.
+inline bool kdn_emu_reverse() {
+    static const bool reverse = [] {
+        const char* v = std::getenv("KDN_EMU_ORDER");
+        return v && std::strcmp(v, "reverse") == 0;
+    }();
+    return reverse;
+}
+
+template <class... P, class... A>
+inline void kdn_emu_launch(void (*kernel)(P...), dim3 grid, dim3 block, A&&... args) {
+    gridDim = grid;
+    blockDim = block;
+    const uint64_t blocks = (uint64_t)grid.x * grid.y * grid.z, threads = (uint64_t)block.x * block.y * block.z;
+    const bool reverse = kdn_emu_reverse();
+    for (uint64_t bi = 0; bi < blocks; ++bi) {
+        const uint64_t b = reverse ? blocks - 1 - bi : bi;
+        blockIdx = dim3((uint32_t)(b % grid.x), (uint32_t)(b / grid.x % grid.y), (uint32_t)(b / grid.x / grid.y));
+        for (uint64_t ti = 0; ti < threads; ++ti) {
+            const uint64_t t = reverse ? threads - 1 - ti : ti;
+            threadIdx = dim3((uint32_t)(t % block.x), (uint32_t)(t / block.x % block.y), (uint32_t)(t / block.x / block.y));
+            kernel(args...);
+        }
+    }
+}
+#define hipLaunchKernelGGL(kernel, grid, block, shared, stream, ...) kdn_emu_launch(kernel, dim3(grid), dim3(block), __VA_ARGS__)
+
+inline uint32_t __float_as_uint(float x) {
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    return b;
+}
+inline float __uint_as_float(uint32_t b) {
+    float x;
+    std::memcpy(&x, &b, 4);
+    return x;
+}
+inline int __float_as_int(float x) { return (int)__float_as_uint(x); }
+inline float __int_as_float(int b) { return __uint_as_float((uint32_t)b); }
+
+template <class T>
+inline T atomicAdd(T* p, T v) {
+    const T old = *p;
+    *p = old + v;
+    return old;
+}
+template <class T>
+inline T atomicMax(T* p, T v) {
+    const T old = *p;
+    if (v > old) *p = v;
+    return old;
+}
+template <class T>
+inline T atomicMin(T* p, T v) {
+    const T old = *p;
+    if (v < old) *p = v;
+    return old;
+}

@@ -117,9 +117,12 @@ def test_hits_against_fp64_sphere_trace(device, built, name, count):
     n = ref["normal"][both].astype(np.float64)
     facing = np.abs(np.sum(d * n, axis=1))
     note(f"[fp64] {name}: median rel {np.median(rel):.2e} p99 {np.percentile(rel, 99):.2e} max {rel.max():.2e} "
-         f"max facing>0.2 {rel[facing > 0.2].max():.2e}")
-    assert np.all(rel[facing > 0.2] < 1e-4)
-    assert np.mean(rel < 1e-4) > 0.99
+         f"max facing>=0.01 {rel[facing >= 0.01].max():.2e}")
+    # the landing's Newton step puts every hit met at a cosine above 0.01
+    # onto the surface; below it the march's stop, a tolerance above the
+    # surface, is 1/cosine tolerances short along the ray
+    assert np.all(rel[facing >= 0.01] < 1e-4)
+    assert np.mean(rel < 1e-4) > 0.995
     same_leaf = np.mean(hits["leaf"][both] == ref["leaf"][both])
     note(f"[fp64] {name}: leaf agreement {same_leaf:.4f}")
     assert same_leaf > 0.99
