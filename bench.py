@@ -62,13 +62,26 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device (the known-size kernel that scales HBM counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        src = torch.empty(args.copy_probe << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 4}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the render path here is built for a single MI355X "
               f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
         sys.exit(2)
 
-    import torch
+    # no torch in this process: the library runs on the HIP runtime it was
+    # built against (every pass below returns after its stream is drained)
     import kodanuki_amd as k
     from kodanuki_amd import scenes
 
@@ -90,12 +103,9 @@ def main():
         sample += S
     # the instrumented pass: algorithmic FP32 work of the march (not timed)
     counted = dev.render_pass(W, H, S, first_sample=sample, seed=rs.seed, surfaces=surfaces, keep=True, flags=k.PASS_COUNT)
-    torch.cuda.synchronize()
 
     trace_ms = 0.0
     launches = 0
-    t0 = time.perf_counter()
-    torch.cuda.synchronize()
     start = time.perf_counter()
     for step in range(args.steps):
         st = dev.render_pass(W, H, S, first_sample=sample if step == 0 else sample + step * S, seed=rs.seed,
@@ -104,7 +114,6 @@ def main():
         launches += st.trace_launches
         if step == 0:
             first_stats = st
-    torch.cuda.synchronize()
     elapsed = time.perf_counter() - start
     samples = args.steps * W * H * S
     value = samples / elapsed / 1e6

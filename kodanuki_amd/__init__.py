@@ -273,11 +273,27 @@ class Scene:
         return np.array(lower[:], np.float32), np.array(size[:], np.float32)
 
 
+def hip_runtimes():
+    """The HIP runtime libraries mapped into this process."""
+    try:
+        with open("/proc/self/maps") as f:
+            maps = f.read()
+    except OSError:
+        return []
+    return sorted({line.split()[-1] for line in maps.splitlines() if "libamdhip64" in line and line.split()[-1].startswith("/")})
+
+
 class Device:
     """A context on one HIP device: scene, octree and integrator in HBM."""
 
     def __init__(self, device=0):
         lib = library()
+        runtimes = hip_runtimes()
+        if len(runtimes) > 1:
+            # two HIP runtimes (e.g. one a Python package bundles, loaded
+            # after this library) would manage the same device apart
+            raise KodanukiError("two HIP runtimes are loaded in this process (" + ", ".join(runtimes) +
+                                "): open the device before importing a package that bundles its own")
         self._h = C.c_void_p()
         _check(lib.kdn_open(device, C.byref(self._h)))
 

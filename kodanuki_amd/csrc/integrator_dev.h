@@ -146,11 +146,17 @@ KDN_D Basis basis_around(V3 n) {
 KDN_D V3 to_world(const Basis& B, V3 v) { return B.t * v.x + B.b * v.y + B.n * v.z; }
 KDN_D V3 to_local(const Basis& B, V3 v) { return v3(dot(v, B.t), dot(v, B.b), dot(v, B.n)); }
 
+// Sine and cosine of a float angle through the tape's double polynomials,
+// rounded once: the same bits on every IEEE machine, so a CPU build of
+// these kernels follows the device's paths exactly.
+KDN_D float f_cos(float x) { return (float)d_cos((double)x); }
+KDN_D float f_sin(float x) { return (float)d_sin((double)x); }
+
 KDN_D V3 cosine_hemisphere(const Basis& B, float u, float v) {
     const float reach = __builtin_sqrtf(u);
     const float angle = 2 * kPi * v;
     const float height = __builtin_sqrtf(fmax_(1 - u, 0.0f));
-    return to_world(B, v3(reach * __builtin_cosf(angle), reach * __builtin_sinf(angle), height));
+    return to_world(B, v3(reach * f_cos(angle), reach * f_sin(angle), height));
 }
 
 // ---- microfacets (microfacet.cpp) --------------------------------------------
@@ -183,9 +189,9 @@ KDN_D V3 raised_facet(V3 s, float u, float v) {
     const V3 second = cross(s, first);
     const float reach = __builtin_sqrtf(u);
     const float angle = 2 * kPi * v;
-    const float across = reach * __builtin_cosf(angle);
+    const float across = reach * f_cos(angle);
     const float lean = 0.5f * (1 + s.z);
-    const float along = (1 - lean) * __builtin_sqrtf(fmax_(1 - across * across, 0.0f)) + lean * reach * __builtin_sinf(angle);
+    const float along = (1 - lean) * __builtin_sqrtf(fmax_(1 - across * across, 0.0f)) + lean * reach * f_sin(angle);
     const float height = __builtin_sqrtf(fmax_(1 - across * across - along * along, 0.0f));
     return first * across + second * along + s * height;
 }
@@ -234,7 +240,9 @@ KDN_D void rough_value(const MaterialDev& m, const Lanes& w, V3 n, V3 out, V3 in
     const V3 h = halfway_of(out, in, n);
     const float onto = fmax_(dot(out, h), 0.0f);
     const float facets = facet_distribution(dot(n, h), m.width) * smith_visibility(leaving, arriving, m.width);
-    const float rest = __builtin_powf(fmin_(fmax_(1 - onto, 0.0f), 1.0f), 5.0f);
+    const float apart = fmin_(fmax_(1 - onto, 0.0f), 1.0f);
+    const float apart2 = apart * apart;
+    const float rest = apart2 * apart2 * apart;
     float schlick[3], avg[3];
     for (int k = 0; k < 3; ++k) {
         schlick[k] = m.albedo[k] + (1 - m.albedo[k]) * rest;

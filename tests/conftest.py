@@ -27,11 +27,10 @@ def pytest_configure(config):
 
 
 def has_gpu():
-    try:
-        import torch
-        return torch.cuda.is_available()
-    except Exception:
-        return False
+    """A HIP device node is present (the tests open the device through the
+    library itself: the process then runs on the HIP runtime the library
+    was built against, with no second runtime beside it)."""
+    return os.path.exists("/dev/kfd")
 
 
 @pytest.fixture(scope="session")

@@ -36,8 +36,7 @@ def test_scene_error_is_a_value_with_position():
 
 
 def test_device_calls_fail_loudly_without_a_gpu():
-    import torch
-    if torch.cuda.is_available():
+    if os.path.exists("/dev/kfd"):
         pytest.skip("a GPU is present")
     with pytest.raises(k.KodanukiError):
         k.Device(0)

@@ -15,7 +15,6 @@ import pytest
 import kodanuki_amd as k
 from kodanuki_amd import scenes
 from tests import oracle_py as O
-from tests.conftest import EMULATION
 
 pytestmark = pytest.mark.gpu
 
@@ -48,16 +47,7 @@ def _render(device, text, depth, spp, w=None, h=None, surfaces=None, seed=None, 
     return s, lower, size, rgb, sq, n, st
 
 
-# The rough-metal furnace is held off the device: a device fault was
-# reported right after its render once, and no cause is shown yet (the
-# library's CPU build under AddressSanitizer runs it clean, see
-# tests/test_cpu_emulation.py). It runs on that CPU build.
-_METAL_HELD = pytest.mark.skipif(not EMULATION, reason="held off the device: a device fault followed this render once, "
-                                                       "cause not found; runs on the bounds-checked CPU build")
-
-
-@pytest.mark.parametrize("material", ["matte([1, 1, 1])",
-                                      pytest.param("metal([1, 1, 1], roughness = 0.6)", marks=_METAL_HELD),
+@pytest.mark.parametrize("material", ["matte([1, 1, 1])", "metal([1, 1, 1], roughness = 0.6)",
                                       "glass(refraction = 1.5, dispersion = 0.008)"])
 def test_white_furnace(device, material):
     s, lower, size, rgb, sq, n, st = _render(device, furnace(material), 5, 512)
