@@ -446,6 +446,8 @@ int kdn_upload_scene(kdn_ctx* ctx, const kdn_scene* s) {
         c.error = "kdn_upload_scene: device allocation or copy failed";
         return ctxFail(ctx, KDN_E_MEMORY);
     }
+    c.scene_has_glass = false;
+    for (uint32_t m : leafMat) c.scene_has_glass = c.scene_has_glass || (m < mats.size() && mats[m].kind == KDN_GLASS);
     c.n_materials_dev = (uint32_t)md.size();
     c.n_lamps = (uint32_t)lamps.size();
     c.integrator_ready = true;
@@ -556,6 +558,11 @@ int kdn_render_pass(kdn_ctx* ctx, const kdn_pass_desc* desc, kdn_pass_stats* sta
     }
     if (!c.tree.built) {
         c.error = "kdn_render_pass: no octree";
+        return ctxFail(ctx, KDN_E_STATE);
+    }
+    if (c.scene_has_glass && !KDN_GLASS_ON_DEVICE) {
+        c.error = "kdn_render_pass: glass scattering is held off the device (renders with glass faulted it; the cause is "
+                  "not found): paint the scene without glass";
         return ctxFail(ctx, KDN_E_STATE);
     }
     if (!kdn::renderPass(c, *desc, stats)) return ctxFail(ctx, KDN_E_DEVICE);

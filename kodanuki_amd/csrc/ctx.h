@@ -69,7 +69,16 @@ struct Ctx {
     void* lamps_dev = nullptr;      // LampDev per quad lamp
     uint32_t n_lamps = 0;
     bool integrator_ready = false;
+    bool scene_has_glass = false;   // a leaf is painted with a glass
 };
+
+// Glass scattering is held off the device: renders with glass faulted the
+// device and no cause is shown yet. kdn_render_pass refuses such scenes
+// unless this is set (the bounds-checking CPU build of the kernels sets
+// it; the device build does not).
+#ifndef KDN_GLASS_ON_DEVICE
+#define KDN_GLASS_ON_DEVICE 0
+#endif
 
 bool ensure(DevBuf& b, size_t bytes, std::string& err, bool keep);
 void releaseAll(Ctx& c);

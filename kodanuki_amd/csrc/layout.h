@@ -5,6 +5,12 @@
 
 #include <cstdint>
 
+// A bounds assertion that only the bounds-checking CPU build of the
+// kernels defines; on the device it compiles to nothing.
+#ifndef KDN_CHECK
+#define KDN_CHECK(cond) ((void)0)
+#endif
+
 namespace kdn {
 
 // An octree node as kept on the device, 32 bytes: the downloaded

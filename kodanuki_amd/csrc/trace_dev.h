@@ -43,11 +43,17 @@ struct TreeView {
 
 struct Regs {
     Dl* d;
-    KDN_D float& operator[](uint32_t i) const { return d[i].v; }
+    KDN_D float& operator[](uint32_t i) const {
+        KDN_CHECK(i < (uint32_t)kMaxRegs);
+        return d[i].v;
+    }
 };
 struct DRegs {
     Dl* d;
-    KDN_D Dl& operator[](uint32_t i) const { return d[i]; }
+    KDN_D Dl& operator[](uint32_t i) const {
+        KDN_CHECK(i < (uint32_t)kMaxRegs);
+        return d[i];
+    }
 };
 
 KDN_D float eval_program(const TreeView& tv, uint32_t at, const Regs& r, float px, float py, float pz) {

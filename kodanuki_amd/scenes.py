@@ -64,8 +64,14 @@ def _primitive(rng, scale=1.0):
     return f"torus({big:.6g}, {small:.6g})", big + small
 
 
-def _material(rng, name, kinds):
+def _material(rng, name, kinds, glass=True):
     k = rng.pick(kinds)
+    if k == 2 and not glass:
+        # the same draws as a glass (so the rest of the scene is the same),
+        # painted as a white matte
+        rng.uniform(1.4, 1.7)
+        rng.uniform(0.003, 0.01)
+        return f"let {name} = matte({_v([0.9, 0.9, 0.9])});"
     if k == 0:
         a = [rng.uniform(0.2, 0.8) for _ in range(3)]
         return f"let {name} = matte({_v(a)});"
@@ -125,8 +131,9 @@ def _nest(rng, depth, leaves, materials):
     return [head] + ["\t" + line for line in body] + ["}"]
 
 
-def production(seed=0, count=512, width=1920, height=1080, samples=64, surfaces=13):
-    """The production scene: groups of 4 primitives nested to depth 4."""
+def production(seed=0, count=512, width=1920, height=1080, samples=64, surfaces=13, glass=True):
+    """The production scene: groups of 4 primitives nested to depth 4.
+    With glass=False its glass materials are white mattes (same geometry)."""
     rng = Rng(seed + 1000)
     out = ['import "standard.sdf";',
            f"render {{ width = {width}; height = {height}; seed = {seed + 1}; samples = {samples}; surfaces = {surfaces}; far = 8; }}",
@@ -137,7 +144,7 @@ def production(seed=0, count=512, width=1920, height=1080, samples=64, surfaces=
         out.append(f"light quad {{ at = {_v([x, y, 5])}; along = [0, 1.2, 0]; across = [1.2, 0, 0]; power = {_v([p, p * 0.95, p * 0.9])}; }}")
     nm = 48
     for i in range(nm):
-        out.append(_material(rng, f"m{i}", [0.4, 0.3, 0.3]))
+        out.append(_material(rng, f"m{i}", [0.4, 0.3, 0.3], glass))
     out.append("let floor = matte([0.5, 0.5, 0.48]);")
     groups = count // 4
     side = int(math.ceil(math.sqrt(groups)))
@@ -154,9 +161,15 @@ def production(seed=0, count=512, width=1920, height=1080, samples=64, surfaces=
     return "\n".join(out) + "\n"
 
 
-def tiny(seed=0):
-    """A handful of shapes for quick checks: a sphere, a box and a torus."""
-    return """import "standard.sdf";
+def tiny(seed=0, glass=True):
+    """A handful of shapes for quick checks: a sphere, a box and a torus.
+    With glass=False the torus is a white matte instead of glass."""
+    text = _TINY if glass else _TINY.replace("let pane = glass(refraction = 1.5, dispersion = 0.005);",
+                                             "let pane = matte([0.9, 0.9, 0.9]);")
+    return text
+
+
+_TINY = """import "standard.sdf";
 render { width = 64; height = 48; seed = 1; samples = 4; surfaces = 4; far = 3; }
 camera { eye = [0, -5, 2]; target = [0, 0, 0.4]; up = [0, 0, 1]; fov = 0.8; }
 light quad { at = [-0.5, -0.5, 3]; along = [0, 1, 0]; across = [1, 0, 0]; power = [20, 20, 20]; }

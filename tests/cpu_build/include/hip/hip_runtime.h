@@ -12,10 +12,23 @@
 #pragma once
 
 #include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+
+// The kernels' bounds assertions, live in this build only.
+[[noreturn]] inline void kdn_emu_fail(const char* what, const char* file, int line) {
+    std::fprintf(stderr, "KDN_CHECK failed: %s at %s:%d\n", what, file, line);
+    std::abort();
+}
+#define KDN_CHECK(cond) ((cond) ? (void)0 : kdn_emu_fail(#cond, __FILE__, __LINE__))
+// this build runs glass scattering, which the device build holds back
+// (-DKDN_EMU_AS_DEVICE keeps the device build's refusal, to test it)
+#ifndef KDN_EMU_AS_DEVICE
+#define KDN_GLASS_ON_DEVICE 1
+#endif
 
 #define __device__
 #define __host__

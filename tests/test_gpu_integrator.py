@@ -15,6 +15,7 @@ import pytest
 import kodanuki_amd as k
 from kodanuki_amd import scenes
 from tests import oracle_py as O
+from tests.conftest import EMULATION
 
 pytestmark = pytest.mark.gpu
 
@@ -47,8 +48,18 @@ def _render(device, text, depth, spp, w=None, h=None, surfaces=None, seed=None, 
     return s, lower, size, rgb, sq, n, st
 
 
+# Glass scattering is held off the device (kdn_render_pass refuses scenes
+# with glass there): two renders of the glass furnace faulted the device and
+# no cause is shown. The bounds-checking CPU build of the library runs the
+# glass cases (KDN_EMULATION); on the device the scenes paint the glass as a
+# white matte.
+GLASS = bool(EMULATION)
+_GLASS_HELD = pytest.mark.skipif(not EMULATION, reason="glass scattering is held off the device: its furnace render "
+                                                       "faulted the device twice, cause not found")
+
+
 @pytest.mark.parametrize("material", ["matte([1, 1, 1])", "metal([1, 1, 1], roughness = 0.6)",
-                                      "glass(refraction = 1.5, dispersion = 0.008)"])
+                                      pytest.param("glass(refraction = 1.5, dispersion = 0.008)", marks=_GLASS_HELD)])
 def test_white_furnace(device, material):
     s, lower, size, rgb, sq, n, st = _render(device, furnace(material), 5, 512)
     mean = rgb / n[..., None]
@@ -64,8 +75,18 @@ def test_white_furnace(device, material):
         assert np.mean(np.abs(mean - 1.0) < 4 * se + 1e-6) > 0.99
 
 
+@pytest.mark.skipif(bool(EMULATION), reason="the device's refusal; the CPU build runs glass")
+def test_device_refuses_glass(device):
+    s = k.Scene(scenes.tiny())
+    lower, size = s.region(5)
+    device.upload(s)
+    device.build_octree(lower, size, 5)
+    with pytest.raises(k.KodanukiError, match="glass scattering is held off the device"):
+        device.render_pass(24, 18, 1)
+
+
 def test_progressive_passes_equal_one_pass(device):
-    text = scenes.tiny()
+    text = scenes.tiny(glass=GLASS)
     _, _, _, a, _, na, _ = _render(device, text, 5, 16)
     _render(device, text, 5, 8)
     _, _, _, b, _, nb, _ = _render(device, text, 5, 8, first=8, keep=True)
@@ -74,7 +95,7 @@ def test_progressive_passes_equal_one_pass(device):
 
 
 def test_matches_fp64_oracle(device):
-    text = scenes.tiny()
+    text = scenes.tiny(glass=GLASS)
     W, H = 24, 18
     spp_gpu, spp_cpu = 8192, 256
     s, lower, size, rgb, sq, n, st = _render(device, text, 5, spp_gpu, w=W, h=H, surfaces=5)
