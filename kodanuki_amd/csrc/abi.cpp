@@ -205,6 +205,7 @@ void releaseAll(Ctx& c) {
     freeBuf(c.tree.nodes);
     freeBuf(c.tree.entries);
     freeBuf(c.tree.pool);
+    freeBuf(c.tree.spool);
     if (c.tree.level_first_dev) hipFree(c.tree.level_first_dev);
     c.tree = OctreeDev{};
     for (DevBuf& b : c.scratch) freeBuf(b);

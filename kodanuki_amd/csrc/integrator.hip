@@ -28,22 +28,26 @@ __global__ void k_generate(PassParams P, uint32_t slots) {
     if (i < slots) generate_one(P, i);
 }
 
-__global__ void __launch_bounds__(256) k_trace_queue(TreeView tv, const uint32_t* queue, const uint32_t* count, const kdn_ray* rays,
-                                                     kdn_hit* hits) {
+__global__ void __launch_bounds__(kTraceBlock) k_trace_queue(TreeView tv, const uint32_t* queue, const uint32_t* count,
+                                                             const kdn_ray* rays, kdn_hit* hits) {
+    __shared__ float slots[kSlots * kTraceBlock];
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= *count) return;
     Dl buf[kMaxRegs];
-    trace_item(tv, queue[q], rays, hits, buf, nullptr);
+    const LaneRegs lr{buf, SlotRegs{slots + threadIdx.x, kTraceBlock}};
+    trace_item(tv, queue[q], rays, hits, lr, nullptr);
 }
 
 // The same march counting the FP32 operations of every evaluation (the
 // instrumented pass the roofline's algorithmic work comes from).
-__global__ void __launch_bounds__(256) k_trace_queue_count(TreeView tv, const uint32_t* queue, const uint32_t* count,
-                                                           const kdn_ray* rays, kdn_hit* hits, double* work) {
+__global__ void __launch_bounds__(kTraceBlock) k_trace_queue_count(TreeView tv, const uint32_t* queue, const uint32_t* count,
+                                                                   const kdn_ray* rays, kdn_hit* hits, double* work) {
+    __shared__ float slots[kSlots * kTraceBlock];
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= *count) return;
     Dl buf[kMaxRegs];
-    trace_item(tv, queue[q], rays, hits, buf, work);
+    const LaneRegs lr{buf, SlotRegs{slots + threadIdx.x, kTraceBlock}};
+    trace_item(tv, queue[q], rays, hits, lr, work);
 }
 
 __global__ void __launch_bounds__(256) k_shade(PassParams P, uint32_t inQueue, uint32_t outQueue) {
@@ -51,11 +55,13 @@ __global__ void __launch_bounds__(256) k_shade(PassParams P, uint32_t inQueue, u
     if (q < P.s.counters[inQueue]) shade_one(P, q, inQueue, outQueue);
 }
 
-__global__ void __launch_bounds__(256) k_shadow(TreeView tv, PathState s) {
+__global__ void __launch_bounds__(kTraceBlock) k_shadow(TreeView tv, PathState s) {
+    __shared__ float slots[kSlots * kTraceBlock];
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= s.counters[2]) return;
     Dl buf[kMaxRegs];
-    shadow_one(tv, s, q, buf);
+    const LaneRegs lr{buf, SlotRegs{slots + threadIdx.x, kTraceBlock}};
+    shadow_one(tv, s, q, lr);
 }
 
 __global__ void k_fold(PassParams P, double* accum, double* accumSq, uint32_t* accumN) {
@@ -261,10 +267,10 @@ bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
             const uint32_t blocks = (count + 255) / 256;
             timed(&traceMs, [&] {
                 if (counting) {
-                    hipLaunchKernelGGL(k_trace_queue_count, dim3(blocks), dim3(256), 0, st, tv, s->queue[in], &s->counters[in], s->rays,
+                    hipLaunchKernelGGL(k_trace_queue_count, dim3(blocks), dim3(kTraceBlock), 0, st, tv, s->queue[in], &s->counters[in], s->rays,
                                        s->hits, work);
                 } else {
-                    hipLaunchKernelGGL(k_trace_queue, dim3(blocks), dim3(256), 0, st, tv, s->queue[in], &s->counters[in], s->rays, s->hits);
+                    hipLaunchKernelGGL(k_trace_queue, dim3(blocks), dim3(kTraceBlock), 0, st, tv, s->queue[in], &s->counters[in], s->rays, s->hits);
                 }
             });
             KDN_HIP(hipMemsetAsync(&s->counters[out], 0, 4, st));
@@ -274,7 +280,7 @@ bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
             KDN_HIP(hipMemcpyAsync(&sc, &s->counters[2], 4, hipMemcpyDeviceToHost, st));
             KDN_HIP(hipStreamSynchronize(st));
             shadows += sc;
-            if (sc) timed(&shadowMs, [&] { hipLaunchKernelGGL(k_shadow, dim3((sc + 255) / 256), dim3(256), 0, st, tvShadow, *s); });
+            if (sc) timed(&shadowMs, [&] { hipLaunchKernelGGL(k_shadow, dim3((sc + kTraceBlock - 1) / kTraceBlock), dim3(kTraceBlock), 0, st, tvShadow, *s); });
             std::swap(in, out);
         }
         timed(&otherMs, [&] {

@@ -110,16 +110,16 @@ KDN_D void generate_one(PassParams& P, uint32_t i) {
 }
 
 // extend: trace path i's ray; with work set, count the march's operations
-KDN_D void trace_item(const TreeView& tv, uint32_t i, const kdn_ray* rays, kdn_hit* hits, Dl* buf, double* work) {
+KDN_D void trace_item(const TreeView& tv, uint32_t i, const kdn_ray* rays, kdn_hit* hits, const LaneRegs& lr, double* work) {
     kdn_hit h;
     if (work) {
         double w = 0;
         TreeView counted = tv;
         counted.work = &w;
-        trace_ray(counted, rays[i], buf, h);
+        trace_ray(counted, rays[i], lr, h);
         atomicAdd(work, w);
     } else {
-        trace_ray(tv, rays[i], buf, h);
+        trace_ray(tv, rays[i], lr, h);
     }
     hits[i] = h;
 }
@@ -292,14 +292,14 @@ KDN_D void shade_one(PassParams& P, uint32_t q, uint32_t inQueue, uint32_t outQu
 }
 
 // shadow: entry q of the shadow queue
-KDN_D void shadow_one(const TreeView& tv, PathState& s, uint32_t q, Dl* buf) {
+KDN_D void shadow_one(const TreeView& tv, PathState& s, uint32_t q, const LaneRegs& lr) {
     const uint32_t i = s.queue[2][q];
     const kdn_ray r = s.shadow[i];
     kdn_hit h;
     TreeView t = tv;
     double w = 0;
     if (tv.work) t.work = &w;
-    trace_ray(t, r, buf, h);
+    trace_ray(t, r, lr, h);
     if (tv.work) atomicAdd(tv.work, w);
     if (h.kind == KDN_HIT_MISSED || !(h.distance < r.far_)) {
         const size_t N = s.capacity;

@@ -22,6 +22,11 @@ struct NodeDev {
 // Registers of the trace's per-lane register file: the longest program a
 // surface leaf may hold (the octree build fails beyond it).
 constexpr int kMaxRegs = 384;
+// Slot registers of the march's LDS register file per lane: the most
+// values a program keeps live at once after the build's allocation (the
+// build fails beyond it), and the trace kernels' block size.
+constexpr int kSlots = 32;
+constexpr int kTraceBlock = 256;
 
 enum NodeKind : uint32_t { KindEmpty = 0, KindFilled = 1, KindSurface = 2, KindBranch = 3 };
 

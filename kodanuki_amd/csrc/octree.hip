@@ -328,6 +328,60 @@ __global__ void k_surface_program_max(const NodeDev* nodes, uint32_t n, const En
     atomicMax(most, longest);
 }
 
+// Slot registers for every entry's program (one lane per entry): a step's
+// value takes the lowest free slot once the operands whose last use it is
+// have given theirs back, so a program needs as many slots as it keeps
+// values live at once. The result lives to the end. Writes the program on
+// slots at the same offsets of `spool`; the destination slot goes in the
+// step's flags. A program that needs more than kSlots raises *overflow.
+__global__ void k_slot_programs(const EntryDev* entries, uint32_t nEntries, const kdn_step* pool, kdn_step* spool,
+                                uint16_t* lastUse, uint8_t* slotOf, uint32_t* overflow) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nEntries) return;
+    const uint32_t at = entries[e].program;
+    const kdn_step h = pool[at];
+    const uint32_t L = (uint32_t)h.a | ((uint32_t)h.b << 16);
+    const kdn_step* prog = pool + at + 1;
+    uint16_t* last = lastUse + at + 1;
+    uint8_t* slot = slotOf + at + 1;
+    constexpr uint16_t kUnread = 0xFFFF, kKept = 0xFFFE;
+    for (uint32_t i = 0; i < L; ++i) last[i] = kUnread;
+    for (uint32_t i = 0; i < L; ++i) {
+        const kdn_step st = prog[i];
+        const int n = operand_count(st.op);
+        if (n > 0) last[st.a] = (uint16_t)i;
+        if (n > 1) last[st.b] = (uint16_t)i;
+        if (n > 2) last[st.c] = (uint16_t)i;
+    }
+    if (L > 0) last[h.c] = kKept;
+    uint32_t freeSlots = kSlots >= 32 ? 0xFFFFFFFFu : ((1u << kSlots) - 1);
+    kdn_step* out = spool + at + 1;
+    for (uint32_t i = 0; i < L; ++i) {
+        const kdn_step st = prog[i];
+        const int n = operand_count(st.op);
+        if (n > 0 && last[st.a] == i) freeSlots |= 1u << slot[st.a];
+        if (n > 1 && last[st.b] == i) freeSlots |= 1u << slot[st.b];
+        if (n > 2 && last[st.c] == i) freeSlots |= 1u << slot[st.c];
+        if (freeSlots == 0) {
+            atomicMax(overflow, L);
+            return;
+        }
+        const uint32_t d = (uint32_t)__builtin_ctz(freeSlots);
+        freeSlots &= ~(1u << d);
+        slot[i] = (uint8_t)d;
+        kdn_step o = st;
+        o.flags = (uint8_t)d;
+        o.a = n > 0 ? slot[st.a] : 0;
+        o.b = n > 1 ? slot[st.b] : 0;
+        o.c = n > 2 ? slot[st.c] : 0;
+        out[i] = o;
+        if (last[i] == kUnread) freeSlots |= 1u << d;  // a value nothing reads
+    }
+    kdn_step oh = h;
+    oh.c = L > 0 ? slot[h.c] : 0;
+    spool[at] = oh;
+}
+
 static inline uint32_t blocksFor(uint64_t n, uint32_t t = 256) { return (uint32_t)((n + t - 1) / t); }
 
 // Grow a device buffer, keeping its first `keep` bytes.
@@ -510,6 +564,26 @@ bool buildOctree(Ctx& c, const float lower[3], const float size[3], int depth, k
         if (longest > (uint32_t)kMaxRegs) {
             c.error = "a surface leaf's program of " + std::to_string(longest) + " steps passes the trace's register file of " +
                       std::to_string(kMaxRegs) + " (a deeper octree cuts programs shorter)";
+            return false;
+        }
+    }
+    {
+        // the march's programs on slot registers
+        if (!growKeep(c, t.spool, t.n_pool * sizeof(kdn_step), 0)) return false;
+        if (!ensure(bufS, t.n_pool * 3 + 256, c.error, false)) return false;
+        if (!ensure(bufC, 64, c.error, false)) return false;
+        uint16_t* lastUse = (uint16_t*)bufS.ptr;
+        uint8_t* slotOf = (uint8_t*)(lastUse + t.n_pool);
+        uint32_t* overflow = (uint32_t*)bufC.ptr;
+        KDN_HIP(hipMemsetAsync(overflow, 0, 4, s));
+        hipLaunchKernelGGL(k_slot_programs, dim3(blocksFor(t.n_entries)), dim3(256), 0, s, (const EntryDev*)t.entries.ptr,
+                           (uint32_t)t.n_entries, (const kdn_step*)t.pool.ptr, (kdn_step*)t.spool.ptr, lastUse, slotOf, overflow);
+        uint32_t longest = 0;
+        KDN_HIP(hipMemcpyAsync(&longest, overflow, 4, hipMemcpyDeviceToHost, s));
+        KDN_HIP(hipStreamSynchronize(s));
+        if (longest) {
+            c.error = "a program of " + std::to_string(longest) + " steps keeps more than " + std::to_string(kSlots) +
+                      " values live (the march's slot registers)";
             return false;
         }
     }

@@ -6,12 +6,14 @@
 
 namespace kdn {
 
-__global__ void __launch_bounds__(256) k_extend(TreeView tv, const kdn_ray* rays, uint32_t n, kdn_hit* hits) {
+__global__ void __launch_bounds__(kTraceBlock) k_extend(TreeView tv, const kdn_ray* rays, uint32_t n, kdn_hit* hits) {
+    __shared__ float slots[kSlots * kTraceBlock];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Dl buf[kMaxRegs];
+    const LaneRegs lr{buf, SlotRegs{slots + threadIdx.x, kTraceBlock}};
     kdn_hit hit;
-    trace_ray(tv, rays[i], buf, hit);
+    trace_ray(tv, rays[i], lr, hit);
     hits[i] = hit;
 }
 
@@ -20,6 +22,7 @@ TreeView treeView(const Ctx& c) {
     tv.nodes = (const NodeDev*)c.tree.nodes.ptr;
     tv.entries = (const EntryDev*)c.tree.entries.ptr;
     tv.pool = (const kdn_step*)c.tree.pool.ptr;
+    tv.spool = (const kdn_step*)c.tree.spool.ptr;
     tv.leafMaterial = c.scene.leaf_material;
     for (int k = 0; k < 3; ++k) {
         tv.lower[k] = c.tree.lower[k];
@@ -36,7 +39,7 @@ bool extendRays(Ctx& c, const kdn_ray* rays, uint32_t n, kdn_hit* hits, hipStrea
         return false;
     }
     if (n == 0) return true;
-    hipLaunchKernelGGL(k_extend, dim3((n + 255) / 256), dim3(256), 0, stream, treeView(c), rays, n, hits);
+    hipLaunchKernelGGL(k_extend, dim3((n + kTraceBlock - 1) / kTraceBlock), dim3(kTraceBlock), 0, stream, treeView(c), rays, n, hits);
     KDN_HIP(hipGetLastError());
     return true;
 }

@@ -34,6 +34,7 @@ struct TreeView {
     const NodeDev* nodes;
     const EntryDev* entries;
     const kdn_step* pool;
+    const kdn_step* spool;  // the pool's programs on slot registers, same offsets
     const uint32_t* leafMaterial;
     float lower[3];
     float size[3];
@@ -55,6 +56,40 @@ struct DRegs {
         return d[i];
     }
 };
+
+// A lane's slot registers: slot k at s[k * stride] (in LDS, one column per
+// lane of the block, so a wave's lanes touch consecutive banks).
+struct SlotRegs {
+    float* s;
+    uint32_t stride;
+    KDN_D float& operator[](uint32_t i) const {
+        KDN_CHECK(i < (uint32_t)kSlots);
+        return s[i * stride];
+    }
+};
+
+// A lane's register files: the slot registers every march evaluation runs
+// in, and the private SSA registers of a landing's dual evaluation and
+// source walk.
+struct LaneRegs {
+    Dl* ssa;
+    SlotRegs slots;
+};
+
+// A program on slot registers (the build's allocation, same offsets as the
+// SSA pool): each step writes its value to the slot in its flags field.
+KDN_D float eval_slots(const TreeView& tv, uint32_t at, const SlotRegs& r, float px, float py, float pz) {
+    const kdn_step h = tv.spool[at];
+    if (tv.work) *tv.work += h.value;
+    const uint32_t L = (uint32_t)h.a | ((uint32_t)h.b << 16);
+    const kdn_step* prog = tv.spool + at + 1;
+    for (uint32_t i = 0; i < L; ++i) {
+        const kdn_step st = prog[i];
+        const float v = point_step(st, r, px, py, pz);
+        r[st.flags] = v;
+    }
+    return r[h.c];
+}
 
 KDN_D float eval_program(const TreeView& tv, uint32_t at, const Regs& r, float px, float py, float pz) {
     const kdn_step* pool = tv.pool;
@@ -82,10 +117,10 @@ struct LeafField {
     uint32_t arg;     // entry of the least value (first on ties)
 };
 
-KDN_D LeafField leaf_field(const TreeView& tv, const NodeDev& n, const Regs& r, float px, float py, float pz) {
+KDN_D LeafField leaf_field(const TreeView& tv, const NodeDev& n, const SlotRegs& r, float px, float py, float pz) {
     LeafField f{kInf, kInf, 0};
     for (uint32_t j = 0; j < n.entry_count; ++j) {
-        const float v = eval_program(tv, tv.entries[n.entry_first + j].program, r, px, py, pz);
+        const float v = eval_slots(tv, tv.entries[n.entry_first + j].program, r, px, py, pz);
         if (v < f.least) {
             f.least = v;
             f.arg = j;
@@ -149,19 +184,19 @@ KDN_D void land(const TreeView& tv, const NodeDev& n, uint32_t nodeIndex, uint32
 
 // The winner of a landing: the least value, ties going to the least
 // value a little further along (sphereWinner, sphere_trace.cpp:103-113).
-KDN_D uint32_t winner_at(const TreeView& tv, const NodeDev& n, const Regs& r, const float o[3], const float d[3], float t,
+KDN_D uint32_t winner_at(const TreeView& tv, const NodeDev& n, const SlotRegs& r, const float o[3], const float d[3], float t,
                          float inset, const LeafField& f) {
     if (n.entry_count == 1) return 0;
     const float px = o[0] + d[0] * t, py = o[1] + d[1] * t, pz = o[2] + d[2] * t;
     const float band = kTie * (1 + vmax3(__builtin_fabsf(px), __builtin_fabsf(py), __builtin_fabsf(pz)));
     const float qx = o[0] + d[0] * (t + inset), qy = o[1] + d[1] * (t + inset), qz = o[2] + d[2] * (t + inset);
     uint32_t winner = f.arg;
-    float winnerOn = eval_program(tv, tv.entries[n.entry_first + f.arg].program, r, qx, qy, qz);
+    float winnerOn = eval_slots(tv, tv.entries[n.entry_first + f.arg].program, r, qx, qy, qz);
     for (uint32_t j = 0; j < n.entry_count; ++j) {
         const uint32_t at = tv.entries[n.entry_first + j].program;
-        const float v = eval_program(tv, at, r, px, py, pz);
+        const float v = eval_slots(tv, at, r, px, py, pz);
         if (!(v - f.least <= band)) continue;
-        const float on = eval_program(tv, at, r, qx, qy, qz);
+        const float on = eval_slots(tv, at, r, qx, qy, qz);
         if (on < winnerOn) {
             winner = j;
             winnerOn = on;
@@ -186,7 +221,8 @@ KDN_D float slope_of(float part) {
     return 1.0f / (__builtin_fabsf(part) < m ? (part < 0 ? -m : m) : part);
 }
 
-KDN_D void trace_ray(const TreeView& tv, const kdn_ray& ray, Dl* buf, kdn_hit& hit) {
+KDN_D void march_ray(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr, kdn_hit& hit, int& iterations) {
+    Dl* const buf = lr.ssa;
     const float o[3] = {ray.origin[0], ray.origin[1], ray.origin[2]};
     const float d[3] = {ray.direction[0], ray.direction[1], ray.direction[2]};
     const bool inside = ray.inside != 0;
@@ -212,9 +248,8 @@ KDN_D void trace_ray(const TreeView& tv, const kdn_ray& ray, Dl* buf, kdn_hit& h
         const float q = __builtin_floorf((p - tv.lower[k]) / finest[k]);
         ci[k] = q < 0 ? 0u : (q >= (float)N ? N - 1 : (uint32_t)q);
     }
-    const Regs r{buf};
+    const SlotRegs& r = lr.slots;
     float t = t0;
-    int iterations = 0;
     while (iterations < kMaxIterations) {
         // descend by integer coordinates
         uint32_t index = 0;
@@ -308,9 +343,10 @@ KDN_D void trace_ray(const TreeView& tv, const kdn_ray& ray, Dl* buf, kdn_hit& h
                 const uint32_t at = tv.entries[n.entry_first].program;
                 const kdn_step head = tv.pool[at];
                 if (((uint32_t)head.a | ((uint32_t)head.b << 16)) <= (uint32_t)kMaxRegs) {
-                    (void)eval_program(tv, at, r, hit.point[0], hit.point[1], hit.point[2]);
+                    const Regs ssa{buf};
+                    (void)eval_program(tv, at, ssa, hit.point[0], hit.point[1], hit.point[2]);
                     float sh;
-                    source_of(tv.pool + at + 1, head.c, r, &hit.leaf, &hit.other, &sh);
+                    source_of(tv.pool + at + 1, head.c, ssa, &hit.leaf, &hit.other, &sh);
                     hit.share = sh;
                 }
             }
@@ -350,6 +386,18 @@ KDN_D void trace_ray(const TreeView& tv, const kdn_ray& ray, Dl* buf, kdn_hit& h
         if (next[axis] >= N) return;
         ci[0] = next[0]; ci[1] = next[1]; ci[2] = next[2];
     }
+}
+
+// The extend step of one ray: its hit (missed when it leaves the region or
+// its span). KDN_TRACE_ITERATIONS sees the cells and march steps the ray
+// took (a hook of the CPU build; nothing on the device).
+#ifndef KDN_TRACE_ITERATIONS
+#define KDN_TRACE_ITERATIONS(n) ((void)0)
+#endif
+KDN_D void trace_ray(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr, kdn_hit& hit) {
+    int iterations = 0;
+    march_ray(tv, ray, lr, hit, iterations);
+    KDN_TRACE_ITERATIONS(iterations);
 }
 
 }  // namespace kdn

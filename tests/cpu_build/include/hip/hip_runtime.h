@@ -24,6 +24,13 @@
     std::abort();
 }
 #define KDN_CHECK(cond) ((cond) ? (void)0 : kdn_emu_fail(#cond, __FILE__, __LINE__))
+// A histogram of the cells plus march steps each traced ray took
+// (bucket b counts rays with 2^(b-1) < iterations <= 2^b), read through
+// kdn_emu_iterations().
+inline uint64_t kdn_emu_histogram[16];
+#define KDN_TRACE_ITERATIONS(n) (++kdn_emu_histogram[(n) <= 1 ? 0 : (32 - __builtin_clz((unsigned)(n) - 1)) & 15])
+extern "C" __attribute__((used, visibility("default"))) inline const uint64_t* kdn_emu_iterations() { return kdn_emu_histogram; }
+
 // this build runs glass scattering, which the device build holds back
 // (-DKDN_EMU_AS_DEVICE keeps the device build's refusal, to test it)
 #ifndef KDN_EMU_AS_DEVICE
@@ -35,6 +42,9 @@
 #define __global__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+// a block's shared memory: one array for every thread (threads run one
+// after another here, each in its own column)
+#define __shared__ static
 
 struct dim3 {
     uint32_t x, y, z;
