@@ -328,17 +328,22 @@ __global__ void k_surface_program_max(const NodeDev* nodes, uint32_t n, const En
     atomicMax(most, longest);
 }
 
-// Slot registers for every entry's program (one lane per entry): a step's
-// value takes the lowest free slot once the operands whose last use it is
-// have given theirs back, so a program needs as many slots as it keeps
-// values live at once. The result lives to the end. Writes the program on
-// slots at the same offsets of `spool`; the destination slot goes in the
-// step's flags. A program that needs more than kSlots raises *overflow.
+// Slot registers for every program an entry names (one lane per entry): a
+// step's value takes the lowest free slot once the operands whose last use
+// it is have given theirs back, so a program needs as many slots as it
+// keeps values live at once. The result lives to the end. Writes the
+// program on slots at the same offsets of `spool`; the destination slot
+// goes in the step's flags. A program that needs more than kSlots raises
+// *overflow. Entries share programs (a cut that changes nothing keeps its
+// parent's), so the first lane to claim a program's bit in `claimed`
+// allocates it and the others leave it alone.
 __global__ void k_slot_programs(const EntryDev* entries, uint32_t nEntries, const kdn_step* pool, kdn_step* spool,
-                                uint16_t* lastUse, uint8_t* slotOf, uint32_t* overflow) {
+                                uint16_t* lastUse, uint8_t* slotOf, uint32_t* claimed, uint32_t* overflow) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nEntries) return;
     const uint32_t at = entries[e].program;
+    const uint32_t bit = 1u << (at & 31);
+    if (atomicOr(&claimed[at >> 5], bit) & bit) return;
     const kdn_step h = pool[at];
     const uint32_t L = (uint32_t)h.a | ((uint32_t)h.b << 16);
     const kdn_step* prog = pool + at + 1;
@@ -570,14 +575,17 @@ bool buildOctree(Ctx& c, const float lower[3], const float size[3], int depth, k
     {
         // the march's programs on slot registers
         if (!growKeep(c, t.spool, t.n_pool * sizeof(kdn_step), 0)) return false;
+        const size_t claimWords = (t.n_pool + 31) / 32;
         if (!ensure(bufS, t.n_pool * 3 + 256, c.error, false)) return false;
-        if (!ensure(bufC, 64, c.error, false)) return false;
+        if (!ensure(bufC, claimWords * 4 + 64, c.error, false)) return false;
         uint16_t* lastUse = (uint16_t*)bufS.ptr;
         uint8_t* slotOf = (uint8_t*)(lastUse + t.n_pool);
         uint32_t* overflow = (uint32_t*)bufC.ptr;
-        KDN_HIP(hipMemsetAsync(overflow, 0, 4, s));
+        uint32_t* claimed = overflow + 16;
+        KDN_HIP(hipMemsetAsync(bufC.ptr, 0, claimWords * 4 + 64, s));
         hipLaunchKernelGGL(k_slot_programs, dim3(blocksFor(t.n_entries)), dim3(256), 0, s, (const EntryDev*)t.entries.ptr,
-                           (uint32_t)t.n_entries, (const kdn_step*)t.pool.ptr, (kdn_step*)t.spool.ptr, lastUse, slotOf, overflow);
+                           (uint32_t)t.n_entries, (const kdn_step*)t.pool.ptr, (kdn_step*)t.spool.ptr, lastUse, slotOf, claimed,
+                           overflow);
         uint32_t longest = 0;
         KDN_HIP(hipMemcpyAsync(&longest, overflow, 4, hipMemcpyDeviceToHost, s));
         KDN_HIP(hipStreamSynchronize(s));

@@ -174,6 +174,12 @@ inline T atomicAdd(T* p, T v) {
     return old;
 }
 template <class T>
+inline T atomicOr(T* p, T v) {
+    const T old = *p;
+    *p = old | v;
+    return old;
+}
+template <class T>
 inline T atomicMax(T* p, T v) {
     const T old = *p;
     if (v > old) *p = v;
