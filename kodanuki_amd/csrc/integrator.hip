@@ -31,10 +31,11 @@ __global__ void k_generate(PassParams P, uint32_t slots) {
 __global__ void __launch_bounds__(kTraceBlock) k_trace_queue(TreeView tv, const uint32_t* queue, const uint32_t* count,
                                                              const kdn_ray* rays, kdn_hit* hits) {
     __shared__ float slots[kSlots * kTraceBlock];
+    __shared__ uint32_t paths[kPathLevels * kTraceBlock];
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= *count) return;
     Dl buf[kMaxRegs];
-    const LaneRegs lr{buf, SlotRegs{slots + threadIdx.x, kTraceBlock}};
+    const LaneRegs lr{buf, SlotRegs{slots + threadIdx.x, kTraceBlock}, PathRegs{paths + threadIdx.x, kTraceBlock}};
     trace_item(tv, queue[q], rays, hits, lr, nullptr);
 }
 
@@ -43,10 +44,11 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_queue(TreeView tv, const 
 __global__ void __launch_bounds__(kTraceBlock) k_trace_queue_count(TreeView tv, const uint32_t* queue, const uint32_t* count,
                                                                    const kdn_ray* rays, kdn_hit* hits, double* work) {
     __shared__ float slots[kSlots * kTraceBlock];
+    __shared__ uint32_t paths[kPathLevels * kTraceBlock];
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= *count) return;
     Dl buf[kMaxRegs];
-    const LaneRegs lr{buf, SlotRegs{slots + threadIdx.x, kTraceBlock}};
+    const LaneRegs lr{buf, SlotRegs{slots + threadIdx.x, kTraceBlock}, PathRegs{paths + threadIdx.x, kTraceBlock}};
     trace_item(tv, queue[q], rays, hits, lr, work);
 }
 
@@ -57,10 +59,11 @@ __global__ void __launch_bounds__(256) k_shade(PassParams P, uint32_t inQueue, u
 
 __global__ void __launch_bounds__(kTraceBlock) k_shadow(TreeView tv, PathState s) {
     __shared__ float slots[kSlots * kTraceBlock];
+    __shared__ uint32_t paths[kPathLevels * kTraceBlock];
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= s.counters[2]) return;
     Dl buf[kMaxRegs];
-    const LaneRegs lr{buf, SlotRegs{slots + threadIdx.x, kTraceBlock}};
+    const LaneRegs lr{buf, SlotRegs{slots + threadIdx.x, kTraceBlock}, PathRegs{paths + threadIdx.x, kTraceBlock}};
     shadow_one(tv, s, q, lr);
 }
 

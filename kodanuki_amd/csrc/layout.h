@@ -25,8 +25,10 @@ constexpr int kMaxRegs = 384;
 // Slot registers of the march's LDS register file per lane: the most
 // values a program keeps live at once after the build's allocation (the
 // build fails beyond it), and the trace kernels' block size.
-constexpr int kSlots = 32;
+constexpr int kSlots = 16;
 constexpr int kTraceBlock = 256;
+// Octree levels a march keeps the ancestry of (depth caps 0..14).
+constexpr int kPathLevels = 16;
 
 enum NodeKind : uint32_t { KindEmpty = 0, KindFilled = 1, KindSurface = 2, KindBranch = 3 };
 

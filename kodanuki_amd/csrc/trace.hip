@@ -8,10 +8,11 @@ namespace kdn {
 
 __global__ void __launch_bounds__(kTraceBlock) k_extend(TreeView tv, const kdn_ray* rays, uint32_t n, kdn_hit* hits) {
     __shared__ float slots[kSlots * kTraceBlock];
+    __shared__ uint32_t paths[kPathLevels * kTraceBlock];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Dl buf[kMaxRegs];
-    const LaneRegs lr{buf, SlotRegs{slots + threadIdx.x, kTraceBlock}};
+    const LaneRegs lr{buf, SlotRegs{slots + threadIdx.x, kTraceBlock}, PathRegs{paths + threadIdx.x, kTraceBlock}};
     kdn_hit hit;
     trace_ray(tv, rays[i], lr, hit);
     hits[i] = hit;

@@ -68,12 +68,24 @@ struct SlotRegs {
     }
 };
 
+// A lane's ancestry in the octree: the node index at each level of the
+// current leaf's path from the root (in LDS, one column per lane).
+struct PathRegs {
+    uint32_t* s;
+    uint32_t stride;
+    KDN_D uint32_t& operator[](uint32_t level) const {
+        KDN_CHECK(level < (uint32_t)kPathLevels);
+        return s[level * stride];
+    }
+};
+
 // A lane's register files: the slot registers every march evaluation runs
-// in, and the private SSA registers of a landing's dual evaluation and
-// source walk.
+// in, its octree ancestry, and the private SSA registers of a landing's
+// dual evaluation and source walk.
 struct LaneRegs {
     Dl* ssa;
     SlotRegs slots;
+    PathRegs path;
 };
 
 // A program on slot registers (the build's allocation, same offsets as the
@@ -249,17 +261,22 @@ KDN_D void march_ray(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr,
         ci[k] = q < 0 ? 0u : (q >= (float)N ? N - 1 : (uint32_t)q);
     }
     const SlotRegs& r = lr.slots;
+    const PathRegs& path = lr.path;
     float t = t0;
+    // the deepest known ancestor of the next cell: the root at first, then
+    // the lowest common ancestor of the cell left and the cell entered
+    int lvl = 0;
+    path[0] = 0;
     while (iterations < kMaxIterations) {
-        // descend by integer coordinates
-        uint32_t index = 0;
-        NodeDev n = tv.nodes[0];
-        int lvl = 0;
+        // descend by integer coordinates from that ancestor
+        uint32_t index = path[(uint32_t)lvl];
+        NodeDev n = tv.nodes[index];
         while (n.kind == KindBranch) {
             ++lvl;
             const int sh = D - lvl;
             const uint32_t oct = ((ci[0] >> sh) & 1u) | (((ci[1] >> sh) & 1u) << 1) | (((ci[2] >> sh) & 1u) << 2);
             index = n.first_child + oct;
+            path[(uint32_t)lvl] = index;
             n = tv.nodes[index];
         }
         const uint32_t nc[3] = {n.x, n.y, n.z};
@@ -384,6 +401,11 @@ KDN_D void march_ray(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr,
             }
         }
         if (next[axis] >= N) return;
+        // levels whose cell holds both the cell left and the one entered:
+        // those whose shift passes every bit where their coordinates differ
+        const uint32_t differ = (next[0] ^ ci[0]) | (next[1] ^ ci[1]) | (next[2] ^ ci[2]);
+        const int shared = D - (32 - __builtin_clz(differ));
+        lvl = shared < lvl ? shared : lvl;
         ci[0] = next[0]; ci[1] = next[1]; ci[2] = next[2];
     }
 }
