@@ -41,11 +41,16 @@ def cpu_baseline(scene, lower, size, width, height, seed, surfaces, budget_s=12.
     probe = time.perf_counter() - t0
     per_sample = probe / (side * side)
     n = max(16, int(budget_s / max(per_sample, 1e-6)))
-    side = max(8, min(256, int(n ** 0.5)))
-    window = (cx - side // 2, cy - side // 2, cx - side // 2 + side, cy - side // 2 + side)
-    t0 = time.perf_counter()
-    O.render64(scene, A, lower, size, width, height, 1, seed=seed, surfaces=surfaces, window=window, threads=threads)
-    dt = time.perf_counter() - t0
+    side = max(8, min(512, int(n ** 0.5)))
+    # time the window; grow it once if it finished well inside the budget
+    for _ in range(2):
+        window = (cx - side // 2, cy - side // 2, cx - side // 2 + side, cy - side // 2 + side)
+        t0 = time.perf_counter()
+        O.render64(scene, A, lower, size, width, height, 1, seed=seed, surfaces=surfaces, window=window, threads=threads)
+        dt = time.perf_counter() - t0
+        if dt > budget_s / 2 or side >= 512:
+            break
+        side = min(512, int(side * (budget_s / max(dt, 1e-3)) ** 0.5))
     samples = side * side
     return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
             "sample": f"{side}x{side} centre window of the same 1920x1080 image, 1 spp, FP64 oracle renderer "
@@ -127,8 +132,16 @@ def main():
     flops_per_launch = counted.trace_flops / max(counted.trace_launches, 1)
     ms_per_launch = first_stats.trace_ms / max(first_stats.trace_launches, 1)
     achieved = flops_per_launch / (ms_per_launch * 1e-3) / 1e12
+    # HBM bytes per launch of the same kernel, from the counter runs whose
+    # summary is committed beside the kernel statistics (scaled FETCH_SIZE
+    # plus WRITE_SIZE)
+    traffic = None
+    traffic_file = os.path.join(ROOT, "profiles", "k_trace_queue_traffic.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            traffic = json.load(f).get("traffic_bytes_per_launch")
     roofline = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
                 "kernel": "k_trace_queue (extend march, FP32 vector work)",
                 "flops_per_launch": flops_per_launch, "launch_ms": ms_per_launch,
                 "kernel_ms_share": {"trace": first_stats.trace_ms, "shade": first_stats.shade_ms,

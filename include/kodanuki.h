@@ -290,8 +290,10 @@ int kdn_extend_device(kdn_ctx* ctx, const void* rays, uint32_t count, void* hits
 /*
  * One pixel pass: `spp` samples of every pixel, from sample index
  * `first_sample`, through the wavefront integrator (ray generation,
- * extend, material-sorted shading, NEE shadow rays, MIS, hero-wavelength
- * throughput, roulette), folded into FP64 accumulators. Stands in for the
+ * extend, shading, NEE shadow rays, MIS, hero-wavelength throughput,
+ * roulette), folded into FP64 accumulators. On the device the pass
+ * refuses scenes painted with glass (KDN_E_STATE): glass scattering is
+ * held back there until a device fault of its renders is explained. Stands in for the
  * Backend hook the reference lacks (a whole pixel pass per call) over the
  * Transport steps extendRays → arriveAtHits → connectLights →
  * scatterWalks → fold (core/worker.cpp:31-39,
