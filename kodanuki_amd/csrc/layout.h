@@ -30,6 +30,22 @@ constexpr int kTraceBlock = 256;
 // Octree levels a march keeps the ancestry of (depth caps 0..14).
 constexpr int kPathLevels = 16;
 
+// A step of a program on slot registers, 8 bytes: code = op (bits 0-5) |
+// destination slot (6-9) | operand slots a (10-13), b (14-17), c (18-21);
+// value = the constant. A program's header has code = length (bits 0-23) |
+// result slot (24-27) and value = its FP32 operation count.
+struct SlotStep {
+    uint32_t code;
+    float value;
+};
+
+// Steps fetched at once by an evaluation (independent loads in flight
+// together), and the steps every program pool keeps readable past its
+// last program so that a fetch may run over the end.
+constexpr int kSlotFetch = 8;
+constexpr int kSsaFetch = 4;
+constexpr int kPoolPad = 16;
+
 enum NodeKind : uint32_t { KindEmpty = 0, KindFilled = 1, KindSurface = 2, KindBranch = 3 };
 
 struct EntryDev {

@@ -332,12 +332,11 @@ __global__ void k_surface_program_max(const NodeDev* nodes, uint32_t n, const En
 // step's value takes the lowest free slot once the operands whose last use
 // it is have given theirs back, so a program needs as many slots as it
 // keeps values live at once. The result lives to the end. Writes the
-// program on slots at the same offsets of `spool`; the destination slot
-// goes in the step's flags. A program that needs more than kSlots raises
-// *overflow. Entries share programs (a cut that changes nothing keeps its
+// program as 8-byte slot steps at the same offsets of `spool`. A program
+// that needs more than kSlots raises *overflow. Entries share programs (a cut that changes nothing keeps its
 // parent's), so the first lane to claim a program's bit in `claimed`
 // allocates it and the others leave it alone.
-__global__ void k_slot_programs(const EntryDev* entries, uint32_t nEntries, const kdn_step* pool, kdn_step* spool,
+__global__ void k_slot_programs(const EntryDev* entries, uint32_t nEntries, const kdn_step* pool, SlotStep* spool,
                                 uint16_t* lastUse, uint8_t* slotOf, uint32_t* claimed, uint32_t* overflow) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nEntries) return;
@@ -360,7 +359,7 @@ __global__ void k_slot_programs(const EntryDev* entries, uint32_t nEntries, cons
     }
     if (L > 0) last[h.c] = kKept;
     uint32_t freeSlots = kSlots >= 32 ? 0xFFFFFFFFu : ((1u << kSlots) - 1);
-    kdn_step* out = spool + at + 1;
+    SlotStep* out = spool + at + 1;
     for (uint32_t i = 0; i < L; ++i) {
         const kdn_step st = prog[i];
         const int n = operand_count(st.op);
@@ -374,17 +373,11 @@ __global__ void k_slot_programs(const EntryDev* entries, uint32_t nEntries, cons
         const uint32_t d = (uint32_t)__builtin_ctz(freeSlots);
         freeSlots &= ~(1u << d);
         slot[i] = (uint8_t)d;
-        kdn_step o = st;
-        o.flags = (uint8_t)d;
-        o.a = n > 0 ? slot[st.a] : 0;
-        o.b = n > 1 ? slot[st.b] : 0;
-        o.c = n > 2 ? slot[st.c] : 0;
-        out[i] = o;
+        const uint32_t a = n > 0 ? slot[st.a] : 0, b = n > 1 ? slot[st.b] : 0, cc = n > 2 ? slot[st.c] : 0;
+        out[i] = SlotStep{(uint32_t)st.op | (d << 6) | (a << 10) | (b << 14) | (cc << 18), st.value};
         if (last[i] == kUnread) freeSlots |= 1u << d;  // a value nothing reads
     }
-    kdn_step oh = h;
-    oh.c = L > 0 ? slot[h.c] : 0;
-    spool[at] = oh;
+    spool[at] = SlotStep{L | ((uint32_t)(L > 0 ? slot[h.c] : 0) << 24), h.value};
 }
 
 static inline uint32_t blocksFor(uint64_t n, uint32_t t = 256) { return (uint32_t)((n + t - 1) / t); }
@@ -440,7 +433,7 @@ bool buildOctree(Ctx& c, const float lower[3], const float size[3], int depth, k
         hipLaunchKernelGGL(k_object_sizes, dim3(blocksFor(nObj)), dim3(256), 0, s, nObj, c.scene.objects, sizes);
         uint64_t total = 0;
         if (!scanU32(sizes, offs, nObj, &total, tmp, s, c.error)) return false;
-        if (!growKeep(c, t.pool, total * sizeof(kdn_step), 0)) return false;
+        if (!growKeep(c, t.pool, (total + kPoolPad) * sizeof(kdn_step), 0)) return false;
         if (!growKeep(c, t.entries, (size_t)nObj * sizeof(EntryDev), 0)) return false;
         if (!ensure(bufA, (size_t)nObj * (sizeof(ActEntry) + 8) + 64, c.error, false)) return false;
         ActEntry* act = (ActEntry*)bufA.ptr;
@@ -494,7 +487,7 @@ bool buildOctree(Ctx& c, const float lower[3], const float size[3], int depth, k
             c.error = "the octree program pool passes 2^32 steps";
             return false;
         }
-        if (!growKeep(c, t.pool, (t.n_pool + newSteps) * sizeof(kdn_step), t.n_pool * sizeof(kdn_step))) return false;
+        if (!growKeep(c, t.pool, (t.n_pool + newSteps + kPoolPad) * sizeof(kdn_step), t.n_pool * sizeof(kdn_step))) return false;
         if (!growKeep(c, t.entries, (t.n_entries + keptTotal) * sizeof(EntryDev), t.n_entries * sizeof(EntryDev))) return false;
         hipLaunchKernelGGL(k_emit, dim3(blocksFor(nItems)), dim3(256), 0, s, act, (uint32_t)nItems, octants,
                            (const EntryDev*)t.entries.ptr, scratchOffsets, sts, als, items, keep, poolOffsets, t.n_pool,
@@ -574,7 +567,7 @@ bool buildOctree(Ctx& c, const float lower[3], const float size[3], int depth, k
     }
     {
         // the march's programs on slot registers
-        if (!growKeep(c, t.spool, t.n_pool * sizeof(kdn_step), 0)) return false;
+        if (!growKeep(c, t.spool, (t.n_pool + kPoolPad) * sizeof(SlotStep), 0)) return false;
         const size_t claimWords = (t.n_pool + 31) / 32;
         if (!ensure(bufS, t.n_pool * 3 + 256, c.error, false)) return false;
         if (!ensure(bufC, claimWords * 4 + 64, c.error, false)) return false;
@@ -584,7 +577,7 @@ bool buildOctree(Ctx& c, const float lower[3], const float size[3], int depth, k
         uint32_t* claimed = overflow + 16;
         KDN_HIP(hipMemsetAsync(bufC.ptr, 0, claimWords * 4 + 64, s));
         hipLaunchKernelGGL(k_slot_programs, dim3(blocksFor(t.n_entries)), dim3(256), 0, s, (const EntryDev*)t.entries.ptr,
-                           (uint32_t)t.n_entries, (const kdn_step*)t.pool.ptr, (kdn_step*)t.spool.ptr, lastUse, slotOf, claimed,
+                           (uint32_t)t.n_entries, (const kdn_step*)t.pool.ptr, (SlotStep*)t.spool.ptr, lastUse, slotOf, claimed,
                            overflow);
         uint32_t longest = 0;
         KDN_HIP(hipMemcpyAsync(&longest, overflow, 4, hipMemcpyDeviceToHost, s));

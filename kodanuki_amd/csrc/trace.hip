@@ -23,7 +23,7 @@ TreeView treeView(const Ctx& c) {
     tv.nodes = (const NodeDev*)c.tree.nodes.ptr;
     tv.entries = (const EntryDev*)c.tree.entries.ptr;
     tv.pool = (const kdn_step*)c.tree.pool.ptr;
-    tv.spool = (const kdn_step*)c.tree.spool.ptr;
+    tv.spool = (const SlotStep*)c.tree.spool.ptr;
     tv.leafMaterial = c.scene.leaf_material;
     for (int k = 0; k < 3; ++k) {
         tv.lower[k] = c.tree.lower[k];

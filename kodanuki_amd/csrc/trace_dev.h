@@ -34,7 +34,7 @@ struct TreeView {
     const NodeDev* nodes;
     const EntryDev* entries;
     const kdn_step* pool;
-    const kdn_step* spool;  // the pool's programs on slot registers, same offsets
+    const SlotStep* spool;  // the pool's programs on slot registers, same offsets
     const uint32_t* leafMaterial;
     float lower[3];
     float size[3];
@@ -89,18 +89,63 @@ struct LaneRegs {
 };
 
 // A program on slot registers (the build's allocation, same offsets as the
-// SSA pool): each step writes its value to the slot in its flags field.
+// SSA pool). Its steps are fetched kSlotFetch at a time, the header with
+// the first of them: the loads of a batch are in flight together, so a
+// lane waits once per batch instead of once per step (the pools keep
+// kPoolPad steps readable past their end for the last batch).
 KDN_D float eval_slots(const TreeView& tv, uint32_t at, const SlotRegs& r, float px, float py, float pz) {
-    const kdn_step h = tv.spool[at];
-    if (tv.work) *tv.work += h.value;
-    const uint32_t L = (uint32_t)h.a | ((uint32_t)h.b << 16);
-    const kdn_step* prog = tv.spool + at + 1;
+    const SlotStep* p = tv.spool + at;
+    SlotStep q[kSlotFetch];
+#pragma unroll
+    for (int k = 0; k < kSlotFetch; ++k) q[k] = p[k];
+    p += kSlotFetch;
+    const SlotStep head = q[0];
+#pragma unroll
+    for (int k = 0; k + 1 < kSlotFetch; ++k) q[k] = q[k + 1];
+    uint32_t left = kSlotFetch - 1;
+    if (tv.work) *tv.work += head.value;
+    const uint32_t L = head.code & 0xFFFFFFu;
     for (uint32_t i = 0; i < L; ++i) {
-        const kdn_step st = prog[i];
-        const float v = point_step(st, r, px, py, pz);
-        r[st.flags] = v;
+        if (left == 0) {
+#pragma unroll
+            for (int k = 0; k < kSlotFetch; ++k) q[k] = p[k];
+            p += kSlotFetch;
+            left = kSlotFetch;
+        }
+        const uint32_t code = q[0].code;
+        kdn_step st;
+        st.op = (uint8_t)(code & 63u);
+        st.flags = 0;
+        st.a = (uint16_t)((code >> 10) & 15u);
+        st.b = (uint16_t)((code >> 14) & 15u);
+        st.c = (uint16_t)((code >> 18) & 15u);
+        st.value = q[0].value;
+        st.leaf = 0;
+        r[(code >> 6) & 15u] = point_step(st, r, px, py, pz);
+#pragma unroll
+        for (int k = 0; k + 1 < kSlotFetch; ++k) q[k] = q[k + 1];
+        --left;
     }
-    return r[h.c];
+    return r[head.code >> 24];
+}
+
+// SSA steps of a program, fetched kSsaFetch at a time (as eval_slots).
+template <class F>
+KDN_D void for_each_step(const kdn_step* prog, uint32_t L, F&& f) {
+    kdn_step q[kSsaFetch];
+    uint32_t left = 0;
+    for (uint32_t i = 0; i < L; ++i) {
+        if (left == 0) {
+#pragma unroll
+            for (int k = 0; k < kSsaFetch; ++k) q[k] = prog[k];
+            prog += kSsaFetch;
+            left = kSsaFetch;
+        }
+        f(i, q[0]);
+#pragma unroll
+        for (int k = 0; k + 1 < kSsaFetch; ++k) q[k] = q[k + 1];
+        --left;
+    }
 }
 
 KDN_D float eval_program(const TreeView& tv, uint32_t at, const Regs& r, float px, float py, float pz) {
@@ -108,8 +153,7 @@ KDN_D float eval_program(const TreeView& tv, uint32_t at, const Regs& r, float p
     const kdn_step h = pool[at];
     if (tv.work) *tv.work += h.value;
     const uint32_t L = (uint32_t)h.a | ((uint32_t)h.b << 16);
-    const kdn_step* prog = pool + at + 1;
-    for (uint32_t i = 0; i < L; ++i) r[i] = point_step(prog[i], r, px, py, pz);
+    for_each_step(pool + at + 1, L, [&](uint32_t i, const kdn_step& st) { r[i] = point_step(st, r, px, py, pz); });
     return r[h.c];
 }
 
@@ -118,8 +162,7 @@ KDN_D void eval_dual(const TreeView& tv, uint32_t at, const DRegs& r, float px, 
     const kdn_step h = pool[at];
     if (tv.work) *tv.work += 4 * h.value;
     const uint32_t L = (uint32_t)h.a | ((uint32_t)h.b << 16);
-    const kdn_step* prog = pool + at + 1;
-    for (uint32_t i = 0; i < L; ++i) r[i] = dual_step(prog[i], r, px, py, pz);
+    for_each_step(pool + at + 1, L, [&](uint32_t i, const kdn_step& st) { r[i] = dual_step(st, r, px, py, pz); });
     *out = r[h.c];
 }
 

@@ -28,7 +28,12 @@
 // (bucket b counts rays with 2^(b-1) < iterations <= 2^b), read through
 // kdn_emu_iterations().
 inline uint64_t kdn_emu_histogram[16];
-#define KDN_TRACE_ITERATIONS(n) (++kdn_emu_histogram[(n) <= 1 ? 0 : (32 - __builtin_clz((unsigned)(n) - 1)) & 15])
+inline int kdn_emu_launch_most = 0;  // the most any ray of the running launch took
+inline void kdn_emu_count_iterations(int n) {
+    ++kdn_emu_histogram[n <= 1 ? 0 : (32 - __builtin_clz((unsigned)n - 1)) & 15];
+    kdn_emu_launch_most = n > kdn_emu_launch_most ? n : kdn_emu_launch_most;
+}
+#define KDN_TRACE_ITERATIONS(n) kdn_emu_count_iterations(n)
 extern "C" __attribute__((used, visibility("default"))) inline const uint64_t* kdn_emu_iterations() { return kdn_emu_histogram; }
 
 // this build runs glass scattering, which the device build holds back
@@ -136,8 +141,27 @@ inline bool kdn_emu_reverse() {
     return reverse;
 }
 
+// KDN_EMU_LAUNCHES=1 prints, after every launch that traced rays, its
+// thread count and the most cells plus march steps one ray took.
+inline bool kdn_emu_print_launches() {
+    static const bool on = std::getenv("KDN_EMU_LAUNCHES") != nullptr;
+    return on;
+}
+
+template <class... P, class... A>
+inline void kdn_emu_run(void (*kernel)(P...), dim3 grid, dim3 block, A&&... args);
+
 template <class... P, class... A>
 inline void kdn_emu_launch(void (*kernel)(P...), dim3 grid, dim3 block, A&&... args) {
+    kdn_emu_launch_most = 0;
+    kdn_emu_run(kernel, grid, block, args...);
+    if (kdn_emu_print_launches() && kdn_emu_launch_most > 0) {
+        std::fprintf(stderr, "[launch] %u threads, most iterations %d\n", grid.x * block.x, kdn_emu_launch_most);
+    }
+}
+
+template <class... P, class... A>
+inline void kdn_emu_run(void (*kernel)(P...), dim3 grid, dim3 block, A&&... args) {
     gridDim = grid;
     blockDim = block;
     const uint64_t blocks = (uint64_t)grid.x * grid.y * grid.z, threads = (uint64_t)block.x * block.y * block.z;
