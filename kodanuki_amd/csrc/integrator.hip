@@ -28,7 +28,7 @@ __global__ void k_generate(PassParams P, uint32_t slots) {
     if (i < slots) generate_one(P, i);
 }
 
-__global__ void __launch_bounds__(kTraceBlock) k_trace_queue(TreeView tv, const uint32_t* queue, const uint32_t* count,
+__global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_trace_queue(TreeView tv, const uint32_t* queue, const uint32_t* count,
                                                              const kdn_ray* rays, kdn_hit* hits) {
     __shared__ float slots[kSlots * kTraceBlock];
     __shared__ uint32_t paths[kPathLevels * kTraceBlock];
@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_queue(TreeView tv, const 
 
 // The same march counting the FP32 operations of every evaluation (the
 // instrumented pass the roofline's algorithmic work comes from).
-__global__ void __launch_bounds__(kTraceBlock) k_trace_queue_count(TreeView tv, const uint32_t* queue, const uint32_t* count,
+__global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_trace_queue_count(TreeView tv, const uint32_t* queue, const uint32_t* count,
                                                                    const kdn_ray* rays, kdn_hit* hits, double* work) {
     __shared__ float slots[kSlots * kTraceBlock];
     __shared__ uint32_t paths[kPathLevels * kTraceBlock];
@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(256) k_shade(PassParams P, uint32_t inQueue, u
     if (q < P.s.counters[inQueue]) shade_one(P, q, inQueue, outQueue);
 }
 
-__global__ void __launch_bounds__(kTraceBlock) k_shadow(TreeView tv, PathState s) {
+__global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_shadow(TreeView tv, PathState s) {
     __shared__ float slots[kSlots * kTraceBlock];
     __shared__ uint32_t paths[kPathLevels * kTraceBlock];
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;

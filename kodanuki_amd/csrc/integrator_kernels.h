@@ -299,7 +299,7 @@ KDN_D void shadow_one(const TreeView& tv, PathState& s, uint32_t q, const LaneRe
     TreeView t = tv;
     double w = 0;
     if (tv.work) t.work = &w;
-    trace_ray(t, r, lr, h);
+    occlusion_ray(t, r, lr, h);
     if (tv.work) atomicAdd(tv.work, w);
     if (h.kind == KDN_HIT_MISSED || !(h.distance < r.far_)) {
         const size_t N = s.capacity;

@@ -27,6 +27,10 @@ constexpr int kMaxRegs = 384;
 // build fails beyond it), and the trace kernels' block size.
 constexpr int kSlots = 16;
 constexpr int kTraceBlock = 256;
+// Waves per SIMD the trace kernels are compiled for: the march is bound by
+// the latency of its dependent fetches, so more waves in flight hide more
+// of it (the LDS of a block, 32 KB, allows five).
+constexpr int kTraceWaves = 5;
 // Octree levels a march keeps the ancestry of (depth caps 0..14).
 constexpr int kPathLevels = 16;
 

@@ -6,7 +6,7 @@
 
 namespace kdn {
 
-__global__ void __launch_bounds__(kTraceBlock) k_extend(TreeView tv, const kdn_ray* rays, uint32_t n, kdn_hit* hits) {
+__global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_extend(TreeView tv, const kdn_ray* rays, uint32_t n, kdn_hit* hits) {
     __shared__ float slots[kSlots * kTraceBlock];
     __shared__ uint32_t paths[kPathLevels * kTraceBlock];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -276,6 +276,11 @@ KDN_D float slope_of(float part) {
     return 1.0f / (__builtin_fabsf(part) < m ? (part < 0 ? -m : m) : part);
 }
 
+// AnyHit: a shadow test, which only asks whether a surface lies on the ray
+// before its far end: the march stops where it first comes within its
+// tolerance of a surface (or enters a filled cell) and reports that point,
+// with no tie-break, landing or source.
+template <bool AnyHit>
 KDN_D void march_ray(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr, kdn_hit& hit, int& iterations) {
     Dl* const buf = lr.ssa;
     const float o[3] = {ray.origin[0], ray.origin[1], ray.origin[2]};
@@ -348,6 +353,12 @@ KDN_D void march_ray(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr,
                 float step;
                 if (!inside) {
                     if (f.least < tol) {
+                        if (AnyHit) {
+                            hit.kind = KDN_HIT_SURFACE;
+                            hit.distance = t;
+                            hit.cell = index;
+                            return;
+                        }
                         const uint32_t w = winner_at(tv, n, r, o, d, t, inset, f);
                         land(tv, n, index, w, o, d, t, false, buf, hit);
                         return;
@@ -399,7 +410,7 @@ KDN_D void march_ray(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr,
             // the solid's leaf from its program where the register file
             // holds it (a filled cell's program may be longer than any
             // surface leaf's; the build checks surface leaves only)
-            if (n.entry_count > 0) {
+            if (!AnyHit && n.entry_count > 0) {
                 const uint32_t at = tv.entries[n.entry_first].program;
                 const kdn_step head = tv.pool[at];
                 if (((uint32_t)head.a | ((uint32_t)head.b << 16)) <= (uint32_t)kMaxRegs) {
@@ -461,7 +472,15 @@ KDN_D void march_ray(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr,
 #endif
 KDN_D void trace_ray(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr, kdn_hit& hit) {
     int iterations = 0;
-    march_ray(tv, ray, lr, hit, iterations);
+    march_ray<false>(tv, ray, lr, hit, iterations);
+    KDN_TRACE_ITERATIONS(iterations);
+}
+
+// The shadow test of a ray: its first approach within the tolerance of a
+// surface (kind SURFACE, distance) or MISSED.
+KDN_D void occlusion_ray(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr, kdn_hit& hit) {
+    int iterations = 0;
+    march_ray<true>(tv, ray, lr, hit, iterations);
     KDN_TRACE_ITERATIONS(iterations);
 }
 
