@@ -166,44 +166,6 @@ KDN_D void eval_dual(const TreeView& tv, uint32_t at, const DRegs& r, float px, 
     *out = r[h.c];
 }
 
-// Steps a march takes in one cell before it tries to prove the rest of its
-// segment there clear of surfaces (rays that graze a surface otherwise take
-// hundreds of tolerance-sized steps per cell).
-constexpr int kClearAfter = 8;
-
-struct IvRegs {
-    Iv* p;
-    KDN_D Iv& operator[](uint32_t i) const {
-        KDN_CHECK(i < (uint32_t)(2 * kMaxRegs));
-        return p[i];
-    }
-};
-
-// Whether no surface of leaf n comes within twice the tolerance of the
-// ray's segment [t, tEnd]: every entry's interval over a box that holds the
-// segment's FP32 sample points stays above 2 tol(tEnd). A march of the
-// segment then cannot land, and leaves the cell at its exit. The intervals
-// use the SSA programs and the lane's private registers.
-KDN_D bool segment_clear(const TreeView& tv, const NodeDev& n, Dl* buf, const float o[3], const float d[3], float t, float tEnd) {
-    Iv box[3];
-    for (int k = 0; k < 3; ++k) {
-        const float a = o[k] + d[k] * t, b = o[k] + d[k] * tEnd;
-        // a sample o + d t' rounds twice: widen by two ulps of its parts
-        const float slack = (__builtin_fabsf(o[k]) + __builtin_fabsf(d[k]) * tEnd) * 2.4e-7f;
-        box[k] = iv(down(fmin_(a, b) - slack), up(fmax_(a, b) + slack));
-    }
-    const float clear = 2 * (kTolAbs + kTolRel * tEnd);
-    const IvRegs R{reinterpret_cast<Iv*>(buf)};
-    for (uint32_t j = 0; j < n.entry_count; ++j) {
-        const uint32_t at = tv.entries[n.entry_first + j].program;
-        const kdn_step h = tv.pool[at];
-        const uint32_t L = (uint32_t)h.a | ((uint32_t)h.b << 16);
-        for_each_step(tv.pool + at + 1, L, [&](uint32_t i, const kdn_step& st) { R[i] = interval_step(st, R, box); });
-        if (!(R[h.c].lo > clear)) return false;
-    }
-    return true;
-}
-
 struct LeafField {
     float least;      // least value over the leaf's programs
     float nearest;    // least |value|
@@ -382,13 +344,8 @@ KDN_D void march_ray(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr,
         }
         const float tEnd = fmin_(tExit, t1);
         if (n.kind == KindSurface) {
-            int stepsHere = 0;
             while (t <= tEnd && iterations < kMaxIterations) {
                 ++iterations;
-                if (!inside && ++stepsHere == kClearAfter && segment_clear(tv, n, buf, o, d, t, tEnd)) {
-                    t = tExit;
-                    break;
-                }
                 const float px = o[0] + d[0] * t, py = o[1] + d[1] * t, pz = o[2] + d[2] * t;
                 const LeafField f = leaf_field(tv, n, r, px, py, pz);
                 const float tol = kTolAbs + kTolRel * t;
@@ -509,14 +466,14 @@ KDN_D void march_ray(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr,
 
 // The extend step of one ray: its hit (missed when it leaves the region or
 // its span). KDN_TRACE_ITERATIONS sees the cells and march steps the ray
-// took, the ray and its hit (a hook of the CPU build; nothing on the device).
+// took (a hook of the CPU build; nothing on the device).
 #ifndef KDN_TRACE_ITERATIONS
-#define KDN_TRACE_ITERATIONS(n, ray, hit) ((void)0)
+#define KDN_TRACE_ITERATIONS(n) ((void)0)
 #endif
 KDN_D void trace_ray(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr, kdn_hit& hit) {
     int iterations = 0;
     march_ray<false>(tv, ray, lr, hit, iterations);
-    KDN_TRACE_ITERATIONS(iterations, ray, hit);
+    KDN_TRACE_ITERATIONS(iterations);
 }
 
 // The shadow test of a ray: its first approach within the tolerance of a
@@ -524,7 +481,7 @@ KDN_D void trace_ray(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr,
 KDN_D void occlusion_ray(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr, kdn_hit& hit) {
     int iterations = 0;
     march_ray<true>(tv, ray, lr, hit, iterations);
-    KDN_TRACE_ITERATIONS(iterations, ray, hit);
+    KDN_TRACE_ITERATIONS(iterations);
 }
 
 }  // namespace kdn

@@ -1048,31 +1048,6 @@ typedef struct {
     uint32_t arg;
 } Field;
 
-/* Whether no surface of leaf n comes within twice the tolerance of the
- * ray's segment [t, tEnd] (the GPU march's segment_clear, operation for
- * operation): every entry's FP32 interval over a box holding the segment's
- * sample points stays above 2 tol(tEnd). */
-#define CLEAR_AFTER 8
-static int segment_clear(const View* v, const kdn_node_record* n, const float o[3], const float d[3], float t, float tEnd) {
-    Iv box[3];
-    for (int k = 0; k < 3; ++k) {
-        const float a = o[k] + d[k] * t, b = o[k] + d[k] * tEnd;
-        const float slack = (fabsf(o[k]) + fabsf(d[k]) * tEnd) * 2.4e-7f;
-        box[k].lo = down(fmin_(a, b) - slack);
-        box[k].hi = up(fmax_(a, b) + slack);
-    }
-    const float clear = 2 * (kTolAbs + kTolRel * tEnd);
-    Iv* R = (Iv*)v->buf;
-    for (uint32_t j = 0; j < n->entry_count; ++j) {
-        const uint32_t at = v->t->entries[n->entry_first + j].program;
-        const kdn_step* h = &v->t->pool[at];
-        const uint32_t L = header_length(h);
-        for (uint32_t i = 0; i < L; ++i) R[i] = interval_step(&h[1 + i], R, box);
-        if (!(R[h->c].lo > clear)) return 0;
-    }
-    return 1;
-}
-
 static Field leaf_field(const View* v, const kdn_node_record* n, float px, float py, float pz) {
     Field f = {KINF, KINF, 0};
     for (uint32_t j = 0; j < n->entry_count; ++j) {
@@ -1270,13 +1245,8 @@ static void trace_one(const View* v, const kdn_ray* ray, kdn_hit* hit) {
         }
         const float tEnd = fmin_(tExit, t1);
         if (n.kind == 2) {
-            int stepsHere = 0;
             while (t <= tEnd && iterations < MAX_ITER) {
                 ++iterations;
-                if (!inside && ++stepsHere == CLEAR_AFTER && segment_clear(v, &n, o, d, t, tEnd)) {
-                    t = tExit;
-                    break;
-                }
                 const float px = o[0] + d[0] * t, py = o[1] + d[1] * t, pz = o[2] + d[2] * t;
                 const Field f = leaf_field(v, &n, px, py, pz);
                 const float tol = kTolAbs + kTolRel * t;

@@ -29,22 +29,11 @@
 // kdn_emu_iterations().
 inline uint64_t kdn_emu_histogram[16];
 inline int kdn_emu_launch_most = 0;  // the most any ray of the running launch took
-// KDN_EMU_SLOW_RAYS=N prints every ray that takes more than N iterations.
-template <class Ray, class Hit>
-inline void kdn_emu_count_iterations(int n, const Ray& ray, const Hit& hit) {
+inline void kdn_emu_count_iterations(int n) {
     ++kdn_emu_histogram[n <= 1 ? 0 : (32 - __builtin_clz((unsigned)n - 1)) & 15];
     kdn_emu_launch_most = n > kdn_emu_launch_most ? n : kdn_emu_launch_most;
-    static const int slow = [] {
-        const char* v = std::getenv("KDN_EMU_SLOW_RAYS");
-        return v ? std::atoi(v) : 0;
-    }();
-    if (slow > 0 && n > slow) {
-        std::fprintf(stderr, "[slow ray] %d iterations: o (%.6g %.6g %.6g) d (%.6g %.6g %.6g) far %g inside %u -> kind %u t %.6g leaf %u\n", n,
-                     ray.origin[0], ray.origin[1], ray.origin[2], ray.direction[0], ray.direction[1], ray.direction[2], ray.far_,
-                     ray.inside, hit.kind, hit.distance, hit.leaf);
-    }
 }
-#define KDN_TRACE_ITERATIONS(n, ray, hit) kdn_emu_count_iterations(n, ray, hit)
+#define KDN_TRACE_ITERATIONS(n) kdn_emu_count_iterations(n)
 extern "C" __attribute__((used, visibility("default"))) inline const uint64_t* kdn_emu_iterations() { return kdn_emu_histogram; }
 
 // this build runs glass scattering, which the device build holds back
