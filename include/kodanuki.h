@@ -309,7 +309,9 @@ typedef struct kdn_pass_desc {
     uint32_t flags;         /* bit 0: keep the accumulators (progressive);
                                bit 1: time the kernels with HIP events;
                                bit 2: count the march's FP32 operations
-                                      (an instrumented, slower pass) */
+                                      (an instrumented, slower pass);
+                               bit 3: no finishing launch (every bounce a
+                                      wavefront launch to the end) */
 } kdn_pass_desc;
 
 typedef struct kdn_pass_stats {
@@ -324,6 +326,8 @@ typedef struct kdn_pass_stats {
     double other_ms;         /* generate and fold */
     double trace_flops;      /* FP32 operations of the extend march (bit 2) */
     double shadow_flops;
+    double finish_ms;        /* the finishing launch (paths to their ends) */
+    double finish_flops;     /* its march operations (bit 2) */
 } kdn_pass_stats;
 
 int kdn_render_pass(kdn_ctx* ctx, const kdn_pass_desc* desc, kdn_pass_stats* stats);

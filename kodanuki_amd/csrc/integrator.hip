@@ -67,6 +67,20 @@ __global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_shadow(TreeView tv
     shadow_one(tv, s, q, lr);
 }
 
+// The paths still in the input queue, each to its end in one lane (the
+// wavefront's last bounces carry few paths, and each launch of them costs
+// as long as its slowest ray).
+__global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_finish(TreeView tv, TreeView tvShadow, PassParams P, uint32_t inQueue,
+                                                                    uint32_t* tally, double* work) {
+    __shared__ float slots[kSlots * kTraceBlock];
+    __shared__ uint32_t paths[kPathLevels * kTraceBlock];
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= P.s.counters[inQueue]) return;
+    Dl buf[kMaxRegs];
+    const LaneRegs lr{buf, SlotRegs{slots + threadIdx.x, kTraceBlock}, PathRegs{paths + threadIdx.x, kTraceBlock}};
+    finish_one(tv, tvShadow, P, q, inQueue, lr, tally, work);
+}
+
 __global__ void k_fold(PassParams P, double* accum, double* accumSq, uint32_t* accumN) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     fold_one(P, t, accum, accumSq, accumN);
@@ -240,17 +254,19 @@ bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
         }
         return true;
     };
+    const bool finishing = (d.flags & 8) == 0;
     double* work = nullptr;
     if (counting) {
         if (!ensure(c.scratch[7], 64, c.error, false)) return false;
         work = (double*)c.scratch[7].ptr;
-        KDN_HIP(hipMemsetAsync(work, 0, 16, st));
+        KDN_HIP(hipMemsetAsync(work, 0, 24, st));
     }
     TreeView tvShadow = tv;
     tvShadow.work = counting ? work + 1 : nullptr;
 
     uint64_t segments = 0, shadows = 0;
     uint32_t bounces = 0, launches = 0;
+    double finishMs = 0;
     for (uint32_t first = 0; first < d.spp; first += chunk) {
         const uint32_t n = std::min(chunk, d.spp - first);
         P.first_sample = d.first_sample + first;
@@ -264,6 +280,21 @@ bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
             KDN_HIP(hipMemcpyAsync(&count, &s->counters[in], 4, hipMemcpyDeviceToHost, st));
             KDN_HIP(hipStreamSynchronize(st));
             if (count == 0) break;
+            if (finishing && count <= kFinishBelow) {
+                // the remaining paths to their ends in one launch
+                KDN_HIP(hipMemsetAsync(&s->counters[4], 0, 8, st));
+                timed(&finishMs, [&] {
+                    hipLaunchKernelGGL(k_finish, dim3((count + kTraceBlock - 1) / kTraceBlock), dim3(kTraceBlock), 0, st, tv, tvShadow,
+                                       P, in, &s->counters[4], counting ? work + 2 : nullptr);
+                });
+                uint32_t tally[2] = {0, 0};
+                KDN_HIP(hipMemcpyAsync(tally, &s->counters[4], 8, hipMemcpyDeviceToHost, st));
+                KDN_HIP(hipStreamSynchronize(st));
+                segments += tally[0];
+                shadows += tally[1];
+                ++bounces;
+                break;
+            }
             segments += count;
             ++bounces;
             ++launches;
@@ -292,8 +323,8 @@ bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
     }
     KDN_HIP(hipStreamSynchronize(st));
     KDN_HIP(hipGetLastError());
-    double counted[2] = {0, 0};
-    if (counting) KDN_HIP(hipMemcpy(counted, work, 16, hipMemcpyDeviceToHost));
+    double counted[3] = {0, 0, 0};
+    if (counting) KDN_HIP(hipMemcpy(counted, work, 24, hipMemcpyDeviceToHost));
     if (timing) {
         KDN_HIP(hipEventDestroy(ev[0]));
         KDN_HIP(hipEventDestroy(ev[1]));
@@ -311,6 +342,8 @@ bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
         stats->other_ms = otherMs;
         stats->trace_flops = counted[0];
         stats->shadow_flops = counted[1];
+        stats->finish_ms = finishMs;
+        stats->finish_flops = counted[2];
     }
     return true;
 }

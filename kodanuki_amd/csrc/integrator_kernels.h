@@ -128,11 +128,13 @@ KDN_D float lift_at(V3 x) {
     return 8e-5f + 8e-6f * fmax_(fmax_(__builtin_fabsf(x.x), __builtin_fabsf(x.y)), __builtin_fabsf(x.z));
 }
 
-// shade: queue entry q of the input queue
-KDN_D void shade_one(PassParams& P, uint32_t q, uint32_t inQueue, uint32_t outQueue) {
+// shade path i at its hit: emission met, a shadow test toward one lamp
+// (its ray in s.shadow[i] and what it carries in s.srad; *shadow set) and
+// the scattered ray that continues the path (in s.rays[i]; returns true).
+KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
     PathState& s = P.s;
     const size_t N = s.capacity;
-    const uint32_t i = s.queue[inQueue][q];
+    *shadow = false;
     const kdn_hit hit = s.hits[i];
     const kdn_ray ray = s.rays[i];
     const uint64_t stream = s.stream[i];
@@ -227,8 +229,7 @@ KDN_D void shade_one(PassParams& P, uint32_t q, uint32_t inQueue, uint32_t outQu
                     sr.within = KDN_NO_MATERIAL;
                     s.shadow[i] = sr;
                     for (int l = 0; l < kLanes; ++l) s.srad[l * N + i] = c[l];
-                    const uint32_t at = atomicAdd(&s.counters[2], 1u);
-                    s.queue[2][at] = i;
+                    *shadow = true;
                 }
             }
         }
@@ -285,15 +286,29 @@ KDN_D void shade_one(PassParams& P, uint32_t q, uint32_t inQueue, uint32_t outQu
         s.thr[l * N + i] = thr[l];
     }
     s.meta[i] = meta;
+    return alive;
+}
+
+// shade: queue entry q of the input queue; shadow tests and live paths
+// are appended to their queues
+KDN_D void shade_one(PassParams& P, uint32_t q, uint32_t inQueue, uint32_t outQueue) {
+    PathState& s = P.s;
+    const uint32_t i = s.queue[inQueue][q];
+    bool shadow;
+    const bool alive = shade_path(P, i, &shadow);
+    if (shadow) {
+        const uint32_t at = atomicAdd(&s.counters[2], 1u);
+        s.queue[2][at] = i;
+    }
     if (alive) {
         const uint32_t at = atomicAdd(&s.counters[outQueue], 1u);
         s.queue[outQueue][at] = i;
     }
 }
 
-// shadow: entry q of the shadow queue
-KDN_D void shadow_one(const TreeView& tv, PathState& s, uint32_t q, const LaneRegs& lr) {
-    const uint32_t i = s.queue[2][q];
+// the shadow test of path i: what it carries reaches the path when nothing
+// lies on its ray
+KDN_D void shadow_path(const TreeView& tv, PathState& s, uint32_t i, const LaneRegs& lr) {
     const kdn_ray r = s.shadow[i];
     kdn_hit h;
     TreeView t = tv;
@@ -305,6 +320,36 @@ KDN_D void shadow_one(const TreeView& tv, PathState& s, uint32_t q, const LaneRe
         const size_t N = s.capacity;
         for (int l = 0; l < kLanes; ++l) s.rad[l * N + i] += s.srad[l * N + i];
     }
+}
+
+// shadow: entry q of the shadow queue
+KDN_D void shadow_one(const TreeView& tv, PathState& s, uint32_t q, const LaneRegs& lr) {
+    shadow_path(tv, s, s.queue[2][q], lr);
+}
+
+// finish: entry q of the input queue, its path to the end in one lane:
+// extend, shade and the shadow test in turn until the path stops (shading
+// stops every path at its last surface). The same per-path arithmetic as
+// the wavefront's launches, so the estimates do not change; counts
+// segments and shadow tests in tally[0] and tally[1].
+KDN_D void finish_one(const TreeView& tv, const TreeView& tvShadow, PassParams& P, uint32_t q, uint32_t inQueue, const LaneRegs& lr,
+                      uint32_t* tally, double* work) {
+    PathState& s = P.s;
+    const uint32_t i = s.queue[inQueue][q];
+    uint32_t segments = 0, shadows = 0;
+    for (;;) {
+        trace_item(tv, i, s.rays, s.hits, lr, work);
+        ++segments;
+        bool shadow;
+        const bool alive = shade_path(P, i, &shadow);
+        if (shadow) {
+            shadow_path(tvShadow, s, i, lr);
+            ++shadows;
+        }
+        if (!alive) break;
+    }
+    atomicAdd(&tally[0], segments);
+    atomicAdd(&tally[1], shadows);
 }
 
 // Fold the chunk's samples of each pixel into the FP64 accumulators, in

@@ -59,3 +59,27 @@ def test_scene_settings_round_trip():
     assert (rs.width, rs.height, rs.samples) == (320, 200, 9)
     assert s.info.lights == 1
     assert s.info.leaves == 33
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """Every struct of include/kodanuki.h has the size its ctypes mirror has
+    (a C program built against the header prints them)."""
+    import ctypes as C
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("needs gcc")
+    names = {"kdn_step": k.Step, "kdn_object": None, "kdn_ray": k.Ray, "kdn_hit": k.Hit, "kdn_material": k.Material,
+             "kdn_light": k.Light, "kdn_leaf": k.Leaf, "kdn_node": k.Node, "kdn_camera": k.Camera,
+             "kdn_render_settings": k.RenderSettings, "kdn_scene_info": k.SceneInfo, "kdn_octree_stats": k.OctreeStats,
+             "kdn_node_record": k.NodeRecord, "kdn_entry_record": k.EntryRecord, "kdn_pass_desc": k.PassDesc,
+             "kdn_pass_stats": k.PassStats}
+    src = tmp_path / "sizes.c"
+    src.write_text('#include <stdio.h>\n#include "kodanuki.h"\nint main(void) {\n' +
+                   "".join(f'    printf("{n} %zu\\n", sizeof({n}));\n' for n in names) + "    return 0;\n}\n")
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    sizes = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines())
+    for n, t in names.items():
+        expect = k.OBJECT_DTYPE.itemsize if t is None else C.sizeof(t)
+        assert int(sizes[n]) == expect, (n, sizes[n], expect)

@@ -35,7 +35,7 @@ paint(m) at([0.3, 0.3, 0.3]) box([0.4, 0.4, 0.4], 0.05);
 """
 
 
-def _render(device, text, depth, spp, w=None, h=None, surfaces=None, seed=None, keep=False, first=0, min_depth=4):
+def _render(device, text, depth, spp, w=None, h=None, surfaces=None, seed=None, keep=False, first=0, min_depth=4, flags=0):
     s = k.Scene(text)
     cam, rs = s.settings()
     w, h = w or rs.width, h or rs.height
@@ -43,7 +43,7 @@ def _render(device, text, depth, spp, w=None, h=None, surfaces=None, seed=None, 
     device.upload(s)
     device.build_octree(lower, size, depth)
     st = device.render_pass(w, h, spp, first_sample=first, seed=seed or rs.seed, surfaces=surfaces or rs.surfaces,
-                            min_depth=min_depth, keep=keep)
+                            min_depth=min_depth, keep=keep, flags=flags)
     rgb, sq, n = device.accumulation(w, h)
     return s, lower, size, rgb, sq, n, st
 
@@ -92,6 +92,20 @@ def test_progressive_passes_equal_one_pass(device):
     _, _, _, b, _, nb, _ = _render(device, text, 5, 8, first=8, keep=True)
     assert np.array_equal(na, nb)
     assert np.allclose(a, b, rtol=0, atol=1e-9)
+
+
+def test_finishing_launch_changes_nothing(device):
+    """The paths finished in one launch get the same estimates, to the bit,
+    as bounce-by-bounce wavefront launches (same per-path arithmetic, folds
+    in sample order), and the same segment and shadow counts."""
+    text = scenes.tiny(glass=GLASS)
+    W, H, spp = 24, 18, 512
+    _, _, _, a, sa, na, st_a = _render(device, text, 5, spp, w=W, h=H)
+    _, _, _, b, sb, nb, st_b = _render(device, text, 5, spp, w=W, h=H, flags=k.PASS_NO_FINISH)
+    note(f"[finish] segments {st_a.segments} / {st_b.segments}, finish {st_a.finish_ms:.1f} ms")
+    assert np.array_equal(na, nb)
+    assert a.tobytes() == b.tobytes() and sa.tobytes() == sb.tobytes()
+    assert (st_a.segments, st_a.shadow_rays) == (st_b.segments, st_b.shadow_rays)
 
 
 def test_matches_fp64_oracle(device):
