@@ -145,7 +145,8 @@ def main():
                 "kernel": "k_trace_queue (extend march, FP32 vector work)",
                 "flops_per_launch": flops_per_launch, "launch_ms": ms_per_launch,
                 "kernel_ms_share": {"trace": first_stats.trace_ms, "shade": first_stats.shade_ms,
-                                    "shadow": first_stats.shadow_ms, "other": first_stats.other_ms}}
+                                    "shadow": first_stats.shadow_ms, "finish": first_stats.finish_ms,
+                                    "other": first_stats.other_ms}}
 
     cpu = None
     if not args.no_cpu:
