@@ -126,23 +126,34 @@ def main():
     samples = args.steps * W * H * S
     value = samples / elapsed / 1e6
 
-    # roofline of the extend march: counted work of one step over the mean
+    # roofline of the dominant march kernel of a step (the one with the
+    # most time): its counted FP32 operations per launch over the mean
     # launch time of the same step (the first timed step has the counted
     # pass's inputs: same samples, same rays)
-    flops_per_launch = counted.trace_flops / max(counted.trace_launches, 1)
-    ms_per_launch = first_stats.trace_ms / max(first_stats.trace_launches, 1)
+    kinds = {
+        "trace": ("k_trace_queue (extend march)", counted.trace_flops, counted.trace_launches, first_stats.trace_ms,
+                  first_stats.trace_launches),
+        "shadow": ("k_shadow (shadow-test march)", counted.shadow_flops, counted.shadow_launches, first_stats.shadow_ms,
+                   first_stats.shadow_launches),
+        "finish": ("k_finish (paths to their ends: march, shading, shadow tests)", counted.finish_flops,
+                   counted.finish_launches, first_stats.finish_ms, first_stats.finish_launches),
+    }
+    dominant = max(kinds, key=lambda kk: kinds[kk][3] if kinds[kk][4] else -1)
+    name, flops, flaunches, kms, klaunches = kinds[dominant]
+    flops_per_launch = flops / max(flaunches, 1)
+    ms_per_launch = kms / max(klaunches, 1)
     achieved = flops_per_launch / (ms_per_launch * 1e-3) / 1e12
     # HBM bytes per launch of the same kernel, from the counter runs whose
     # summary is committed beside the kernel statistics (scaled FETCH_SIZE
     # plus WRITE_SIZE)
     traffic = None
-    traffic_file = os.path.join(ROOT, "profiles", "k_trace_queue_traffic.json")
+    traffic_file = os.path.join(ROOT, "profiles", "march_traffic.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
-            traffic = json.load(f).get("traffic_bytes_per_launch")
+            traffic = json.load(f).get(dominant, {}).get("traffic_bytes_per_launch")
     roofline = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
-                "kernel": "k_trace_queue (extend march, FP32 vector work)",
+                "kernel": name + ", FP32 vector work",
                 "flops_per_launch": flops_per_launch, "launch_ms": ms_per_launch,
                 "kernel_ms_share": {"trace": first_stats.trace_ms, "shade": first_stats.shade_ms,
                                     "shadow": first_stats.shadow_ms, "finish": first_stats.finish_ms,

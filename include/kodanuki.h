@@ -328,6 +328,8 @@ typedef struct kdn_pass_stats {
     double shadow_flops;
     double finish_ms;        /* the finishing launch (paths to their ends) */
     double finish_flops;     /* its march operations (bit 2) */
+    uint32_t shadow_launches;
+    uint32_t finish_launches;
 } kdn_pass_stats;
 
 int kdn_render_pass(kdn_ctx* ctx, const kdn_pass_desc* desc, kdn_pass_stats* stats);

@@ -101,7 +101,8 @@ class PassStats(C.Structure):
     _fields_ = [("ms", C.c_double), ("segments", C.c_uint64), ("shadow_rays", C.c_uint64), ("bounces", C.c_uint32),
                 ("trace_launches", C.c_uint32), ("trace_ms", C.c_double), ("shadow_ms", C.c_double), ("shade_ms", C.c_double),
                 ("other_ms", C.c_double), ("trace_flops", C.c_double), ("shadow_flops", C.c_double),
-                ("finish_ms", C.c_double), ("finish_flops", C.c_double)]
+                ("finish_ms", C.c_double), ("finish_flops", C.c_double), ("shadow_launches", C.c_uint32),
+                ("finish_launches", C.c_uint32)]
 
 
 PASS_KEEP, PASS_TIME, PASS_COUNT, PASS_NO_FINISH = 1, 2, 4, 8

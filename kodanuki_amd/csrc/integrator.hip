@@ -267,6 +267,7 @@ bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
     uint64_t segments = 0, shadows = 0;
     uint32_t bounces = 0, launches = 0;
     double finishMs = 0;
+    uint32_t shadowLaunches = 0, finishLaunches = 0;
     for (uint32_t first = 0; first < d.spp; first += chunk) {
         const uint32_t n = std::min(chunk, d.spp - first);
         P.first_sample = d.first_sample + first;
@@ -280,7 +281,7 @@ bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
             KDN_HIP(hipMemcpyAsync(&count, &s->counters[in], 4, hipMemcpyDeviceToHost, st));
             KDN_HIP(hipStreamSynchronize(st));
             if (count == 0) break;
-            if (finishing && count <= kFinishBelow) {
+            if (finishing && (uint64_t)count * kFinishShare <= slots) {
                 // the remaining paths to their ends in one launch
                 KDN_HIP(hipMemsetAsync(&s->counters[4], 0, 8, st));
                 timed(&finishMs, [&] {
@@ -293,6 +294,7 @@ bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
                 segments += tally[0];
                 shadows += tally[1];
                 ++bounces;
+                ++finishLaunches;
                 break;
             }
             segments += count;
@@ -314,6 +316,7 @@ bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
             KDN_HIP(hipMemcpyAsync(&sc, &s->counters[2], 4, hipMemcpyDeviceToHost, st));
             KDN_HIP(hipStreamSynchronize(st));
             shadows += sc;
+            shadowLaunches += sc ? 1 : 0;
             if (sc) timed(&shadowMs, [&] { hipLaunchKernelGGL(k_shadow, dim3((sc + kTraceBlock - 1) / kTraceBlock), dim3(kTraceBlock), 0, st, tvShadow, *s); });
             std::swap(in, out);
         }
@@ -344,6 +347,8 @@ bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
         stats->shadow_flops = counted[1];
         stats->finish_ms = finishMs;
         stats->finish_flops = counted[2];
+        stats->shadow_launches = shadowLaunches;
+        stats->finish_launches = finishLaunches;
     }
     return true;
 }

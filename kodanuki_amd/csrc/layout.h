@@ -31,9 +31,11 @@ constexpr int kTraceBlock = 256;
 // the latency of its dependent fetches, so more waves in flight hide more
 // of it (the LDS of a block, 32 KB, allows five).
 constexpr int kTraceWaves = 5;
-// A pass finishes its paths in one launch once no more than this many are
-// left (each wavefront launch costs as long as its slowest ray).
-constexpr uint32_t kFinishBelow = 1u << 18;
+// A pass finishes its paths in one launch once no more than one in
+// kFinishShare of a chunk's paths are left (each wavefront launch costs as
+// long as its slowest ray; the first bounces' many coherent rays go faster
+// in a launch per kernel).
+constexpr uint32_t kFinishShare = 8;
 // Octree levels a march keeps the ancestry of (depth caps 0..14).
 constexpr int kPathLevels = 16;
 
