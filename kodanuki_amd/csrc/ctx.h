@@ -70,6 +70,8 @@ struct Ctx {
     void* lamps_dev = nullptr;      // LampDev per quad lamp
     uint32_t n_lamps = 0;
     bool integrator_ready = false;
+    void* round_dev = nullptr;      // the running round's job and slice tables
+    size_t round_bytes = 0;
     bool scene_has_glass = false;   // a leaf is painted with a glass
 };
 

@@ -2,16 +2,21 @@
 //
 //   generate → [extend → shade → shadow] × surfaces → fold
 //
-// generate starts one path per (pixel, sample) of a chunk; extend traces
+// A round runs the paths of one or more slices (samples of one job each:
+// a job is a render of one scene at one size) through the same queues.
+// generate starts one path per (pixel, sample) of each slice; extend traces
 // the queued rays through the octree; shade meets lamps and sky, lights
 // the vertex toward one lamp (a shadow ray with its MIS-weighted
 // contribution) and draws the next direction, appending survivors to the
 // next queue; shadow traces the shadow rays and adds the unblocked
 // contributions; fold turns each path's four lanes into linear RGB and
-// adds the chunk's samples of a pixel, in sample order, to FP64
-// accumulators (so the sums do not depend on queue order).
+// adds a slice's samples of a pixel, in sample order, to its job's FP64
+// accumulators (so the sums do not depend on queue order or on which
+// other jobs share the round). Once few paths are left, one launch takes
+// each to its end.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <vector>
@@ -23,33 +28,31 @@ namespace kdn {
 
 TreeView treeView(const Ctx& c);
 
-__global__ void k_generate(PassParams P, uint32_t slots) {
+__global__ void k_generate(PassParams P) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < slots) generate_one(P, i);
+    if (i < P.paths) generate_one(P, i);
 }
 
-__global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_trace_queue(TreeView tv, const uint32_t* queue, const uint32_t* count,
-                                                             const kdn_ray* rays, kdn_hit* hits) {
-    __shared__ float slots[kSlots * kTraceBlock];
-    __shared__ uint32_t paths[kPathLevels * kTraceBlock];
+#define KDN_LANE_REGS                                              \
+    __shared__ float slots[kSlots * kTraceBlock];                  \
+    __shared__ uint32_t paths[kPathLevels * kTraceBlock];          \
+    Dl buf[kMaxRegs];                                              \
+    const LaneRegs lr{buf, SlotRegs{slots + threadIdx.x, kTraceBlock}, PathRegs{paths + threadIdx.x, kTraceBlock}}
+
+__global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_trace_queue(PassParams P, uint32_t inQueue) {
+    KDN_LANE_REGS;
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= *count) return;
-    Dl buf[kMaxRegs];
-    const LaneRegs lr{buf, SlotRegs{slots + threadIdx.x, kTraceBlock}, PathRegs{paths + threadIdx.x, kTraceBlock}};
-    trace_item(tv, queue[q], rays, hits, lr, nullptr);
+    if (q >= P.s.counters[inQueue]) return;
+    trace_path(P, P.s.queue[inQueue][q], lr, nullptr);
 }
 
 // The same march counting the FP32 operations of every evaluation (the
 // instrumented pass the roofline's algorithmic work comes from).
-__global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_trace_queue_count(TreeView tv, const uint32_t* queue, const uint32_t* count,
-                                                                   const kdn_ray* rays, kdn_hit* hits, double* work) {
-    __shared__ float slots[kSlots * kTraceBlock];
-    __shared__ uint32_t paths[kPathLevels * kTraceBlock];
+__global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_trace_queue_count(PassParams P, uint32_t inQueue, double* work) {
+    KDN_LANE_REGS;
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= *count) return;
-    Dl buf[kMaxRegs];
-    const LaneRegs lr{buf, SlotRegs{slots + threadIdx.x, kTraceBlock}, PathRegs{paths + threadIdx.x, kTraceBlock}};
-    trace_item(tv, queue[q], rays, hits, lr, work);
+    if (q >= P.s.counters[inQueue]) return;
+    trace_path(P, P.s.queue[inQueue][q], lr, work);
 }
 
 __global__ void __launch_bounds__(256) k_shade(PassParams P, uint32_t inQueue, uint32_t outQueue) {
@@ -57,62 +60,53 @@ __global__ void __launch_bounds__(256) k_shade(PassParams P, uint32_t inQueue, u
     if (q < P.s.counters[inQueue]) shade_one(P, q, inQueue, outQueue);
 }
 
-__global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_shadow(TreeView tv, PathState s) {
-    __shared__ float slots[kSlots * kTraceBlock];
-    __shared__ uint32_t paths[kPathLevels * kTraceBlock];
+__global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_shadow(PassParams P, double* work) {
+    KDN_LANE_REGS;
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= s.counters[2]) return;
-    Dl buf[kMaxRegs];
-    const LaneRegs lr{buf, SlotRegs{slots + threadIdx.x, kTraceBlock}, PathRegs{paths + threadIdx.x, kTraceBlock}};
-    shadow_one(tv, s, q, lr);
+    if (q >= P.s.counters[2]) return;
+    shadow_one(P, q, lr, work);
 }
 
 // The paths still in the input queue, each to its end in one lane (the
 // wavefront's last bounces carry few paths, and each launch of them costs
 // as long as its slowest ray).
-__global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_finish(TreeView tv, TreeView tvShadow, PassParams P, uint32_t inQueue,
-                                                                    uint32_t* tally, double* work) {
-    __shared__ float slots[kSlots * kTraceBlock];
-    __shared__ uint32_t paths[kPathLevels * kTraceBlock];
+__global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_finish(PassParams P, uint32_t inQueue, uint32_t* tally, double* work) {
+    KDN_LANE_REGS;
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= P.s.counters[inQueue]) return;
-    Dl buf[kMaxRegs];
-    const LaneRegs lr{buf, SlotRegs{slots + threadIdx.x, kTraceBlock}, PathRegs{paths + threadIdx.x, kTraceBlock}};
-    finish_one(tv, tvShadow, P, q, inQueue, lr, tally, work);
+    finish_one(P, q, inQueue, lr, tally, work);
 }
 
-__global__ void k_fold(PassParams P, double* accum, double* accumSq, uint32_t* accumN) {
+__global__ void k_fold(PassParams P, uint32_t slice) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    fold_one(P, t, accum, accumSq, accumN);
+    fold_one(P, slice, t);
 }
 
-static uint64_t mixed_bits_host(uint64_t v) {
-    v += 0x9E3779B97F4A7C15ULL;
-    v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ULL;
-    v = (v ^ (v >> 27)) * 0x94D049BB133111EBULL;
-    return v ^ (v >> 31);
-}
+#undef KDN_LANE_REGS
 
 void freeIntegrator(Ctx& c) {
     if (c.paths) {
-        (void)hipFree(c.paths->rays);
+        (void)hipFree(c.paths->slice);
         delete c.paths;
         c.paths = nullptr;
     }
     if (c.colour) (void)hipFree(c.colour);
     c.colour = nullptr;
+    if (c.round_dev) (void)hipFree(c.round_dev);
+    c.round_dev = nullptr;
+    c.round_bytes = 0;
 }
 
 static bool allocPaths(Ctx& c, size_t n) {
     if (c.paths && c.paths->capacity >= n) return true;
     if (c.paths) {
-        (void)hipFree(c.paths->rays);
+        (void)hipFree(c.paths->slice);
         delete c.paths;
         c.paths = nullptr;
     }
     // one allocation, 256-byte aligned pieces
-    const size_t pieces[] = {n * sizeof(kdn_ray), n * sizeof(kdn_hit), n * sizeof(kdn_ray), n * 16, n * 16, n * 16, n * 16, n * 16,
-                             n * 4, n * 4, n * 4, n * 4, n * 8, n * 4, n * 4, n * 4, 64};
+    const size_t pieces[] = {n * 4, n * sizeof(kdn_ray), n * sizeof(kdn_hit), n * sizeof(kdn_ray), n * 16, n * 16, n * 16, n * 16,
+                             n * 16, n * 4, n * 4, n * 4, n * 4, n * 8, n * 4, n * 4, n * 4, 64};
     size_t bytes = 0;
     for (size_t b : pieces) bytes += (b + 255) & ~size_t(255);
     char* base = nullptr;
@@ -128,6 +122,7 @@ static bool allocPaths(Ctx& c, size_t n) {
         at += (pieces[k++] + 255) & ~size_t(255);
         return p;
     };
+    s->slice = (uint32_t*)take();
     s->rays = (kdn_ray*)take();
     s->hits = (kdn_hit*)take();
     s->shadow = (kdn_ray*)take();
@@ -148,17 +143,203 @@ static bool allocPaths(Ctx& c, size_t n) {
     return true;
 }
 
+bool ensureColour(Ctx& c) {
+    if (c.colour) return true;
+    ColourTables t;
+    buildColourTables(t);
+    KDN_HIP(hipMalloc(&c.colour, sizeof(ColourTables)));
+    KDN_HIP(hipMemcpy(c.colour, &t, sizeof(ColourTables), hipMemcpyHostToDevice));
+    return true;
+}
+
+static uint64_t mixedBitsHost(uint64_t v) {
+    v += 0x9E3779B97F4A7C15ULL;
+    v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    v = (v ^ (v >> 27)) * 0x94D049BB133111EBULL;
+    return v ^ (v >> 31);
+}
+
+uint32_t planeOf(int width, int height) { return (uint32_t)(((width + 7) / 8) * ((height + 7) / 8) * 64); }
+
+JobDev jobOf(const Ctx& scene, int width, int height, uint32_t seed, uint32_t surfaces, uint32_t minDepth) {
+    JobDev J{};
+    J.tv = treeView(scene);
+    J.width = width;
+    J.height = height;
+    J.plane = planeOf(width, height);
+    J.seed_bits = mixedBitsHost(seed);
+    J.surfaces = surfaces;
+    J.min_depth = minDepth;
+    // the lens (core/camera.cpp:9-25)
+    const kdn_camera& cam = scene.scene.camera;
+    double fwd[3], rt[3], upv[3];
+    double len = 0;
+    for (int k = 0; k < 3; ++k) {
+        fwd[k] = (double)cam.target[k] - cam.eye[k];
+        len += fwd[k] * fwd[k];
+    }
+    len = std::sqrt(len);
+    for (int k = 0; k < 3; ++k) fwd[k] /= len;
+    rt[0] = fwd[1] * cam.up[2] - fwd[2] * cam.up[1];
+    rt[1] = fwd[2] * cam.up[0] - fwd[0] * cam.up[2];
+    rt[2] = fwd[0] * cam.up[1] - fwd[1] * cam.up[0];
+    len = std::sqrt(rt[0] * rt[0] + rt[1] * rt[1] + rt[2] * rt[2]);
+    for (int k = 0; k < 3; ++k) rt[k] /= len;
+    upv[0] = rt[1] * fwd[2] - rt[2] * fwd[1];
+    upv[1] = rt[2] * fwd[0] - rt[0] * fwd[2];
+    upv[2] = rt[0] * fwd[1] - rt[1] * fwd[0];
+    for (int k = 0; k < 3; ++k) {
+        J.eye[k] = cam.eye[k];
+        J.forward[k] = (float)fwd[k];
+        J.right[k] = (float)rt[k];
+        J.up[k] = (float)upv[k];
+        J.sky_zenith[k] = scene.scene.render.sky_zenith[k];
+        J.sky_horizon[k] = scene.scene.render.sky_horizon[k];
+        J.sky_ground[k] = scene.scene.render.sky_ground[k];
+    }
+    J.reach = (float)std::tan((double)cam.fov / 2);
+    J.aspect = (float)((double)width / height);
+    J.n_lamps = scene.n_lamps;
+    J.n_materials = scene.n_materials_dev;
+    J.fallback_material = scene.n_materials_dev - 1;
+    J.lamps = (const LampDev*)scene.lamps_dev;
+    J.materials = (const MaterialDev*)scene.materials_dev;
+    J.leaf_material = scene.scene.leaf_material;
+    return J;
+}
+
+bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDev>& slices, uint32_t flags, kdn_pass_stats* stats) {
+    if (!ensureColour(c)) return false;
+    hipStream_t st = c.stream;
+    const SliceDev& lastSlice = slices.back();
+    const uint32_t total = lastSlice.first_path + lastSlice.spp * jobs[lastSlice.job].plane;
+    if (!allocPaths(c, total)) return false;
+    // the round's job and slice tables in HBM
+    const size_t jobBytes = jobs.size() * sizeof(JobDev), sliceBytes = slices.size() * sizeof(SliceDev);
+    const size_t roundBytes = ((jobBytes + 255) & ~size_t(255)) + sliceBytes;
+    if (c.round_bytes < roundBytes) {
+        if (c.round_dev) KDN_HIP(hipFree(c.round_dev));
+        c.round_dev = nullptr;
+        KDN_HIP(hipMalloc(&c.round_dev, roundBytes));
+        c.round_bytes = roundBytes;
+    }
+    char* table = (char*)c.round_dev;
+    KDN_HIP(hipMemcpyAsync(table, jobs.data(), jobBytes, hipMemcpyHostToDevice, st));
+    KDN_HIP(hipMemcpyAsync(table + ((jobBytes + 255) & ~size_t(255)), slices.data(), sliceBytes, hipMemcpyHostToDevice, st));
+    PassParams P{};
+    P.jobs = (const JobDev*)table;
+    P.slices = (const SliceDev*)(table + ((jobBytes + 255) & ~size_t(255)));
+    P.n_slices = (uint32_t)slices.size();
+    P.paths = total;
+    P.colour = (const ColourTables*)c.colour;
+    P.s = *c.paths;
+    PathState* s = c.paths;
+
+    const bool timing = (flags & 2) != 0, counting = (flags & 4) != 0, finishing = (flags & 8) == 0;
+    // kernel timing: an event pair per launch, read after each bounce's sync
+    hipEvent_t ev[2];
+    if (timing) {
+        KDN_HIP(hipEventCreate(&ev[0]));
+        KDN_HIP(hipEventCreate(&ev[1]));
+    }
+    auto timed = [&](double* acc, auto&& launch) -> bool {
+        if (timing) KDN_HIP(hipEventRecord(ev[0], st));
+        launch();
+        if (timing) {
+            KDN_HIP(hipEventRecord(ev[1], st));
+            KDN_HIP(hipEventSynchronize(ev[1]));
+            float ms = 0;
+            KDN_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            *acc += ms;
+        }
+        return true;
+    };
+    double* work = nullptr;
+    if (counting) {
+        if (!ensure(c.scratch[7], 64, c.error, false)) return false;
+        work = (double*)c.scratch[7].ptr;
+        KDN_HIP(hipMemsetAsync(work, 0, 32, st));
+    }
+
+    KDN_HIP(hipMemsetAsync(s->counters, 0, 64, st));
+    timed(&stats->other_ms, [&] { hipLaunchKernelGGL(k_generate, dim3((total + 255) / 256), dim3(256), 0, st, P); });
+    uint32_t in = 0, out = 1;
+    uint32_t maxSurfaces = 0;
+    for (const JobDev& J : jobs) maxSurfaces = std::max(maxSurfaces, J.surfaces);
+    for (uint32_t bounce = 0; bounce < maxSurfaces; ++bounce) {
+        uint32_t count = 0;
+        KDN_HIP(hipMemcpyAsync(&count, &s->counters[in], 4, hipMemcpyDeviceToHost, st));
+        KDN_HIP(hipStreamSynchronize(st));
+        if (count == 0) break;
+        if (finishing && (uint64_t)count * kFinishShare <= total) {
+            // the remaining paths to their ends in one launch
+            KDN_HIP(hipMemsetAsync(&s->counters[4], 0, 8, st));
+            timed(&stats->finish_ms, [&] {
+                hipLaunchKernelGGL(k_finish, dim3((count + kTraceBlock - 1) / kTraceBlock), dim3(kTraceBlock), 0, st, P, in,
+                                   &s->counters[4], counting ? work + 2 : nullptr);
+            });
+            uint32_t tally[2] = {0, 0};
+            KDN_HIP(hipMemcpyAsync(tally, &s->counters[4], 8, hipMemcpyDeviceToHost, st));
+            KDN_HIP(hipStreamSynchronize(st));
+            stats->segments += tally[0];
+            stats->shadow_rays += tally[1];
+            ++stats->bounces;
+            ++stats->finish_launches;
+            break;
+        }
+        stats->segments += count;
+        ++stats->bounces;
+        ++stats->trace_launches;
+        const uint32_t blocks = (count + 255) / 256;
+        timed(&stats->trace_ms, [&] {
+            if (counting) {
+                hipLaunchKernelGGL(k_trace_queue_count, dim3(blocks), dim3(kTraceBlock), 0, st, P, in, work);
+            } else {
+                hipLaunchKernelGGL(k_trace_queue, dim3(blocks), dim3(kTraceBlock), 0, st, P, in);
+            }
+        });
+        KDN_HIP(hipMemsetAsync(&s->counters[out], 0, 4, st));
+        KDN_HIP(hipMemsetAsync(&s->counters[2], 0, 4, st));
+        timed(&stats->shade_ms, [&] { hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, st, P, in, out); });
+        uint32_t sc = 0;
+        KDN_HIP(hipMemcpyAsync(&sc, &s->counters[2], 4, hipMemcpyDeviceToHost, st));
+        KDN_HIP(hipStreamSynchronize(st));
+        stats->shadow_rays += sc;
+        if (sc) {
+            ++stats->shadow_launches;
+            timed(&stats->shadow_ms, [&] {
+                hipLaunchKernelGGL(k_shadow, dim3((sc + kTraceBlock - 1) / kTraceBlock), dim3(kTraceBlock), 0, st, P,
+                                   counting ? work + 1 : nullptr);
+            });
+        }
+        std::swap(in, out);
+    }
+    // each slice's samples into its job's accumulators, slices in order
+    for (uint32_t k = 0; k < (uint32_t)slices.size(); ++k) {
+        const uint32_t plane = jobs[slices[k].job].plane;
+        timed(&stats->other_ms, [&] { hipLaunchKernelGGL(k_fold, dim3((plane + 255) / 256), dim3(256), 0, st, P, k); });
+    }
+    KDN_HIP(hipStreamSynchronize(st));
+    KDN_HIP(hipGetLastError());
+    if (counting) {
+        double counted[3] = {0, 0, 0};
+        KDN_HIP(hipMemcpy(counted, work, 24, hipMemcpyDeviceToHost));
+        stats->trace_flops += counted[0];
+        stats->shadow_flops += counted[1];
+        stats->finish_flops += counted[2];
+    }
+    if (timing) {
+        KDN_HIP(hipEventDestroy(ev[0]));
+        KDN_HIP(hipEventDestroy(ev[1]));
+    }
+    return true;
+}
+
 bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
     const auto start = std::chrono::steady_clock::now();
     if (!c.integrator_ready) {
         c.error = "kdn_render_pass: scene tables are not on the device";
         return false;
-    }
-    if (!c.colour) {
-        ColourTables t;
-        buildColourTables(t);
-        KDN_HIP(hipMalloc(&c.colour, sizeof(ColourTables)));
-        KDN_HIP(hipMemcpy(c.colour, &t, sizeof(ColourTables), hipMemcpyHostToDevice));
     }
     hipStream_t st = c.stream;
     // accumulators
@@ -180,176 +361,18 @@ bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
         KDN_HIP(hipMemsetAsync(c.accum_sq, 0, px * 3 * sizeof(double), st));
         KDN_HIP(hipMemsetAsync(c.accum_n, 0, px * sizeof(uint32_t), st));
     }
-    const uint32_t tilesX = (d.width + 7) / 8, tilesY = (d.height + 7) / 8;
-    const uint32_t plane = tilesX * tilesY * 64;
+    const std::vector<JobDev> jobs{jobOf(c, d.width, d.height, d.seed, d.surfaces, d.min_depth)};
+    const uint32_t plane = jobs[0].plane;
     const size_t maxPaths = c.max_paths ? c.max_paths : (size_t)1 << 23;
     const uint32_t chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(d.spp, maxPaths / plane));
-    if (!allocPaths(c, (size_t)plane * chunk)) return false;
-    PathState* s = c.paths;
-
-    PassParams P;
-    P.width = d.width;
-    P.height = d.height;
-    P.pixels = (uint32_t)px;
-    P.seed_bits = 0;
-    P.surfaces = d.surfaces;
-    P.min_depth = d.min_depth;
-    // the lens (core/camera.cpp:9-25)
-    const kdn_camera& cam = c.scene.camera;
-    double fwd[3], rt[3], upv[3];
-    double len = 0;
-    for (int k = 0; k < 3; ++k) {
-        fwd[k] = (double)cam.target[k] - cam.eye[k];
-        len += fwd[k] * fwd[k];
-    }
-    len = std::sqrt(len);
-    for (int k = 0; k < 3; ++k) fwd[k] /= len;
-    rt[0] = fwd[1] * cam.up[2] - fwd[2] * cam.up[1];
-    rt[1] = fwd[2] * cam.up[0] - fwd[0] * cam.up[2];
-    rt[2] = fwd[0] * cam.up[1] - fwd[1] * cam.up[0];
-    len = std::sqrt(rt[0] * rt[0] + rt[1] * rt[1] + rt[2] * rt[2]);
-    for (int k = 0; k < 3; ++k) rt[k] /= len;
-    upv[0] = rt[1] * fwd[2] - rt[2] * fwd[1];
-    upv[1] = rt[2] * fwd[0] - rt[0] * fwd[2];
-    upv[2] = rt[0] * fwd[1] - rt[1] * fwd[0];
-    for (int k = 0; k < 3; ++k) {
-        P.eye[k] = cam.eye[k];
-        P.forward[k] = (float)fwd[k];
-        P.right[k] = (float)rt[k];
-        P.up[k] = (float)upv[k];
-        P.sky_zenith[k] = c.scene.render.sky_zenith[k];
-        P.sky_horizon[k] = c.scene.render.sky_horizon[k];
-        P.sky_ground[k] = c.scene.render.sky_ground[k];
-    }
-    P.reach = (float)std::tan((double)cam.fov / 2);
-    P.aspect = (float)((double)d.width / d.height);
-    P.n_lamps = c.n_lamps;
-    P.n_materials = c.n_materials_dev;
-    P.fallback_material = c.n_materials_dev - 1;
-    P.lamps = (const LampDev*)c.lamps_dev;
-    P.materials = (const MaterialDev*)c.materials_dev;
-    P.leaf_material = c.scene.leaf_material;
-    P.colour = (const ColourTables*)c.colour;
-    P.s = *s;
-    P.seed_bits = mixed_bits_host(d.seed);
-    const TreeView tv = treeView(c);
-
-    const bool timing = (d.flags & 2) != 0, counting = (d.flags & 4) != 0;
-    // kernel timing: an event pair per launch, read after each bounce's sync
-    hipEvent_t ev[2];
-    if (timing) {
-        KDN_HIP(hipEventCreate(&ev[0]));
-        KDN_HIP(hipEventCreate(&ev[1]));
-    }
-    double traceMs = 0, shadeMs = 0, shadowMs = 0, otherMs = 0;
-    auto timed = [&](double* acc, auto&& launch) -> bool {
-        if (timing) KDN_HIP(hipEventRecord(ev[0], st));
-        launch();
-        if (timing) {
-            KDN_HIP(hipEventRecord(ev[1], st));
-            KDN_HIP(hipEventSynchronize(ev[1]));
-            float ms = 0;
-            KDN_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
-            *acc += ms;
-        }
-        return true;
-    };
-    const bool finishing = (d.flags & 8) == 0;
-    double* work = nullptr;
-    if (counting) {
-        if (!ensure(c.scratch[7], 64, c.error, false)) return false;
-        work = (double*)c.scratch[7].ptr;
-        KDN_HIP(hipMemsetAsync(work, 0, 24, st));
-    }
-    TreeView tvShadow = tv;
-    tvShadow.work = counting ? work + 1 : nullptr;
-
-    uint64_t segments = 0, shadows = 0;
-    uint32_t bounces = 0, launches = 0;
-    double finishMs = 0;
-    uint32_t shadowLaunches = 0, finishLaunches = 0;
+    kdn_pass_stats local{};
     for (uint32_t first = 0; first < d.spp; first += chunk) {
         const uint32_t n = std::min(chunk, d.spp - first);
-        P.first_sample = d.first_sample + first;
-        P.chunk = n;
-        const uint32_t slots = plane * n;
-        KDN_HIP(hipMemsetAsync(s->counters, 0, 64, st));
-        timed(&otherMs, [&] { hipLaunchKernelGGL(k_generate, dim3((slots + 255) / 256), dim3(256), 0, st, P, slots); });
-        uint32_t in = 0, out = 1;
-        for (uint32_t bounce = 0; bounce < d.surfaces; ++bounce) {
-            uint32_t count = 0;
-            KDN_HIP(hipMemcpyAsync(&count, &s->counters[in], 4, hipMemcpyDeviceToHost, st));
-            KDN_HIP(hipStreamSynchronize(st));
-            if (count == 0) break;
-            if (finishing && (uint64_t)count * kFinishShare <= slots) {
-                // the remaining paths to their ends in one launch
-                KDN_HIP(hipMemsetAsync(&s->counters[4], 0, 8, st));
-                timed(&finishMs, [&] {
-                    hipLaunchKernelGGL(k_finish, dim3((count + kTraceBlock - 1) / kTraceBlock), dim3(kTraceBlock), 0, st, tv, tvShadow,
-                                       P, in, &s->counters[4], counting ? work + 2 : nullptr);
-                });
-                uint32_t tally[2] = {0, 0};
-                KDN_HIP(hipMemcpyAsync(tally, &s->counters[4], 8, hipMemcpyDeviceToHost, st));
-                KDN_HIP(hipStreamSynchronize(st));
-                segments += tally[0];
-                shadows += tally[1];
-                ++bounces;
-                ++finishLaunches;
-                break;
-            }
-            segments += count;
-            ++bounces;
-            ++launches;
-            const uint32_t blocks = (count + 255) / 256;
-            timed(&traceMs, [&] {
-                if (counting) {
-                    hipLaunchKernelGGL(k_trace_queue_count, dim3(blocks), dim3(kTraceBlock), 0, st, tv, s->queue[in], &s->counters[in], s->rays,
-                                       s->hits, work);
-                } else {
-                    hipLaunchKernelGGL(k_trace_queue, dim3(blocks), dim3(kTraceBlock), 0, st, tv, s->queue[in], &s->counters[in], s->rays, s->hits);
-                }
-            });
-            KDN_HIP(hipMemsetAsync(&s->counters[out], 0, 4, st));
-            KDN_HIP(hipMemsetAsync(&s->counters[2], 0, 4, st));
-            timed(&shadeMs, [&] { hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, st, P, in, out); });
-            uint32_t sc = 0;
-            KDN_HIP(hipMemcpyAsync(&sc, &s->counters[2], 4, hipMemcpyDeviceToHost, st));
-            KDN_HIP(hipStreamSynchronize(st));
-            shadows += sc;
-            shadowLaunches += sc ? 1 : 0;
-            if (sc) timed(&shadowMs, [&] { hipLaunchKernelGGL(k_shadow, dim3((sc + kTraceBlock - 1) / kTraceBlock), dim3(kTraceBlock), 0, st, tvShadow, *s); });
-            std::swap(in, out);
-        }
-        timed(&otherMs, [&] {
-            hipLaunchKernelGGL(k_fold, dim3((plane + 255) / 256), dim3(256), 0, st, P, c.accum, c.accum_sq, c.accum_n);
-        });
+        const std::vector<SliceDev> slices{SliceDev{0, d.first_sample + first, n, 0, c.accum, c.accum_sq, c.accum_n}};
+        if (!runRound(c, jobs, slices, d.flags, &local)) return false;
     }
-    KDN_HIP(hipStreamSynchronize(st));
-    KDN_HIP(hipGetLastError());
-    double counted[3] = {0, 0, 0};
-    if (counting) KDN_HIP(hipMemcpy(counted, work, 24, hipMemcpyDeviceToHost));
-    if (timing) {
-        KDN_HIP(hipEventDestroy(ev[0]));
-        KDN_HIP(hipEventDestroy(ev[1]));
-    }
-    if (stats) {
-        *stats = kdn_pass_stats{};
-        stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
-        stats->segments = segments;
-        stats->shadow_rays = shadows;
-        stats->bounces = bounces;
-        stats->trace_launches = launches;
-        stats->trace_ms = traceMs;
-        stats->shade_ms = shadeMs;
-        stats->shadow_ms = shadowMs;
-        stats->other_ms = otherMs;
-        stats->trace_flops = counted[0];
-        stats->shadow_flops = counted[1];
-        stats->finish_ms = finishMs;
-        stats->finish_flops = counted[2];
-        stats->shadow_launches = shadowLaunches;
-        stats->finish_launches = finishLaunches;
-    }
+    local.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
+    if (stats) *stats = local;
     return true;
 }
 

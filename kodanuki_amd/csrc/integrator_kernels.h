@@ -8,6 +8,7 @@
 namespace kdn {
 
 struct PathState {
+    uint32_t* slice;   // the round's slice each path belongs to
     kdn_ray* rays;
     kdn_hit* hits;
     kdn_ray* shadow;
@@ -28,10 +29,12 @@ struct PathState {
 
 enum : uint32_t { FlagExact = 1u << 8, FlagDispersed = 1u << 9 };
 
-struct PassParams {
+// The scene and camera of one job (a render of one scene at one size), as
+// the kernels of its paths read them.
+struct JobDev {
+    TreeView tv;
     int width, height;
-    uint32_t pixels;
-    uint32_t first_sample, chunk;
+    uint32_t plane;        // paths of one sample of every pixel (8x8 tiles)
     uint64_t seed_bits;
     uint32_t surfaces, min_depth;
     float eye[3], forward[3], right[3], up[3];
@@ -41,53 +44,80 @@ struct PassParams {
     const LampDev* lamps;
     const MaterialDev* materials;
     const uint32_t* leaf_material;
+};
+
+// A slice of a round: samples [first_sample, first_sample + spp) of every
+// pixel of one job, on the paths [first_path, first_path + spp * plane),
+// folded into that job's accumulators.
+struct SliceDev {
+    uint32_t job;
+    uint32_t first_sample, spp;
+    uint32_t first_path;
+    double* accum;
+    double* accum_sq;
+    uint32_t* accum_n;
+};
+
+// A round: the slices of one or more jobs whose paths share the queues.
+struct PassParams {
+    const JobDev* jobs;
+    const SliceDev* slices;
+    uint32_t n_slices;
+    uint32_t paths;        // paths of the round (the last slice's end)
     const ColourTables* colour;
     PathState s;
 };
 
+KDN_D const JobDev& job_of(const PassParams& P, uint32_t i) { return P.jobs[P.slices[P.s.slice[i]].job]; }
+
 KDN_D float lane(const float* a, uint32_t l, size_t i, size_t n) { return a[l * n + i]; }
 
-// Pixel of path slot i: the chunk's sample planes one after another, each
-// plane in 8x8 tiles so that a wavefront's camera rays are neighbours.
-KDN_D void pixel_of(const PassParams& P, uint32_t p, uint32_t* x, uint32_t* y) {
-    const uint32_t tilesX = (P.width + 7) / 8;
+// Pixel of plane slot p: 8x8 tiles in rows, so that a wavefront's camera
+// rays are neighbours.
+KDN_D void pixel_of(const JobDev& J, uint32_t p, uint32_t* x, uint32_t* y) {
+    const uint32_t tilesX = (J.width + 7) / 8;
     const uint32_t tile = p / 64, in = p % 64;
     const uint32_t tx = tile % tilesX, ty = tile / tilesX;
     *x = tx * 8 + (in % 8);
     *y = ty * 8 + (in / 8);
 }
 
-// generate: the path of slot i (one per pixel and sample of the chunk)
+// generate: the path of slot i of the round (its slice's sample planes one
+// after another)
 KDN_D void generate_one(PassParams& P, uint32_t i) {
     PathState& s = P.s;
     const size_t N = s.capacity;
-    const uint32_t tilesX = (P.width + 7) / 8, tilesY = (P.height + 7) / 8;
-    const uint32_t plane = tilesX * tilesY * 64;
-    const uint32_t sIdx = i / plane, p = i % plane;
+    uint32_t k = 0;
+    while (k + 1 < P.n_slices && P.slices[k + 1].first_path <= i) ++k;
+    const SliceDev& sl = P.slices[k];
+    const JobDev& J = P.jobs[sl.job];
+    const uint32_t local = i - sl.first_path;
+    const uint32_t sIdx = local / J.plane, p = local % J.plane;
+    s.slice[i] = k;
     uint32_t x, y;
-    pixel_of(P, p, &x, &y);
-    const bool live = x < (uint32_t)P.width && y < (uint32_t)P.height;
-    s.pixel[i] = live ? y * P.width + x : 0xFFFFFFFFu;
+    pixel_of(J, p, &x, &y);
+    const bool live = x < (uint32_t)J.width && y < (uint32_t)J.height;
+    s.pixel[i] = live ? y * J.width + x : 0xFFFFFFFFu;
     if (!live) {
         for (int l = 0; l < kLanes; ++l) s.rad[l * N + i] = 0;
         return;
     }
-    const uint32_t sample = P.first_sample + sIdx;
+    const uint32_t sample = sl.first_sample + sIdx;
     // pixelNumbers and pixelWavelengths (trace.cpp:65-84)
     const uint64_t place = ((uint64_t)x << 32) | (uint64_t)y;
-    const uint64_t stream = P.seed_bits ^ mixed_bits(place);
+    const uint64_t stream = J.seed_bits ^ mixed_bits(place);
     const double start = hashed(stream, ~0ull, 0);
     const Lanes w = sampled_wavelengths(wavelength_share(start, sample), P.colour);
     // filmDirection (core/camera.cpp:32-38) at the jittered film point
     const double fx = x + hashed(stream, sample, PixelX), fy = y + hashed(stream, sample, PixelY);
-    const float u = (float)(2 * fx / P.width - 1), v = (float)(1 - 2 * fy / P.height);
-    V3 d = v3(P.forward[0], P.forward[1], P.forward[2]) + v3(P.right[0], P.right[1], P.right[2]) * (u * P.reach * P.aspect) +
-           v3(P.up[0], P.up[1], P.up[2]) * (v * P.reach);
+    const float u = (float)(2 * fx / J.width - 1), v = (float)(1 - 2 * fy / J.height);
+    V3 d = v3(J.forward[0], J.forward[1], J.forward[2]) + v3(J.right[0], J.right[1], J.right[2]) * (u * J.reach * J.aspect) +
+           v3(J.up[0], J.up[1], J.up[2]) * (v * J.reach);
     d = normalized(d);
     kdn_ray r;
-    r.origin[0] = P.eye[0];
-    r.origin[1] = P.eye[1];
-    r.origin[2] = P.eye[2];
+    r.origin[0] = J.eye[0];
+    r.origin[1] = J.eye[1];
+    r.origin[2] = J.eye[2];
     r.direction[0] = d.x;
     r.direction[1] = d.y;
     r.direction[2] = d.z;
@@ -110,6 +140,13 @@ KDN_D void generate_one(PassParams& P, uint32_t i) {
 }
 
 // extend: trace path i's ray; with work set, count the march's operations
+KDN_D void trace_item(const TreeView& tv, uint32_t i, const kdn_ray* rays, kdn_hit* hits, const LaneRegs& lr, double* work);
+
+// extend path i of a round through its job's octree
+KDN_D void trace_path(const PassParams& P, uint32_t i, const LaneRegs& lr, double* work) {
+    trace_item(job_of(P, i).tv, i, P.s.rays, P.s.hits, lr, work);
+}
+
 KDN_D void trace_item(const TreeView& tv, uint32_t i, const kdn_ray* rays, kdn_hit* hits, const LaneRegs& lr, double* work) {
     kdn_hit h;
     if (work) {
@@ -134,6 +171,7 @@ KDN_D float lift_at(V3 x) {
 KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
     PathState& s = P.s;
     const size_t N = s.capacity;
+    const JobDev& J = job_of(P, i);
     *shadow = false;
     const kdn_hit hit = s.hits[i];
     const kdn_ray ray = s.rays[i];
@@ -158,8 +196,8 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
 
     // lamps met along the ray (metAlong, trace.cpp:352-374), MIS against aiming
     if (!inside) {
-        for (uint32_t j = 0; j < P.n_lamps; ++j) {
-            const LampDev& L = P.lamps[j];
+        for (uint32_t j = 0; j < J.n_lamps; ++j) {
+            const LampDev& L = J.lamps[j];
             LampHit lh;
             if (!meet_quad(L, o, d, tHit, &lh)) continue;
             float Le[kLanes];
@@ -173,12 +211,12 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
     if (hit.kind == KDN_HIT_MISSED) {
         if (!inside) {
             float sky[kLanes];
-            sky_radiance(P.sky_zenith, P.sky_horizon, P.sky_ground, d, w, sky);
+            sky_radiance(J.sky_zenith, J.sky_horizon, J.sky_ground, d, w, sky);
             for (int l = 0; l < kLanes; ++l) rad[l] += thr[l] * sky[l];
         }
     } else if (hit.kind == KDN_HIT_SURFACE) {
-        const uint32_t mi = hit.leaf != KDN_NO_LEAF ? P.leaf_material[hit.leaf] : P.fallback_material;
-        const MaterialDev& m = P.materials[mi < P.n_materials ? mi : P.fallback_material];
+        const uint32_t mi = hit.leaf != KDN_NO_LEAF ? J.leaf_material[hit.leaf] : J.fallback_material;
+        const MaterialDev& m = J.materials[mi < J.n_materials ? mi : J.fallback_material];
         const V3 x = v3(hit.point[0], hit.point[1], hit.point[2]);
         const V3 out = -d;
         V3 n = v3(hit.normal[0], hit.normal[1], hit.normal[2]);
@@ -191,11 +229,11 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
         const float lift = lift_at(x);
         const bool exact = m.kind == KDN_GLASS && m.exact;
         // aim at one lamp (aimedFrom, trace.cpp:896-947), MIS against meeting
-        if (!exact && P.n_lamps > 0) {
+        if (!exact && J.n_lamps > 0) {
             const float uPick = number(stream, sample, depth, Pick);
             uint32_t j = 0;
-            while (j + 1 < P.n_lamps && uPick >= P.lamps[j].pick_cdf) ++j;
-            const LampDev& L = P.lamps[j];
+            while (j + 1 < J.n_lamps && uPick >= J.lamps[j].pick_cdf) ++j;
+            const LampDev& L = J.lamps[j];
             const float u = number(stream, sample, depth, LightU), v = number(stream, sample, depth, LightV);
             const V3 y = v3(L.at[0], L.at[1], L.at[2]) + v3(L.along[0], L.along[1], L.along[2]) * u + v3(L.across[0], L.across[1], L.across[2]) * v;
             const V3 toward = y - x;
@@ -234,7 +272,7 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
             }
         }
         // scatter (scatteredFrom, trace.cpp:1020-1074)
-        if (depth + 1 < (int)P.surfaces) {
+        if (depth + 1 < (int)J.surfaces) {
             const float uLobe = number(stream, sample, depth, Lobe), uSide = number(stream, sample, depth, Side);
             const float du = number(stream, sample, depth, DirectionU), dv = number(stream, sample, depth, DirectionV);
             const Scatter sc = m.kind == KDN_GLASS ? glass_scatter(m, w, n, out, inside, uSide) : rough_scatter(m, w, n, out, uLobe, du, dv);
@@ -250,7 +288,7 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
                         meta |= FlagDispersed;
                     }
                     bool survives = true;
-                    if (depth >= (int)P.min_depth) {
+                    if (depth >= (int)J.min_depth) {
                         const float survival = fmin_(0.95f, fmax_(fmax_(carried[0], carried[1]), fmax_(carried[2], carried[3])));
                         if (!(number(stream, sample, depth, Survival) < survival)) {
                             survives = false;
@@ -307,15 +345,16 @@ KDN_D void shade_one(PassParams& P, uint32_t q, uint32_t inQueue, uint32_t outQu
 }
 
 // the shadow test of path i: what it carries reaches the path when nothing
-// lies on its ray
-KDN_D void shadow_path(const TreeView& tv, PathState& s, uint32_t i, const LaneRegs& lr) {
+// lies on its ray (with work set, the march's operations are counted)
+KDN_D void shadow_path(const PassParams& P, uint32_t i, const LaneRegs& lr, double* work) {
+    const PathState& s = P.s;
     const kdn_ray r = s.shadow[i];
     kdn_hit h;
-    TreeView t = tv;
+    TreeView t = job_of(P, i).tv;
     double w = 0;
-    if (tv.work) t.work = &w;
+    t.work = work ? &w : nullptr;
     occlusion_ray(t, r, lr, h);
-    if (tv.work) atomicAdd(tv.work, w);
+    if (work) atomicAdd(work, w);
     if (h.kind == KDN_HIT_MISSED || !(h.distance < r.far_)) {
         const size_t N = s.capacity;
         for (int l = 0; l < kLanes; ++l) s.rad[l * N + i] += s.srad[l * N + i];
@@ -323,8 +362,8 @@ KDN_D void shadow_path(const TreeView& tv, PathState& s, uint32_t i, const LaneR
 }
 
 // shadow: entry q of the shadow queue
-KDN_D void shadow_one(const TreeView& tv, PathState& s, uint32_t q, const LaneRegs& lr) {
-    shadow_path(tv, s, s.queue[2][q], lr);
+KDN_D void shadow_one(const PassParams& P, uint32_t q, const LaneRegs& lr, double* work) {
+    shadow_path(P, P.s.queue[2][q], lr, work);
 }
 
 // finish: entry q of the input queue, its path to the end in one lane:
@@ -332,18 +371,17 @@ KDN_D void shadow_one(const TreeView& tv, PathState& s, uint32_t q, const LaneRe
 // stops every path at its last surface). The same per-path arithmetic as
 // the wavefront's launches, so the estimates do not change; counts
 // segments and shadow tests in tally[0] and tally[1].
-KDN_D void finish_one(const TreeView& tv, const TreeView& tvShadow, PassParams& P, uint32_t q, uint32_t inQueue, const LaneRegs& lr,
-                      uint32_t* tally, double* work) {
+KDN_D void finish_one(PassParams& P, uint32_t q, uint32_t inQueue, const LaneRegs& lr, uint32_t* tally, double* work) {
     PathState& s = P.s;
     const uint32_t i = s.queue[inQueue][q];
     uint32_t segments = 0, shadows = 0;
     for (;;) {
-        trace_item(tv, i, s.rays, s.hits, lr, work);
+        trace_path(P, i, lr, work);
         ++segments;
         bool shadow;
         const bool alive = shade_path(P, i, &shadow);
         if (shadow) {
-            shadow_path(tvShadow, s, i, lr);
+            shadow_path(P, i, lr, work ? work + 1 : nullptr);
             ++shadows;
         }
         if (!alive) break;
@@ -352,20 +390,19 @@ KDN_D void finish_one(const TreeView& tv, const TreeView& tvShadow, PassParams& 
     atomicAdd(&tally[1], shadows);
 }
 
-// Fold the chunk's samples of each pixel into the FP64 accumulators, in
-// sample order.
-// fold: pixel slot t of the first sample plane
-KDN_D void fold_one(const PassParams& P, uint32_t t, double* accum, double* accumSq, uint32_t* accumN) {
-    const uint32_t tilesX = (P.width + 7) / 8, tilesY = (P.height + 7) / 8;
-    const uint32_t plane = tilesX * tilesY * 64;
-    if (t >= plane) return;
+// fold: pixel slot t of slice k's first sample plane; its samples folded
+// into the job's FP64 accumulators in sample order
+KDN_D void fold_one(const PassParams& P, uint32_t k, uint32_t t) {
+    const SliceDev& sl = P.slices[k];
+    const JobDev& J = P.jobs[sl.job];
+    if (t >= J.plane) return;
     const PathState& s = P.s;
     const size_t N = s.capacity;
-    const uint32_t pixel = s.pixel[t];
+    const uint32_t pixel = s.pixel[sl.first_path + t];
     if (pixel == 0xFFFFFFFFu) return;
     double sum[3] = {0, 0, 0}, sq[3] = {0, 0, 0};
-    for (uint32_t k = 0; k < P.chunk; ++k) {
-        const size_t i = (size_t)k * plane + t;
+    for (uint32_t j = 0; j < sl.spp; ++j) {
+        const size_t i = (size_t)sl.first_path + (size_t)j * J.plane + t;
         Lanes w;
         float rad[kLanes];
         for (int l = 0; l < kLanes; ++l) {
@@ -381,10 +418,10 @@ KDN_D void fold_one(const PassParams& P, uint32_t t, double* accum, double* accu
         }
     }
     for (int c = 0; c < 3; ++c) {
-        accum[3 * (size_t)pixel + c] += sum[c];
-        accumSq[3 * (size_t)pixel + c] += sq[c];
+        sl.accum[3 * (size_t)pixel + c] += sum[c];
+        sl.accum_sq[3 * (size_t)pixel + c] += sq[c];
     }
-    accumN[pixel] += P.chunk;
+    sl.accum_n[pixel] += sl.spp;
 }
 
 }  // namespace kdn
