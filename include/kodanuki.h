@@ -339,6 +339,82 @@ int kdn_read_accumulation(kdn_ctx* ctx, double* rgb_sum, double* sq_sum, uint32_
 /* The mean image as linear RGB floats (width*height*3). */
 int kdn_read_image(kdn_ctx* ctx, float* rgb);
 
+/* ---- serving: many render jobs on one device ------------------------------ */
+/*
+ * A server renders jobs of mixed scenes, sizes and sample counts together.
+ * Each kdn_serve_step is one round of the integrator: the admitted jobs'
+ * next slices (a number of samples of every pixel fixed by the job alone)
+ * share the round's wavefront queues, and each slice folds into its own
+ * job's FP64 accumulators in sample order, so a job's result is the same,
+ * to the bit, alone and in any mix. Jobs are admitted in arrival order
+ * while the device memory they hold (resident scenes and octrees,
+ * accumulators, the shared path state) stays within the budget; a scene
+ * is built once per (scene, depth cap) and kept while jobs use it, then
+ * evicted least recently used. All calls on one server come from one
+ * thread.
+ *
+ * The reference renders one frame per process (core/render.h:212-260,
+ * core/worker.cpp:31-39): this is the layer a render service puts over
+ * kdn_render_pass, and it has no counterpart there.
+ */
+typedef struct kdn_server kdn_server;
+
+typedef struct kdn_job_desc {
+    const kdn_scene* scene;  /* kept alive by the caller until the job ends */
+    int32_t width, height;
+    uint32_t spp;            /* samples per pixel, from sample index 0 */
+    uint32_t seed;
+    uint32_t surfaces;       /* the most surfaces a path stands on */
+    uint32_t min_depth;      /* roulette starts at this depth */
+    int32_t depth;           /* the octree's depth cap */
+} kdn_job_desc;
+
+#define KDN_JOB_WAITING 0
+#define KDN_JOB_RUNNING 1
+#define KDN_JOB_DONE 2
+#define KDN_JOB_FAILED 3    /* not admitted: kdn_serve_job_error says why */
+
+typedef struct kdn_job_status {
+    uint32_t state;
+    uint32_t samples_done;   /* samples of every pixel folded */
+    double submitted_ms;     /* times since kdn_serve_open */
+    double admitted_ms;
+    double finished_ms;
+} kdn_job_status;
+
+typedef struct kdn_serve_stats {
+    kdn_pass_stats pass;     /* the round's integrator statistics */
+    uint64_t paths;          /* paths of the round */
+    uint64_t resident_bytes; /* device memory the server holds */
+    uint32_t rounds;         /* rounds run so far */
+    uint32_t jobs_in_round;
+    uint32_t slices_in_round;
+    uint32_t waiting;        /* jobs not yet admitted */
+    uint32_t running;        /* admitted jobs not done */
+    uint32_t pad;
+    double ms;               /* the step's wall time */
+} kdn_serve_stats;
+
+/* A server on the device of a context. It keeps its own path state for
+ * max_paths paths (the most of one round, at least 65536); budget_bytes
+ * bounds the device memory it holds: that path state, the resident scenes
+ * and the admitted jobs' accumulators. */
+int kdn_serve_open(kdn_ctx* ctx, uint64_t budget_bytes, uint32_t max_paths, kdn_server** out);
+int kdn_serve_submit(kdn_server* server, const kdn_job_desc* job, uint64_t* id);
+/* Admit what fits, then run one round; with nothing admitted it only
+ * admits. */
+int kdn_serve_step(kdn_server* server, kdn_serve_stats* stats);
+int kdn_serve_status(kdn_server* server, uint64_t id, kdn_job_status* status);
+/* A job's accumulators, as kdn_read_accumulation lays them out. */
+int kdn_serve_result(kdn_server* server, uint64_t id, double* rgb_sum, double* sq_sum, uint32_t* count);
+/* Forget a job that is not running, freeing its accumulators. */
+int kdn_serve_release(kdn_server* server, uint64_t id);
+/* The message of the last failure on the server (NULL: of kdn_serve_open). */
+const char* kdn_serve_error(const kdn_server* server);
+/* Why a job failed ("" for one that did not). */
+const char* kdn_serve_job_error(const kdn_server* server, uint64_t id);
+void kdn_serve_close(kdn_server* server);
+
 #ifdef __cplusplus
 }
 #endif

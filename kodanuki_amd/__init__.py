@@ -108,6 +108,25 @@ class PassStats(C.Structure):
 PASS_KEEP, PASS_TIME, PASS_COUNT, PASS_NO_FINISH = 1, 2, 4, 8
 
 
+class JobDesc(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_uint32), ("seed", C.c_uint32),
+                ("surfaces", C.c_uint32), ("min_depth", C.c_uint32), ("depth", C.c_int32)]
+
+
+class JobStatus(C.Structure):
+    _fields_ = [("state", C.c_uint32), ("samples_done", C.c_uint32), ("submitted_ms", C.c_double),
+                ("admitted_ms", C.c_double), ("finished_ms", C.c_double)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("pass_", PassStats), ("paths", C.c_uint64), ("resident_bytes", C.c_uint64), ("rounds", C.c_uint32),
+                ("jobs_in_round", C.c_uint32), ("slices_in_round", C.c_uint32), ("waiting", C.c_uint32),
+                ("running", C.c_uint32), ("pad", C.c_uint32), ("ms", C.c_double)]
+
+
+JOB_WAITING, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
+
+
 # numpy views of the structs
 STEP_DTYPE = np.dtype([("op", "u1"), ("flags", "u1"), ("a", "<u2"), ("b", "<u2"), ("c", "<u2"), ("value", "<f4"), ("leaf", "<u4")])
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("direction", "<f4", 3), ("far_", "<f4"), ("inside", "<u4"), ("within", "<u4")])
@@ -126,6 +145,8 @@ EXPORTS = [
     "kdn_scene_region", "kdn_open", "kdn_close", "kdn_last_error", "kdn_global_error", "kdn_upload_scene",
     "kdn_build_octree", "kdn_octree_sizes", "kdn_octree_download", "kdn_extend", "kdn_extend_device",
     "kdn_render_pass", "kdn_read_accumulation", "kdn_read_image",
+    "kdn_serve_open", "kdn_serve_submit", "kdn_serve_step", "kdn_serve_status", "kdn_serve_result",
+    "kdn_serve_release", "kdn_serve_error", "kdn_serve_job_error", "kdn_serve_close",
 ]
 
 _lib = None
@@ -174,6 +195,15 @@ def _bind(lib):
         "kdn_render_pass": (C.c_int, [P, C.POINTER(PassDesc), C.POINTER(PassStats)]),
         "kdn_read_accumulation": (C.c_int, [P, P, P, P]),
         "kdn_read_image": (C.c_int, [P, P]),
+        "kdn_serve_open": (C.c_int, [P, C.c_uint64, C.c_uint32, C.POINTER(P)]),
+        "kdn_serve_submit": (C.c_int, [P, C.POINTER(JobDesc), C.POINTER(C.c_uint64)]),
+        "kdn_serve_step": (C.c_int, [P, C.POINTER(ServeStats)]),
+        "kdn_serve_status": (C.c_int, [P, C.c_uint64, C.POINTER(JobStatus)]),
+        "kdn_serve_result": (C.c_int, [P, C.c_uint64, P, P, P]),
+        "kdn_serve_release": (C.c_int, [P, C.c_uint64]),
+        "kdn_serve_error": (C.c_char_p, [P]),
+        "kdn_serve_job_error": (C.c_char_p, [P, C.c_uint64]),
+        "kdn_serve_close": (None, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -365,6 +395,81 @@ class Device:
         rgb = np.zeros((height, width, 3), np.float32)
         _check(library().kdn_read_image(self._h, _ptr(rgb)), self._h)
         return rgb
+
+
+class Server:
+    """Many render jobs on one device (kdn_serve_*): each step() is one
+    integrator round over the admitted jobs' next slices."""
+
+    def __init__(self, device, budget_bytes=64 << 30, max_paths=1 << 23):
+        self._dev = device
+        self._h = C.c_void_p()
+        self._scenes = {}  # keeps the scenes of submitted jobs alive
+        self._sizes = {}
+        rc = library().kdn_serve_open(device.handle, budget_bytes, max_paths, C.byref(self._h))
+        if rc != 0:
+            raise KodanukiError(f"kodanuki error {rc}: {library().kdn_serve_error(None).decode()}")
+
+    def _check(self, rc):
+        if rc != 0:
+            raise KodanukiError(f"kodanuki error {rc}: {library().kdn_serve_error(self._h).decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().kdn_serve_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, scene, width, height, spp, seed=1, surfaces=7, min_depth=4, depth=8):
+        d = JobDesc(scene.handle, width, height, spp, seed, surfaces, min_depth, depth)
+        job = C.c_uint64()
+        self._check(library().kdn_serve_submit(self._h, C.byref(d), C.byref(job)))
+        self._scenes[job.value] = scene
+        self._sizes[job.value] = (width, height)
+        return job.value
+
+    def step(self):
+        st = ServeStats()
+        self._check(library().kdn_serve_step(self._h, C.byref(st)))
+        return st
+
+    def status(self, job):
+        st = JobStatus()
+        self._check(library().kdn_serve_status(self._h, job, C.byref(st)))
+        return st
+
+    def job_error(self, job):
+        return library().kdn_serve_job_error(self._h, job).decode()
+
+    def result(self, job):
+        w, h = self._sizes[job]
+        rgb = np.zeros((h, w, 3), np.float64)
+        sq = np.zeros((h, w, 3), np.float64)
+        n = np.zeros((h, w), np.uint32)
+        self._check(library().kdn_serve_result(self._h, job, _ptr(rgb), _ptr(sq), _ptr(n)))
+        return rgb, sq, n
+
+    def release(self, job):
+        self._check(library().kdn_serve_release(self._h, job))
+        self._scenes.pop(job, None)
+        self._sizes.pop(job, None)
+
+    def run(self, until=None):
+        """Step until every submitted job (or the given ones) has ended."""
+        jobs = list(self._sizes) if until is None else list(until)
+        steps = 0
+        while any(self.status(j).state in (JOB_WAITING, JOB_RUNNING) for j in jobs):
+            self.step()
+            steps += 1
+        return steps
 
 
 def make_rays(origins, directions, far=np.inf, inside=0, within=NO_LEAF):

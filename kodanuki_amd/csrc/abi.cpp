@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -17,11 +18,9 @@ struct kdn_scene {
     std::vector<kdn_step> steps;
     std::vector<double> values;
     std::vector<kdn_object> objects;
+    uint64_t serial = 0;  // unique per loaded scene in the process
 };
 
-struct kdn_ctx {
-    kdn::Ctx c;
-};
 
 namespace {
 
@@ -191,6 +190,24 @@ static void freeBuf(DevBuf& b) {
     b.bytes = 0;
 }
 
+uint64_t sceneSerial(const kdn_scene* scene) { return scene->serial; }
+
+void releaseScratch(Ctx& c) {
+    for (DevBuf& b : c.scratch) freeBuf(b);
+}
+
+uint64_t residentBytes(const Ctx& c) {
+    const SceneDev& s = c.scene;
+    uint64_t bytes = (uint64_t)s.n_steps * sizeof(kdn_step) + (uint64_t)s.n_objects * sizeof(kdn_object) +
+                     (uint64_t)s.n_materials * sizeof(kdn_material) + (uint64_t)s.n_lights * sizeof(kdn_light) +
+                     (uint64_t)s.n_leaves * 4;
+    bytes += c.tree.nodes.bytes + c.tree.entries.bytes + c.tree.pool.bytes + c.tree.spool.bytes;
+    for (const DevBuf& b : c.scratch) bytes += b.bytes;
+    bytes += c.paths_bytes + c.round_bytes;
+    if (c.accum) bytes += (uint64_t)c.width * c.height * (6 * sizeof(double) + sizeof(uint32_t));
+    return bytes;
+}
+
 static void freeScene(SceneDev& s) {
     if (s.steps) hipFree(s.steps);
     if (s.objects) hipFree(s.objects);
@@ -230,6 +247,8 @@ const char* kdn_global_error(void) { return g_error.c_str(); }
 int kdn_scene_load(const char* text, const char* base_dir, kdn_scene** out, char* err, size_t err_size) {
     if (!text || !out) return fail("kdn_scene_load: null argument", KDN_E_ARGUMENT);
     kdn_scene* s = new kdn_scene();
+    static std::atomic<uint64_t> loaded{0};
+    s->serial = ++loaded;
     std::string e;
     try {
         if (!kdn::loadScene(text, base_dir ? base_dir : "", s->scene, e)) {

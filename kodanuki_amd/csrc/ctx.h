@@ -62,6 +62,7 @@ struct Ctx {
     double* accum_sq = nullptr;     // luminance squares, FP64, per pixel
     uint32_t* accum_n = nullptr;    // samples per pixel
     PathState* paths = nullptr;     // host record of the path SoA in HBM
+    size_t paths_bytes = 0;         // its one allocation
     size_t max_paths = 0;           // paths in flight per chunk (0: 2^23)
     DevBuf scratch[8];              // reusable scratch for launches
     void* colour = nullptr;         // ColourTables
@@ -98,6 +99,31 @@ void freeIntegrator(Ctx& c);
 
 // scan.hip: exclusive prefix sum of n u32 values into out, returns the total
 bool scanU32(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* total, DevBuf& tmp, hipStream_t s, std::string& err);
+
+}  // namespace kdn
+
+// The context behind the C ABI's opaque handle.
+struct kdn_ctx {
+    kdn::Ctx c;
+};
+
+namespace kdn {
+
+// integrator.hip: a round of slices (the render pass's and the serving
+// loop's unit of work)
+struct JobDev;
+struct SliceDev;
+bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDev>& slices, uint32_t flags, kdn_pass_stats* stats);
+uint32_t planeOf(int width, int height);
+// the path state for n paths in flight, allocated now (it grows on demand)
+bool reservePaths(Ctx& c, size_t n);
+// the device memory a context holds (scene, octree, integrator state)
+uint64_t residentBytes(const Ctx& c);
+// a number that tells loaded scenes apart for the life of the process
+// (a freed scene's address may come back)
+uint64_t sceneSerial(const kdn_scene* scene);
+// free the octree build's scratch buffers
+void releaseScratch(Ctx& c);
 
 #define KDN_HIP(call)                                                                 \
     do {                                                                              \

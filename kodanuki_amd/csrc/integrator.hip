@@ -89,6 +89,7 @@ void freeIntegrator(Ctx& c) {
         (void)hipFree(c.paths->slice);
         delete c.paths;
         c.paths = nullptr;
+        c.paths_bytes = 0;
     }
     if (c.colour) (void)hipFree(c.colour);
     c.colour = nullptr;
@@ -103,6 +104,7 @@ static bool allocPaths(Ctx& c, size_t n) {
         (void)hipFree(c.paths->slice);
         delete c.paths;
         c.paths = nullptr;
+        c.paths_bytes = 0;
     }
     // one allocation, 256-byte aligned pieces
     const size_t pieces[] = {n * 4, n * sizeof(kdn_ray), n * sizeof(kdn_hit), n * sizeof(kdn_ray), n * 16, n * 16, n * 16, n * 16,
@@ -140,8 +142,11 @@ static bool allocPaths(Ctx& c, size_t n) {
     s->counters = (uint32_t*)take();
     s->capacity = n;
     c.paths = s;
+    c.paths_bytes = bytes;
     return true;
 }
+
+bool reservePaths(Ctx& c, size_t n) { return allocPaths(c, n); }
 
 bool ensureColour(Ctx& c) {
     if (c.colour) return true;

@@ -2,7 +2,8 @@
 
 Runs the GPU test session's sequence of calls (octree builds of the tiny,
 plain and production scenes, extend batches, the furnace renders, the
-progressive and oracle renders, a counted and timed production pass) on
+progressive and oracle renders, a counted and timed production pass, a
+mix of jobs through the serving loop) on
 `libkodanuki_emu.so`, the library's own sources built by g++ with
 AddressSanitizer against a CPU stand-in of the HIP runtime. Any out-of-bounds
 access in a kernel or a launch aborts with a report. Start it with the
@@ -106,6 +107,28 @@ def production(dev, quick):
         say(f"[production] flags {flags}: segments {st.segments} shadows {st.shadow_rays} flops {st.trace_flops:.3g}")
 
 
+def serving(dev):
+    """Two jobs alone, then together with a third arriving mid-run, on a
+    round of 65536 paths (the largest job takes two slices)."""
+    plain, tiny = k.Scene(scenes.plain(0)), k.Scene(scenes.tiny())
+    jobs = [(plain, 80, 60, 16, 3, 6, 6), (tiny, 31, 23, 5, 4, 8, 5), (plain, 20, 12, 2, 5, 4, 6)]
+    alone = []
+    with k.Server(dev, max_paths=1 << 16) as server:
+        for scene, w, h, spp, seed, surfaces, depth in jobs:
+            j = server.submit(scene, w, h, spp, seed=seed, surfaces=surfaces, depth=depth)
+            server.run([j])
+            alone.append(server.result(j))
+    with k.Server(dev, max_paths=1 << 16) as server:
+        ids = [server.submit(s, w, h, spp, seed=seed, surfaces=sf, depth=d) for s, w, h, spp, seed, sf, d in jobs[:2]]
+        server.step()
+        ids.append(server.submit(*jobs[2][:4], seed=jobs[2][4], surfaces=jobs[2][5], depth=jobs[2][6]))
+        server.run()
+        for i, j in enumerate(ids):
+            same = all(a.tobytes() == b.tobytes() for a, b in zip(server.result(j), alone[i]))
+            say(f"[serving] job {i}: equal alone and in the mix {same}")
+            assert same
+
+
 def main():
     quick = "--quick" in sys.argv
     O.PATH = os.path.join(HERE, "liboracle_asan.so")
@@ -115,6 +138,7 @@ def main():
     geometry(dev, quick)
     integrator(dev, quick)
     production(dev, quick)
+    serving(dev)
     dev.close()
     say("[emu] done")
 
