@@ -57,6 +57,86 @@ def cpu_baseline(scene, lower, size, width, height, seed, surfaces, budget_s=12.
                       f"(FP64 sphere trace over every object program), {threads} threads, {dt:.1f} s"}
 
 
+def serving_stream(k, scenes, dev, jobs, rate, seed, max_paths, budget, sizes=None, spps=None, pool_texts=None):
+    """The serving stream: `jobs` render jobs with Poisson arrivals at
+    `rate` jobs/s, each a scene of the eight-scene pool (32 to 4096
+    primitives), a size of {256^2, 512^2, 1024^2, 1920x1080} and
+    {1, 4, 16, 64} spp, served by kdn_serve_* (continuous batching into
+    shared rounds under a memory budget). Latency is from a job's arrival
+    to the end of the round that folds its last sample; scene builds are
+    part of the first job of each scene."""
+    import random
+    rng = random.Random(seed)
+    pool = []
+    for text, depth in pool_texts or scenes.serving_pool(glass=False):
+        sc = k.Scene(text)
+        pool.append((sc, depth, sc.settings()[1]))
+    sizes = sizes or [(256, 256), (512, 512), (1024, 1024), (1920, 1080)]
+    spps = spps or [1, 4, 16, 64]
+    stream, t = [], 0.0
+    for _ in range(jobs):
+        t += rng.expovariate(rate)
+        w, h = rng.choice(sizes)
+        stream.append((t, rng.randrange(len(pool)), w, h, rng.choice(spps)))
+    total_samples = sum(w * h * spp for _, _, w, h, spp in stream)
+    print(f"[serve] {jobs} jobs, {rate} jobs/s, {total_samples / 1e6:.1f} Msamples offered over {t:.1f} s",
+          file=sys.stderr, flush=True)
+    server = k.Server(dev, budget_bytes=budget, max_paths=max_paths)
+    t_open = time.perf_counter()
+    start = time.perf_counter()
+    base_ms = (start - t_open) * 1e3
+    pending = list(stream)
+    open_jobs = {}
+    latency = []
+    rounds = jobs_in_rounds = paths_in_rounds = 0
+    max_resident = 0
+    last_finish = 0.0
+    while pending or open_jobs:
+        now = time.perf_counter() - start
+        while pending and pending[0][0] <= now:
+            arrival, si, w, h, spp = pending.pop(0)
+            sc, depth, rs = pool[si]
+            j = server.submit(sc, w, h, spp, seed=rs.seed, surfaces=rs.surfaces, depth=depth)
+            open_jobs[j] = (arrival, si, w, h, spp)
+        if not open_jobs:
+            time.sleep(max(0.0, pending[0][0] - (time.perf_counter() - start)))
+            continue
+        st = server.step()
+        if st.jobs_in_round:
+            rounds += 1
+            jobs_in_rounds += st.jobs_in_round
+            paths_in_rounds += st.paths
+        max_resident = max(max_resident, st.resident_bytes)
+        for j in list(open_jobs):
+            status = server.status(j)
+            if status.state in (k.JOB_DONE, k.JOB_FAILED):
+                arrival, si, w, h, spp = open_jobs.pop(j)
+                if status.state == k.JOB_FAILED:
+                    raise RuntimeError(f"serving job failed: {server.job_error(j)}")
+                ms = status.finished_ms - (base_ms + arrival * 1e3)
+                latency.append(ms)
+                last_finish = max(last_finish, status.finished_ms)
+                print(f"[serve] job {j}: scene {si} {w}x{h} {spp} spp, "
+                      f"{ms:.0f} ms after arrival", file=sys.stderr, flush=True)
+                server.release(j)
+    server.close()
+    wall_s = (last_finish - base_ms) / 1e3
+    lat = sorted(latency)
+
+    def pct(q):
+        return lat[min(len(lat) - 1, int(q * len(lat)))]
+
+    return {"workload": f"{jobs} jobs, Poisson arrivals at {rate} jobs/s (seed {seed}); sizes from "
+                        f"{', '.join(f'{w}x{h}' for w, h in sizes)}; spp from {', '.join(map(str, spps))}; scenes from "
+                        f"{len(pool)} seeded scenes" + (" of 32 to 4096 primitives" if pool_texts is None else "") +
+                        f" (glass painted as white matte); rounds of at most {max_paths} paths, "
+                        f"budget {budget / 2 ** 30:g} GiB",
+            "p50_ms": pct(0.50), "p99_ms": pct(0.99), "mean_ms": sum(lat) / len(lat),
+            "aggregate_msamples_s": total_samples / wall_s / 1e6, "samples": total_samples, "wall_s": wall_s,
+            "rounds": rounds, "mean_jobs_per_round": jobs_in_rounds / max(rounds, 1),
+            "mean_paths_per_round": paths_in_rounds / max(rounds, 1), "max_resident_bytes": max_resident}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +147,12 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    ap.add_argument("--serve-jobs", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals, jobs/s")
+    ap.add_argument("--serve-seed", type=int, default=7)
+    ap.add_argument("--serve-paths", type=int, default=1 << 23, help="paths of one round")
+    ap.add_argument("--serve-budget-gib", type=int, default=192)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (the known-size kernel that scales HBM counters)")
     args = ap.parse_args()
@@ -159,6 +245,11 @@ def main():
                                     "shadow": first_stats.shadow_ms, "finish": first_stats.finish_ms,
                                     "other": first_stats.other_ms}}
 
+    serving = None
+    if not args.no_serve:
+        serving = serving_stream(k, scenes, dev, args.serve_jobs, args.serve_rate, args.serve_seed, args.serve_paths,
+                                 args.serve_budget_gib << 30)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, lower, size, W, H, rs.seed, surfaces)
@@ -181,6 +272,7 @@ def main():
                                f"materials painted as white mattes (glass scattering held off the device)",
                    "method": "plain", "segments_per_step": first_stats.segments, "shadow_rays_per_step": first_stats.shadow_rays},
         "roofline": roofline,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

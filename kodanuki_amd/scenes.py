@@ -12,6 +12,7 @@ Workloads:
                depth 4, 40% matte / 30% rough metal / 30% dispersive glass,
                four quad lights and a gradient sky, 1920x1080.
 - glass_heavy: the plain scene with 60% glass.
+- serving_pool: eight of the above, 32 to 4096 primitives.
 """
 import math
 
@@ -84,25 +85,33 @@ def _material(rng, name, kinds, glass=True):
     return f"let {name} = glass(refraction = {A:.6g}, dispersion = {B:.6g});"
 
 
-def plain(seed=0, count=32, glass=0.0, width=256, height=256, samples=16, surfaces=7):
-    """The plain scene: `count` primitives smooth-merged over a plane."""
+def plain(seed=0, count=32, glass=0.0, width=256, height=256, samples=16, surfaces=7, spread=False):
+    """The plain scene: `count` primitives smooth-merged over a plane.
+    They sit on a square grid 3.6 wide; with spread=True the grid keeps the
+    spacing of 32 primitives (0.6) and grows with the count, and the
+    camera, light and region grow with it (the same density of shapes, so
+    the same length of per-cell programs, at any count)."""
     rng = Rng(seed)
+    side = max(1, int(math.ceil(math.sqrt(count))))
+    spacing = 0.6 if spread else 3.6 / side
+    f = max(1.0, spacing * side / 3.6)
     out = ['import "standard.sdf";',
-           f"render {{ width = {width}; height = {height}; seed = {seed + 1}; samples = {samples}; surfaces = {surfaces}; far = 4; }}",
-           "camera { eye = [0, -6.5, 3.2]; target = [0, 0, 0.45]; up = [0, 0, 1]; fov = 0.75; }",
-           "light quad { at = [-0.75, -1.25, 4]; along = [0, 1.5, 0]; across = [1.5, 0, 0]; power = [40, 38, 34]; }"]
+           f"render {{ width = {width}; height = {height}; seed = {seed + 1}; samples = {samples}; surfaces = {surfaces}; "
+           f"far = {4 * f:.6g}; }}",
+           f"camera {{ eye = {_v([0, -6.5 * f, 3.2 * f])}; target = [0, 0, 0.45]; up = [0, 0, 1]; fov = 0.75; }}",
+           f"light quad {{ at = {_v([-0.75 * f, -1.25 * f, 4 * f])}; along = {_v([0, 1.5 * f, 0])}; "
+           f"across = {_v([1.5 * f, 0, 0])}; power = {_v([40 * f * f, 38 * f * f, 34 * f * f])}; }}"]
     kinds = [1.0 - glass, 0.0, glass] if glass > 0 else [1.0, 0.0, 0.0]
     for i in range(count):
         out.append(_material(rng, f"m{i}", kinds))
     out.append("let floor = matte([0.55, 0.55, 0.5]);")
     out.append("merge(0.05) {")
-    side = max(1, int(math.ceil(math.sqrt(count))))
-    spacing = 3.6 / side
+    half = spacing * side / 2
     for i in range(count):
         shape, reach = _primitive(rng)
         gx, gy = i % side, i // side
-        x = -1.8 + spacing * (gx + 0.5) + rng.uniform(-0.15, 0.15)
-        y = -1.8 + spacing * (gy + 0.5) + rng.uniform(-0.15, 0.15)
+        x = -half + spacing * (gx + 0.5) + rng.uniform(-0.15, 0.15)
+        y = -half + spacing * (gy + 0.5) + rng.uniform(-0.15, 0.15)
         z = reach * rng.uniform(0.6, 1.0)
         ang = rng.uniform(0, math.pi)
         out.append(f"\tpaint(m{i}) at({_v([x, y, z])}) turn(z, {ang:.6g}) {shape};")
@@ -159,6 +168,26 @@ def production(seed=0, count=512, width=1920, height=1080, samples=64, surfaces=
         out.extend(body[1:])
     out.append("paint(floor) plane([0, 0, 1], 0);")
     return "\n".join(out) + "\n"
+
+
+SERVING_COUNTS = (32, 64, 128, 256, 512, 1024, 2048, 4096)
+
+
+def serving_pool(glass=True):
+    """Eight seeded scenes of 32 to 4096 primitives for the serving stream:
+    (text, octree depth) pairs, the plain scene spread at the density of 32
+    primitives (octree depth 6, 7, 8) up to 128 primitives, the production
+    scene (depth 8) from 256 on."""
+    pool = []
+    for i, n in enumerate(SERVING_COUNTS):
+        if n <= 128:
+            # smooth merges of more shapes keep more of them per cell: a
+            # level deeper per doubling keeps every surface cell's program
+            # within the march's 384 registers
+            pool.append((plain(100 + i, count=n, spread=True), {32: 6, 64: 7, 128: 8}[n]))
+        else:
+            pool.append((production(100 + i, count=n, glass=glass), 8))
+    return pool
 
 
 def tiny(seed=0, glass=True):
