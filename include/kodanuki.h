@@ -311,7 +311,11 @@ typedef struct kdn_pass_desc {
                                bit 2: count the march's FP32 operations
                                       (an instrumented, slower pass);
                                bit 3: no finishing launch (every bounce a
-                                      wavefront launch to the end) */
+                                      wavefront launch to the end);
+                               bit 4: the learned guide: sample directions
+                                      with it (one-sample MIS with the
+                                      BSDF) once it has trained, and train
+                                      it on the pass's paths */
 } kdn_pass_desc;
 
 typedef struct kdn_pass_stats {
@@ -330,6 +334,10 @@ typedef struct kdn_pass_stats {
     double finish_flops;     /* its march operations (bit 2) */
     uint32_t shadow_launches;
     uint32_t finish_launches;
+    double guide_ms;         /* the guide's inference and training kernels */
+    double guide_loss;       /* its last training step's loss */
+    uint32_t guide_samples;  /* training samples the pass gave */
+    uint32_t guide_steps;    /* Adam steps taken */
 } kdn_pass_stats;
 
 int kdn_render_pass(kdn_ctx* ctx, const kdn_pass_desc* desc, kdn_pass_stats* stats);
@@ -338,6 +346,76 @@ int kdn_render_pass(kdn_ctx* ctx, const kdn_pass_desc* desc, kdn_pass_stats* sta
 int kdn_read_accumulation(kdn_ctx* ctx, double* rgb_sum, double* sq_sum, uint32_t* count);
 /* The mean image as linear RGB floats (width*height*3). */
 int kdn_read_image(kdn_ctx* ctx, float* rgb);
+
+/* ---- the learned guide ------------------------------------------------------ */
+/*
+ * A network that learns, per scene, where light arrives from: a hash grid
+ * of the position (16 levels of 2^19 entries, 2 features each), real
+ * spherical harmonics of the outgoing direction, the normal and a code of
+ * the hero wavelength feed three ReLU layers of 64 (BF16 weights, MFMA
+ * with FP32 accumulation) whose outputs decode to a mixture of 8 von
+ * Mises-Fisher lobes. A guided pass (kdn_render_pass flag bit 4) samples
+ * half of the directions at the first four vertices of a path from the
+ * mixture and half from the BSDF (one-sample MIS, so the estimate stays
+ * unbiased) and trains the network online, with Adam, on the radiance its
+ * own paths bring back. The reference has no learned guide; it joins the
+ * direction sampling the reference's Sampler dispatches
+ * (plugins/transport/paths/sampler.h:23,357). The calls below expose the
+ * network for tests and tools.
+ */
+typedef struct kdn_guide_query {
+    float point[3];    /* world position */
+    float out[3];      /* unit direction toward the viewer */
+    float normal[3];   /* unit normal facing out */
+    float nm;          /* hero wavelength */
+} kdn_guide_query;
+
+typedef struct kdn_guide_sample {
+    float point[3];
+    float out[3];
+    float normal[3];
+    float nm;
+    float in[3];       /* the direction light came back along */
+    float weight;      /* radiance over the density it was sampled with */
+    float pad[2];
+} kdn_guide_sample;
+
+typedef struct kdn_guide_lobe {
+    float axis[3];
+    float sharpness;
+    float share;
+} kdn_guide_lobe;
+
+typedef struct kdn_guide_info {
+    uint32_t levels, features, table_size, width, layers, lobes, depths;
+    uint32_t train_paths;    /* paths of a round that keep training records, at most */
+    uint32_t resolution[16];
+    uint32_t dense_levels;   /* bit l: level l indexed densely */
+    uint32_t steps;          /* Adam steps taken */
+    float loss_scale;
+    float share;             /* the guide's share of directions */
+    double last_loss;
+    uint32_t last_samples;
+    uint32_t pad;
+    uint64_t grid_params, mlp_params;
+    float lower[3], size[3]; /* the region positions are scaled from */
+} kdn_guide_info;
+
+/* Initialise the network from a seed over a region (NULL: the octree's). */
+int kdn_guide_reset(kdn_ctx* ctx, uint32_t seed, const float* lower3, const float* size3);
+int kdn_guide_describe(kdn_ctx* ctx, kdn_guide_info* info);
+/* FP32 parameters: grid[grid_params] (level, entry, feature) and
+ * mlp[mlp_params] (layer, input, output); NULL skips one. */
+int kdn_guide_params(kdn_ctx* ctx, float* grid, float* mlp);
+int kdn_guide_set_params(kdn_ctx* ctx, const float* grid, const float* mlp);
+/* The mixture at each query: lobes[8 * n]. */
+int kdn_guide_infer(kdn_ctx* ctx, const kdn_guide_query* queries, uint32_t n, kdn_guide_lobe* lobes);
+/* Loss sum over samples of -weight * log q(in) and its gradients (no
+ * normalisation, no loss scale); grid_grad may be NULL. */
+int kdn_guide_gradients(kdn_ctx* ctx, const kdn_guide_sample* samples, uint32_t n, float* grid_grad, float* mlp_grad, double* loss);
+/* Adam steps on the samples (batches in turn, weights normalised by their
+ * mean); losses[steps] receives each step's loss (NaN: a skipped step). */
+int kdn_guide_train(kdn_ctx* ctx, const kdn_guide_sample* samples, uint32_t n, uint32_t steps, uint32_t batch, double* losses);
 
 /* ---- serving: many render jobs on one device ------------------------------ */
 /*

@@ -102,10 +102,33 @@ class PassStats(C.Structure):
                 ("trace_launches", C.c_uint32), ("trace_ms", C.c_double), ("shadow_ms", C.c_double), ("shade_ms", C.c_double),
                 ("other_ms", C.c_double), ("trace_flops", C.c_double), ("shadow_flops", C.c_double),
                 ("finish_ms", C.c_double), ("finish_flops", C.c_double), ("shadow_launches", C.c_uint32),
-                ("finish_launches", C.c_uint32)]
+                ("finish_launches", C.c_uint32), ("guide_ms", C.c_double), ("guide_loss", C.c_double),
+                ("guide_samples", C.c_uint32), ("guide_steps", C.c_uint32)]
 
 
-PASS_KEEP, PASS_TIME, PASS_COUNT, PASS_NO_FINISH = 1, 2, 4, 8
+PASS_KEEP, PASS_TIME, PASS_COUNT, PASS_NO_FINISH, PASS_GUIDE = 1, 2, 4, 8, 16
+
+
+class GuideInfo(C.Structure):
+    _fields_ = [("levels", C.c_uint32), ("features", C.c_uint32), ("table_size", C.c_uint32), ("width", C.c_uint32),
+                ("layers", C.c_uint32), ("lobes", C.c_uint32), ("depths", C.c_uint32), ("train_paths", C.c_uint32),
+                ("resolution", C.c_uint32 * 16), ("dense_levels", C.c_uint32), ("steps", C.c_uint32),
+                ("loss_scale", C.c_float), ("share", C.c_float), ("last_loss", C.c_double), ("last_samples", C.c_uint32),
+                ("pad", C.c_uint32), ("grid_params", C.c_uint64), ("mlp_params", C.c_uint64), ("lower", C.c_float * 3),
+                ("size", C.c_float * 3)]
+
+
+class GuideQuery(C.Structure):
+    _fields_ = [("point", C.c_float * 3), ("out", C.c_float * 3), ("normal", C.c_float * 3), ("nm", C.c_float)]
+
+
+class GuideSample(C.Structure):
+    _fields_ = [("point", C.c_float * 3), ("out", C.c_float * 3), ("normal", C.c_float * 3), ("nm", C.c_float),
+                ("in_", C.c_float * 3), ("weight", C.c_float), ("pad", C.c_float * 2)]
+
+
+class GuideLobe(C.Structure):
+    _fields_ = [("axis", C.c_float * 3), ("sharpness", C.c_float), ("share", C.c_float)]
 
 
 class JobDesc(C.Structure):
@@ -145,6 +168,8 @@ EXPORTS = [
     "kdn_scene_region", "kdn_open", "kdn_close", "kdn_last_error", "kdn_global_error", "kdn_upload_scene",
     "kdn_build_octree", "kdn_octree_sizes", "kdn_octree_download", "kdn_extend", "kdn_extend_device",
     "kdn_render_pass", "kdn_read_accumulation", "kdn_read_image",
+    "kdn_guide_reset", "kdn_guide_describe", "kdn_guide_params", "kdn_guide_set_params", "kdn_guide_infer",
+    "kdn_guide_gradients", "kdn_guide_train",
     "kdn_serve_open", "kdn_serve_submit", "kdn_serve_step", "kdn_serve_status", "kdn_serve_result",
     "kdn_serve_release", "kdn_serve_error", "kdn_serve_job_error", "kdn_serve_close",
 ]
@@ -195,6 +220,13 @@ def _bind(lib):
         "kdn_render_pass": (C.c_int, [P, C.POINTER(PassDesc), C.POINTER(PassStats)]),
         "kdn_read_accumulation": (C.c_int, [P, P, P, P]),
         "kdn_read_image": (C.c_int, [P, P]),
+        "kdn_guide_reset": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "kdn_guide_describe": (C.c_int, [P, C.POINTER(GuideInfo)]),
+        "kdn_guide_params": (C.c_int, [P, P, P]),
+        "kdn_guide_set_params": (C.c_int, [P, P, P]),
+        "kdn_guide_infer": (C.c_int, [P, P, C.c_uint32, P]),
+        "kdn_guide_gradients": (C.c_int, [P, P, C.c_uint32, P, P, C.POINTER(C.c_double)]),
+        "kdn_guide_train": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_uint32, P]),
         "kdn_serve_open": (C.c_int, [P, C.c_uint64, C.c_uint32, C.POINTER(P)]),
         "kdn_serve_submit": (C.c_int, [P, C.POINTER(JobDesc), C.POINTER(C.c_uint64)]),
         "kdn_serve_step": (C.c_int, [P, C.POINTER(ServeStats)]),
@@ -390,6 +422,55 @@ class Device:
         n = np.zeros((height, width), np.uint32)
         _check(library().kdn_read_accumulation(self._h, _ptr(rgb), _ptr(sq), _ptr(n)), self._h)
         return rgb, sq, n
+
+    # ---- the learned guide ----
+    def guide_reset(self, seed=1, lower=None, size=None):
+        lo = (C.c_float * 3)(*[float(x) for x in lower]) if lower is not None else None
+        sz = (C.c_float * 3)(*[float(x) for x in size]) if size is not None else None
+        _check(library().kdn_guide_reset(self._h, seed, lo, sz), self._h)
+
+    def guide_info(self):
+        info = GuideInfo()
+        _check(library().kdn_guide_describe(self._h, C.byref(info)), self._h)
+        return info
+
+    def guide_params(self):
+        """(grid [16, 2^19, 2], mlp [4, 64, 64]) FP32."""
+        info = self.guide_info()
+        grid = np.zeros(info.grid_params, np.float32)
+        mlp = np.zeros(info.mlp_params, np.float32)
+        _check(library().kdn_guide_params(self._h, _ptr(grid), _ptr(mlp)), self._h)
+        return grid.reshape(16, -1, 2), mlp.reshape(4, 64, 64)
+
+    def guide_set_params(self, grid=None, mlp=None):
+        g = np.ascontiguousarray(grid, np.float32) if grid is not None else None
+        m = np.ascontiguousarray(mlp, np.float32) if mlp is not None else None
+        _check(library().kdn_guide_set_params(self._h, _ptr(g) if g is not None else None,
+                                              _ptr(m) if m is not None else None), self._h)
+
+    def guide_infer(self, queries):
+        """queries [n, 10] (point, out, normal, nm) -> lobes [n, 8, 5]
+        (axis, sharpness, share)."""
+        q = np.ascontiguousarray(queries, np.float32)
+        out = np.zeros((len(q), 8, 5), np.float32)
+        _check(library().kdn_guide_infer(self._h, _ptr(q), len(q), _ptr(out)), self._h)
+        return out
+
+    def guide_gradients(self, samples, grid=True):
+        """samples [n, 16] -> (loss, grid grad [16, 2^19, 2] or None, mlp grad [4, 64, 64])."""
+        s = np.ascontiguousarray(samples, np.float32)
+        mg = np.zeros((4, 64, 64), np.float32)
+        gg = np.zeros((16, 1 << 19, 2), np.float32) if grid else None
+        loss = C.c_double()
+        _check(library().kdn_guide_gradients(self._h, _ptr(s), len(s), _ptr(gg) if grid else None, _ptr(mg),
+                                             C.byref(loss)), self._h)
+        return loss.value, gg, mg
+
+    def guide_train(self, samples, steps, batch):
+        s = np.ascontiguousarray(samples, np.float32)
+        losses = np.zeros(steps, np.float64)
+        _check(library().kdn_guide_train(self._h, _ptr(s), len(s), steps, batch, _ptr(losses)), self._h)
+        return losses
 
     def image(self, width, height):
         rgb = np.zeros((height, width, 3), np.float32)

@@ -232,6 +232,7 @@ void releaseAll(Ctx& c) {
     c.accum = c.accum_sq = nullptr;
     c.accum_n = nullptr;
     freeIntegrator(c);
+    freeGuide(c);
     if (c.materials_dev) hipFree(c.materials_dev);
     if (c.lamps_dev) hipFree(c.lamps_dev);
     c.materials_dev = c.lamps_dev = nullptr;
@@ -431,6 +432,7 @@ int kdn_upload_scene(kdn_ctx* ctx, const kdn_scene* s) {
     if (d.leaf_material) hipFree(d.leaf_material);
     d = kdn::SceneDev{};
     c.tree.built = false;
+    kdn::freeGuide(c);  // the guide learns one scene
     std::vector<kdn_material> mats = s->scene.materials;
     const uint32_t fallback = (uint32_t)mats.size();
     kdn_material grey{};

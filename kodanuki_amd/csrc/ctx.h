@@ -74,6 +74,7 @@ struct Ctx {
     void* round_dev = nullptr;      // the running round's job and slice tables
     size_t round_bytes = 0;
     bool scene_has_glass = false;   // a leaf is painted with a glass
+    void* guide = nullptr;          // the learned guide (GuideState), made on first use
 };
 
 // Glass scattering is held off the device: renders with glass faulted the
@@ -96,6 +97,11 @@ bool extendRays(Ctx& c, const kdn_ray* rays, uint32_t n, kdn_hit* hits, hipStrea
 // integrator.hip
 bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats);
 void freeIntegrator(Ctx& c);
+
+// guide.hip: the learned guide's parameters over a region, from a seed
+bool guideReset(Ctx& c, uint32_t seed, const float lower[3], const float size[3]);
+bool guideReady(const Ctx& c);
+void freeGuide(Ctx& c);
 
 // scan.hip: exclusive prefix sum of n u32 values into out, returns the total
 bool scanU32(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* total, DevBuf& tmp, hipStream_t s, std::string& err);

@@ -1,0 +1,27 @@
+// Launchers of the learned guide's network kernels (guide_net.hip on the
+// device; the CPU build of the library has its own of the same names).
+#pragma once
+
+#include "ctx.h"
+#include "integrator_kernels.h"
+
+namespace kdn {
+
+// lobes of queries (kdn_guide_query each) into kGuideOutputs floats each
+bool launchGuideInferQueries(const GuideView& g, const float* queries, uint32_t n, float* lobes, hipStream_t st);
+// lobes of the guided vertices among the first n paths of a queue, into
+// the per-path lobe buffer (lane-major)
+bool launchGuideInferPaths(const GuideView& g, const PassParams& P, uint32_t queue, uint32_t n, float* lobes, hipStream_t st);
+// forward and backward of samples [first, first + count) (modulo n): each
+// tile of 64 writes its weight gradients to partial[tile], adds its grid
+// gradients to gridGrad and its loss to *loss; sample weights times *scale
+bool launchGuideTrain(const GuideView& g, const float* samples, uint32_t n, uint32_t first, uint32_t count, const float* scale,
+                      float* partial, float* gridGrad, float* loss, hipStream_t st);
+
+// a guided round (guide.hip): lobes and records before it, lobes of each
+// guided bounce's vertices before their shading, training after its folds
+bool guideBeginRound(Ctx& c, PassParams& P, uint32_t paths);
+bool guideInferQueue(Ctx& c, const PassParams& P, uint32_t queue, uint32_t count);
+bool guideEndRound(Ctx& c, const PassParams& P, uint32_t paths, kdn_pass_stats* stats);
+
+}  // namespace kdn

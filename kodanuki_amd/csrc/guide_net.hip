@@ -1,0 +1,279 @@
+// The learned guide's network on MFMA: inference (queries or the paths of
+// a queue → vMF lobes) and training (forward, loss gradient, backward:
+// per-tile weight gradients and grid gradients).
+//
+// A wavefront takes a tile of 64 samples. The layers are 64x64 BF16
+// matrices multiplied on v_mfma_f32_16x16x32_bf16 with FP32 accumulation:
+// a tile's layer is 4x4 output blocks of 16x16, two MFMAs deep. Weights sit
+// in LDS (both orientations for training), activations of the tile in LDS
+// as BF16 rows, outputs as FP32.
+#include <hip/hip_runtime.h>
+
+#include "ctx.h"
+#include "guide_net.h"
+
+namespace kdn {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kLd = 72;         // LDS row stride of a 64-wide BF16 matrix (elements)
+constexpr int kOutLd = 65;      // LDS row stride of FP32 outputs
+constexpr int kInferWaves = 4;  // waves per inference block
+
+// acc = A · B for a 64x64 tile: A row-major (rows = samples, lda), B given
+// transposed (Bt[n][k], ldb) so that a lane's eight consecutive k are
+// contiguous. MFMA 16x16x32: lane l holds A[row l%16][k 8(l/16) .. +7],
+// B[k 8(l/16) .. +7][col l%16] and D[rows 4(l/16) .. +3][col l%16].
+__device__ __forceinline__ void wave_gemm(const uint16_t* A, int lda, const uint16_t* Bt, int ldb, f32x4 acc[16]) {
+    const int l = threadIdx.x & 63;
+    const int r = l & 15, kq = (l >> 4) * 8;
+    for (int t = 0; t < 16; ++t) acc[t] = f32x4{0, 0, 0, 0};
+    for (int ks = 0; ks < 64; ks += 32) {
+        bf16x8 a[4], b[4];
+        for (int rb = 0; rb < 4; ++rb) a[rb] = *(const bf16x8*)(A + (rb * 16 + r) * lda + ks + kq);
+        for (int cb = 0; cb < 4; ++cb) b[cb] = *(const bf16x8*)(Bt + (cb * 16 + r) * ldb + ks + kq);
+        for (int rb = 0; rb < 4; ++rb)
+            for (int cb = 0; cb < 4; ++cb) acc[rb * 4 + cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb], b[cb], acc[rb * 4 + cb], 0, 0, 0);
+    }
+}
+
+// acc = A^T · B for a 64x64 tile where both are stored row-major with the
+// reduced dimension (the samples) as rows: A[k][m], B[k][n]. Gathers the
+// lane's eight k of a column element by element.
+__device__ __forceinline__ void wave_gemm_tn(const uint16_t* A, int lda, const uint16_t* B, int ldb, f32x4 acc[16]) {
+    const int l = threadIdx.x & 63;
+    const int r = l & 15, kq = (l >> 4) * 8;
+    for (int t = 0; t < 16; ++t) acc[t] = f32x4{0, 0, 0, 0};
+    for (int ks = 0; ks < 64; ks += 32) {
+        bf16x8 a[4], b[4];
+        for (int rb = 0; rb < 4; ++rb) {
+            uint16_t e[8];
+            for (int j = 0; j < 8; ++j) e[j] = A[(ks + kq + j) * lda + rb * 16 + r];
+            __builtin_memcpy(&a[rb], e, sizeof e);
+        }
+        for (int cb = 0; cb < 4; ++cb) {
+            uint16_t e[8];
+            for (int j = 0; j < 8; ++j) e[j] = B[(ks + kq + j) * ldb + cb * 16 + r];
+            __builtin_memcpy(&b[cb], e, sizeof e);
+        }
+        for (int rb = 0; rb < 4; ++rb)
+            for (int cb = 0; cb < 4; ++cb) acc[rb * 4 + cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb], b[cb], acc[rb * 4 + cb], 0, 0, 0);
+    }
+}
+
+// The tile's D elements: (row, col) of element j of block t.
+__device__ __forceinline__ int d_row(int t, int j) { return (t >> 2) * 16 + ((threadIdx.x & 63) >> 4) * 4 + j; }
+__device__ __forceinline__ int d_col(int t) { return (t & 3) * 16 + (threadIdx.x & 15); }
+
+// ReLU(acc) as BF16 rows.
+__device__ __forceinline__ void store_relu(const f32x4 acc[16], uint16_t* H) {
+    for (int t = 0; t < 16; ++t)
+        for (int j = 0; j < 4; ++j) {
+            const float v = acc[t][j];
+            H[d_row(t, j) * kLd + d_col(t)] = to_bf16(v > 0 ? v : 0.0f);
+        }
+}
+
+// acc masked by the ReLU of the forward activation H (the gradient of the
+// pre-activation), as BF16 rows.
+__device__ __forceinline__ void store_masked(const f32x4 acc[16], const uint16_t* H, uint16_t* G) {
+    for (int t = 0; t < 16; ++t)
+        for (int j = 0; j < 4; ++j) {
+            const int row = d_row(t, j), col = d_col(t);
+            G[row * kLd + col] = H[row * kLd + col] != 0 && (H[row * kLd + col] & 0x8000u) == 0 ? to_bf16(acc[t][j]) : (uint16_t)0;
+        }
+}
+
+__device__ __forceinline__ void store_f32(const f32x4 acc[16], float* O) {
+    for (int t = 0; t < 16; ++t)
+        for (int j = 0; j < 4; ++j) O[d_row(t, j) * kOutLd + d_col(t)] = acc[t][j];
+}
+
+// the queries of a host-given array
+struct ArrayQueries {
+    const float* q;  // kdn_guide_query: point, out, normal, nm
+    float* lobes;    // per query kGuideOutputs floats: axis, sharpness, share per lobe
+    uint32_t n;
+    __device__ bool fetch(const GuideView& g, uint32_t item, float in[kGuideWidth]) const {
+        if (item >= n) return false;
+        const float* e = q + (size_t)item * 10;
+        guide_inputs(g, e, e + 3, e + 6, e[9], in);
+        return true;
+    }
+    __device__ void write(uint32_t item, const GuideLobes& L) const {
+        float* o = lobes + (size_t)item * kGuideOutputs;
+        for (int k = 0; k < kGuideLobes; ++k) {
+            o[5 * k] = L.axis[k][0];
+            o[5 * k + 1] = L.axis[k][1];
+            o[5 * k + 2] = L.axis[k][2];
+            o[5 * k + 3] = L.sharp[k];
+            o[5 * k + 4] = L.share[k];
+        }
+    }
+};
+
+// the guided vertices of a queue of paths
+struct PathQueries {
+    PassParams P;
+    uint32_t queue, n;
+    float* lobes;  // lane-major, kGuideOutputs per path
+    __device__ bool fetch(const GuideView& g, uint32_t item, float in[kGuideWidth]) const {
+        if (item >= n) return false;
+        const uint32_t i = P.s.queue[queue][item];
+        float x[3], out[3], normal[3], nm;
+        if (!guided_vertex(P, i, x, out, normal, &nm)) return false;
+        guide_inputs(g, x, out, normal, nm, in);
+        return true;
+    }
+    __device__ void write(uint32_t item, const GuideLobes& L) const {
+        const uint32_t i = P.s.queue[queue][item];
+        store_lobes(lobes, P.s.capacity, i, L);
+    }
+};
+
+template <class Q>
+__global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideView g, Q src) {
+    __shared__ __attribute__((aligned(16))) uint16_t Wt[kGuideLayers][64 * kLd];
+    __shared__ __attribute__((aligned(16))) uint16_t act[kInferWaves][64 * kLd];
+    __shared__ float fo[kInferWaves][64 * kOutLd];
+    for (int e = threadIdx.x; e < kGuideMlpParams; e += blockDim.x) {
+        const int layer = e >> 12, in = (e >> 6) & 63, out = e & 63;
+        Wt[layer][out * kLd + in] = g.mlp[e];
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint16_t* A = act[wave];
+    float* O = fo[wave];
+    const uint32_t tiles = (src.n + 63) / 64;
+    for (uint32_t base = blockIdx.x * kInferWaves; base < tiles; base += gridDim.x * kInferWaves) {
+        const uint32_t item = (base + wave) * 64 + lane;
+        float in[kGuideWidth];
+        const bool live = src.fetch(g, item, in);
+        for (int k = 0; k < kGuideWidth; ++k) A[lane * kLd + k] = live ? to_bf16(in[k]) : (uint16_t)0;
+        __syncthreads();
+        f32x4 acc[16];
+        for (int layer = 0; layer < kGuideLayers - 1; ++layer) {
+            wave_gemm(A, kLd, Wt[layer], kLd, acc);
+            __syncthreads();
+            store_relu(acc, A);
+            __syncthreads();
+        }
+        wave_gemm(A, kLd, Wt[kGuideLayers - 1], kLd, acc);
+        store_f32(acc, O);
+        __syncthreads();
+        if (live) {
+            float o[kGuideOutputs];
+            for (int c = 0; c < kGuideOutputs; ++c) o[c] = O[lane * kOutLd + c];
+            GuideLobes L;
+            decode_lobes(o, L);
+            src.write(item, L);
+        }
+        __syncthreads();
+    }
+}
+
+// One tile of 64 training samples per block (one wave): forward, the
+// loss gradient of each sample (its weight times *scale), backward.
+// Writes the tile's weight gradients to partial[block][layer][in][out],
+// adds the grid gradients to grid_grad, the tile's loss to *loss.
+__global__ __launch_bounds__(64) void k_guide_train(GuideView g, const float* samples, uint32_t n, uint32_t first, uint32_t count,
+                                                    const float* scale, float* partial, float* grid_grad, float* loss) {
+    __shared__ __attribute__((aligned(16))) uint16_t Wt[kGuideLayers][64 * kLd];  // [out][in]
+    __shared__ __attribute__((aligned(16))) uint16_t Wn[kGuideLayers][64 * kLd];  // [in][out]
+    __shared__ __attribute__((aligned(16))) uint16_t H[kGuideLayers][64 * kLd];   // inputs, h1, h2, h3
+    __shared__ __attribute__((aligned(16))) uint16_t G[2][64 * kLd];              // gradients, ping-pong
+    __shared__ float O[64 * kOutLd];
+    for (int e = threadIdx.x; e < kGuideMlpParams; e += 64) {
+        const int layer = e >> 12, in = (e >> 6) & 63, out = e & 63;
+        Wt[layer][out * kLd + in] = g.mlp[e];
+        Wn[layer][in * kLd + out] = g.mlp[e];
+    }
+    const int lane = threadIdx.x & 63;
+    const uint32_t b = blockIdx.x * 64 + lane;
+    const bool live = b < count && n > 0;
+    const float* smp = samples + (size_t)(live ? (first + b) % n : 0) * kSamFloats;
+    float in[kGuideWidth];
+    if (live) guide_inputs(g, smp + kSamPoint, smp + kSamOut, smp + kSamNormal, smp[kSamNm], in);
+    for (int k = 0; k < kGuideWidth; ++k) H[0][lane * kLd + k] = live ? to_bf16(in[k]) : (uint16_t)0;
+    __syncthreads();
+    f32x4 acc[16];
+    for (int layer = 0; layer < kGuideLayers - 1; ++layer) {
+        wave_gemm(H[layer], kLd, Wt[layer], kLd, acc);
+        store_relu(acc, H[layer + 1]);
+        __syncthreads();
+    }
+    wave_gemm(H[kGuideLayers - 1], kLd, Wt[kGuideLayers - 1], kLd, acc);
+    store_f32(acc, O);
+    __syncthreads();
+    // the loss gradient of each output
+    {
+        float o[kGuideOutputs], grad[kGuideOutputs];
+        for (int c = 0; c < kGuideOutputs; ++c) o[c] = O[lane * kOutLd + c];
+        const float w = live ? smp[kSamWeight] * *scale : 0.0f;
+        const float l = lobe_loss_gradient(o, smp + kSamIn, w, grad);
+        for (int c = 0; c < kGuideWidth; ++c) G[0][lane * kLd + c] = c < kGuideOutputs ? to_bf16(grad[c]) : (uint16_t)0;
+        if (l != 0) atomicAdd(loss, l);
+    }
+    __syncthreads();
+    // backward, last layer first: dW = H^T dZ, dH = dZ W^T masked by ReLU
+    int cur = 0;
+    for (int layer = kGuideLayers - 1; layer >= 0; --layer) {
+        wave_gemm_tn(H[layer], kLd, G[cur], kLd, acc);
+        float* P = partial + ((size_t)blockIdx.x * kGuideLayers + layer) * 4096;
+        for (int t = 0; t < 16; ++t)
+            for (int j = 0; j < 4; ++j) P[d_row(t, j) * 64 + d_col(t)] = acc[t][j];
+        wave_gemm(G[cur], kLd, Wn[layer], kLd, acc);
+        if (layer > 0) {
+            store_masked(acc, H[layer], G[cur ^ 1]);
+        } else {
+            store_f32(acc, O);
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    // the grid's share: each level's two features to its eight corners
+    if (live) {
+        float p[3];
+        guide_position(g, smp + kSamPoint, p);
+        for (int level = 0; level < kGuideLevels; ++level) {
+            uint32_t idx[8];
+            float wt[8];
+            grid_corners(g, level, p, idx, wt);
+            const float d0 = O[lane * kOutLd + kInGrid + 2 * level], d1 = O[lane * kOutLd + kInGrid + 2 * level + 1];
+            float* table = grid_grad + (size_t)level * kGuideTable * kGuideFeatures;
+            for (int c = 0; c < 8; ++c) {
+                if (d0 != 0) atomicAdd(&table[(size_t)idx[c] * 2], wt[c] * d0);
+                if (d1 != 0) atomicAdd(&table[(size_t)idx[c] * 2 + 1], wt[c] * d1);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+bool launchGuideInferQueries(const GuideView& g, const float* queries, uint32_t n, float* lobes, hipStream_t st) {
+    if (n == 0) return true;
+    const uint32_t tiles = (n + 63) / 64;
+    const uint32_t blocks = std::min<uint32_t>((tiles + kInferWaves - 1) / kInferWaves, 4096);
+    hipLaunchKernelGGL(k_guide_infer<ArrayQueries>, dim3(blocks), dim3(64 * kInferWaves), 0, st, g, ArrayQueries{queries, lobes, n});
+    return hipGetLastError() == hipSuccess;
+}
+
+bool launchGuideInferPaths(const GuideView& g, const PassParams& P, uint32_t queue, uint32_t n, float* lobes, hipStream_t st) {
+    if (n == 0) return true;
+    const uint32_t tiles = (n + 63) / 64;
+    const uint32_t blocks = std::min<uint32_t>((tiles + kInferWaves - 1) / kInferWaves, 4096);
+    hipLaunchKernelGGL(k_guide_infer<PathQueries>, dim3(blocks), dim3(64 * kInferWaves), 0, st, g, PathQueries{P, queue, n, lobes});
+    return hipGetLastError() == hipSuccess;
+}
+
+bool launchGuideTrain(const GuideView& g, const float* samples, uint32_t n, uint32_t first, uint32_t count, const float* scale,
+                      float* partial, float* gridGrad, float* loss, hipStream_t st) {
+    if (count == 0) return true;
+    hipLaunchKernelGGL(k_guide_train, dim3((count + 63) / 64), dim3(64), 0, st, g, samples, n, first, count, scale, partial, gridGrad, loss);
+    return hipGetLastError() == hipSuccess;
+}
+
+}  // namespace kdn

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "ctx.h"
+#include "guide_net.h"
 #include "integrator_kernels.h"
 
 namespace kdn {
@@ -240,7 +241,7 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
     P.s = *c.paths;
     PathState* s = c.paths;
 
-    const bool timing = (flags & 2) != 0, counting = (flags & 4) != 0, finishing = (flags & 8) == 0;
+    const bool timing = (flags & 2) != 0, counting = (flags & 4) != 0, finishing = (flags & 8) == 0, guiding = (flags & 16) != 0;
     // kernel timing: an event pair per launch, read after each bounce's sync
     hipEvent_t ev[2];
     if (timing) {
@@ -266,6 +267,10 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
         KDN_HIP(hipMemsetAsync(work, 0, 32, st));
     }
 
+    if (guiding) {
+        if (!guideReady(c) && !guideReset(c, 1, c.tree.lower, c.tree.size)) return false;
+        if (!guideBeginRound(c, P, total)) return false;
+    }
     KDN_HIP(hipMemsetAsync(s->counters, 0, 64, st));
     timed(&stats->other_ms, [&] { hipLaunchKernelGGL(k_generate, dim3((total + 255) / 256), dim3(256), 0, st, P); });
     uint32_t in = 0, out = 1;
@@ -276,7 +281,9 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
         KDN_HIP(hipMemcpyAsync(&count, &s->counters[in], 4, hipMemcpyDeviceToHost, st));
         KDN_HIP(hipStreamSynchronize(st));
         if (count == 0) break;
-        if (finishing && (uint64_t)count * kFinishShare <= total) {
+        // (a guided round finishes paths only past the guided depths: the
+        // finishing launch shades without the network's lobes)
+        if (finishing && (!guiding || bounce >= (uint32_t)kGuideDepths) && (uint64_t)count * kFinishShare <= total) {
             // the remaining paths to their ends in one launch
             KDN_HIP(hipMemsetAsync(&s->counters[4], 0, 8, st));
             timed(&stats->finish_ms, [&] {
@@ -303,6 +310,11 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
                 hipLaunchKernelGGL(k_trace_queue, dim3(blocks), dim3(kTraceBlock), 0, st, P, in);
             }
         });
+        if (guiding && bounce < (uint32_t)kGuideDepths) {
+            bool ok = true;
+            timed(&stats->guide_ms, [&] { ok = guideInferQueue(c, P, in, count); });
+            if (!ok) return false;
+        }
         KDN_HIP(hipMemsetAsync(&s->counters[out], 0, 4, st));
         KDN_HIP(hipMemsetAsync(&s->counters[2], 0, 4, st));
         timed(&stats->shade_ms, [&] { hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, st, P, in, out); });
@@ -323,6 +335,11 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
     for (uint32_t k = 0; k < (uint32_t)slices.size(); ++k) {
         const uint32_t plane = jobs[slices[k].job].plane;
         timed(&stats->other_ms, [&] { hipLaunchKernelGGL(k_fold, dim3((plane + 255) / 256), dim3(256), 0, st, P, k); });
+    }
+    if (guiding) {
+        bool ok = true;
+        timed(&stats->guide_ms, [&] { ok = guideEndRound(c, P, total, stats); });
+        if (!ok) return false;
     }
     KDN_HIP(hipStreamSynchronize(st));
     KDN_HIP(hipGetLastError());

@@ -3,6 +3,7 @@
 // this header (bounds-checked) runs the same code item by item.
 #pragma once
 
+#include "guide.h"
 #include "integrator_dev.h"
 
 namespace kdn {
@@ -66,6 +67,14 @@ struct PassParams {
     uint32_t paths;        // paths of the round (the last slice's end)
     const ColourTables* colour;
     PathState s;
+    // the learned guide: lobes of the guided vertices (lane-major,
+    // kGuideOutputs per path; null: BSDF sampling only), training records
+    // (kGuideDepths for each path of train_stride; null: none kept) and
+    // the guide's share of sampled directions
+    const float* lobes;
+    float* recs;
+    uint32_t train_stride;
+    float guide_share;
 };
 
 KDN_D const JobDev& job_of(const PassParams& P, uint32_t i) { return P.jobs[P.slices[P.s.slice[i]].job]; }
@@ -161,6 +170,94 @@ KDN_D void trace_item(const TreeView& tv, uint32_t i, const kdn_ray* rays, kdn_h
     hits[i] = h;
 }
 
+// the material a hit stands on
+KDN_D const MaterialDev& material_at(const JobDev& J, const kdn_hit& hit, uint32_t* index) {
+    const uint32_t mi = hit.leaf != KDN_NO_LEAF ? J.leaf_material[hit.leaf] : J.fallback_material;
+    *index = mi;
+    return J.materials[mi < J.n_materials ? mi : J.fallback_material];
+}
+
+// A vertex the guide may sample at: a surface hit of a rough material at
+// a depth below kGuideDepths. Gives the guide's query: point, outgoing
+// direction, normal facing it, hero wavelength.
+KDN_D bool guide_vertex_at(const PassParams& P, uint32_t i, float x[3], float out[3], float normal[3], float* nm) {
+    const PathState& s = P.s;
+    if ((s.meta[i] & 0xFF) >= (uint32_t)kGuideDepths) return false;
+    const kdn_hit hit = s.hits[i];
+    if (hit.kind != KDN_HIT_SURFACE) return false;
+    uint32_t mi;
+    if (material_at(job_of(P, i), hit, &mi).kind == KDN_GLASS) return false;
+    const kdn_ray& ray = s.rays[i];
+    V3 n = v3(hit.normal[0], hit.normal[1], hit.normal[2]);
+    const V3 o = v3(-ray.direction[0], -ray.direction[1], -ray.direction[2]);
+    if (dot(n, o) < 0) n = -n;
+    for (int k = 0; k < 3; ++k) x[k] = hit.point[k];
+    out[0] = o.x, out[1] = o.y, out[2] = o.z;
+    normal[0] = n.x, normal[1] = n.y, normal[2] = n.z;
+    *nm = s.nm[i];
+    return true;
+}
+
+KDN_D bool guided_vertex(const PassParams& P, uint32_t i, float x[3], float out[3], float normal[3], float* nm) {
+    return P.lobes != nullptr && guide_vertex_at(P, i, x, out, normal, nm);
+}
+
+KDN_D void store_lobes(float* lobes, size_t N, uint32_t i, const GuideLobes& L) {
+    for (int k = 0; k < kGuideLobes; ++k) {
+        lobes[(5 * k) * N + i] = L.axis[k][0];
+        lobes[(5 * k + 1) * N + i] = L.axis[k][1];
+        lobes[(5 * k + 2) * N + i] = L.axis[k][2];
+        lobes[(5 * k + 3) * N + i] = L.sharp[k];
+        lobes[(5 * k + 4) * N + i] = L.share[k];
+    }
+}
+
+KDN_D void load_lobes(const float* lobes, size_t N, uint32_t i, GuideLobes& L) {
+    for (int k = 0; k < kGuideLobes; ++k) {
+        L.axis[k][0] = lobes[(5 * k) * N + i];
+        L.axis[k][1] = lobes[(5 * k + 1) * N + i];
+        L.axis[k][2] = lobes[(5 * k + 2) * N + i];
+        L.sharp[k] = lobes[(5 * k + 3) * N + i];
+        L.share[k] = lobes[(5 * k + 4) * N + i];
+    }
+}
+
+// numbers of the guided sampling, apart from the reference's dimensions
+KDN_D float guide_number(uint64_t stream, uint32_t sample, int event, int draw) {
+    return (float)hashed(stream, sample, (1ull << 40) + (uint64_t)event * 4 + (uint64_t)draw);
+}
+
+// One-sample MIS of the guide (share) and the BSDF (1 - share): the
+// direction from one of them, its density the mixture of both.
+KDN_D Scatter guided_scatter(const MaterialDev& m, const Lanes& w, V3 n, V3 out, const GuideLobes& L, float share, float uSelect,
+                             float uGuideLobe, float uLobe, float u, float v) {
+    Scatter s;
+    s.found = false;
+    s.exact = false;
+    s.through = false;
+    s.collapse = false;
+    if (dot(n, out) <= 0) return s;
+    V3 in;
+    if (uSelect < share) {
+        in = mixture_sample(L, uGuideLobe, u, v);
+    } else {
+        const Scatter b = rough_scatter(m, w, n, out, uLobe, u, v);
+        if (!b.found) return s;
+        in = b.in;
+    }
+    const float arriving = dot(n, in);
+    if (arriving <= 0) return s;
+    const float density = share * mixture_pdf(L, in) + (1 - share) * rough_density(m, n, out, in);
+    if (!(density > 0)) return s;
+    float f[kLanes];
+    rough_value(m, w, n, out, in, f);
+    s.found = true;
+    s.in = in;
+    s.density = density;
+    for (int l = 0; l < kLanes; ++l) s.weight[l] = f[l] * (arriving / density);
+    return s;
+}
+
 KDN_D float lift_at(V3 x) {
     return 8e-5f + 8e-6f * fmax_(fmax_(__builtin_fabsf(x.x), __builtin_fabsf(x.y)), __builtin_fabsf(x.z));
 }
@@ -193,6 +290,16 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
     const float tHit = hit.kind == KDN_HIT_MISSED ? kInf : hit.distance;
     const bool exactBefore = (meta & FlagExact) != 0;
     const float pdfBefore = s.pdf[i];
+    // training records: the radiance the path holds as this vertex begins
+    // closes the record of the vertex before
+    const bool training = P.recs != nullptr && i % P.train_stride == 0;
+    if (training && depth >= 1 && depth - 1 < kGuideDepths) {
+        float* rec = P.recs + ((size_t)(i / P.train_stride) * kGuideDepths + depth - 1) * kRecFloats;
+        if (rec[kRecState] == 1.0f) {
+            for (int l = 0; l < kLanes; ++l) rec[kRecSnap + l] = rad[l];
+            rec[kRecState] = 3.0f;
+        }
+    }
 
     // lamps met along the ray (metAlong, trace.cpp:352-374), MIS against aiming
     if (!inside) {
@@ -215,8 +322,8 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
             for (int l = 0; l < kLanes; ++l) rad[l] += thr[l] * sky[l];
         }
     } else if (hit.kind == KDN_HIT_SURFACE) {
-        const uint32_t mi = hit.leaf != KDN_NO_LEAF ? J.leaf_material[hit.leaf] : J.fallback_material;
-        const MaterialDev& m = J.materials[mi < J.n_materials ? mi : J.fallback_material];
+        uint32_t mi;
+        const MaterialDev& m = material_at(J, hit, &mi);
         const V3 x = v3(hit.point[0], hit.point[1], hit.point[2]);
         const V3 out = -d;
         V3 n = v3(hit.normal[0], hit.normal[1], hit.normal[2]);
@@ -228,6 +335,12 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
         }
         const float lift = lift_at(x);
         const bool exact = m.kind == KDN_GLASS && m.exact;
+        // the guide's lobes at this vertex (the same test as its inference)
+        const bool guidable = depth < kGuideDepths && m.kind != KDN_GLASS;
+        const bool guided = P.lobes != nullptr && guidable;
+        GuideLobes lobes;
+        if (guided) load_lobes(P.lobes, N, i, lobes);
+        const float share = P.guide_share;
         // aim at one lamp (aimedFrom, trace.cpp:896-947), MIS against meeting
         if (!exact && J.n_lamps > 0) {
             const float uPick = number(stream, sample, depth, Pick);
@@ -246,7 +359,9 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
                 rough_value(m, w, n, out, in, f);
                 spectrum_of(L.radiance, w, Le);
                 const float pdfL = L.pick * dist * dist / (cosL * L.area);
-                const float wt = power_weight(pdfL, rough_density(m, n, out, in));
+                const float pdfS = guided ? share * mixture_pdf(lobes, in) + (1 - share) * rough_density(m, n, out, in)
+                                          : rough_density(m, n, out, in);
+                const float wt = power_weight(pdfL, pdfS);
                 float c[kLanes];
                 float any = 0;
                 for (int l = 0; l < kLanes; ++l) {
@@ -275,7 +390,10 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
         if (depth + 1 < (int)J.surfaces) {
             const float uLobe = number(stream, sample, depth, Lobe), uSide = number(stream, sample, depth, Side);
             const float du = number(stream, sample, depth, DirectionU), dv = number(stream, sample, depth, DirectionV);
-            const Scatter sc = m.kind == KDN_GLASS ? glass_scatter(m, w, n, out, inside, uSide) : rough_scatter(m, w, n, out, uLobe, du, dv);
+            const Scatter sc = m.kind == KDN_GLASS ? glass_scatter(m, w, n, out, inside, uSide)
+                               : guided ? guided_scatter(m, w, n, out, lobes, share, guide_number(stream, sample, depth, 0),
+                                                         guide_number(stream, sample, depth, 1), uLobe, du, dv)
+                                        : rough_scatter(m, w, n, out, uLobe, du, dv);
             if (sc.found) {
                 const float side = dot(sc.in, n);
                 const bool below = sc.through ? side >= 0 : side <= 0;
@@ -314,6 +432,13 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
                         s.pdf[i] = sc.exact ? 0.0f : sc.density;
                         meta = (meta & ~(0xFFu | FlagExact)) | (uint32_t)(depth + 1) | (sc.exact ? FlagExact : 0u);
                         alive = true;
+                        if (training && guidable) {
+                            float* rec = P.recs + ((size_t)(i / P.train_stride) * kGuideDepths + depth) * kRecFloats;
+                            const float q[14] = {x.x, x.y, x.z, out.x, out.y, out.z, n.x, n.y, n.z, w.nm[0], sc.in.x, sc.in.y, sc.in.z, sc.density};
+                            for (int k = 0; k < 14; ++k) rec[k] = q[k];
+                            for (int l = 0; l < kLanes; ++l) rec[kRecThr + l] = carried[l];
+                            rec[kRecState] = 1.0f;
+                        }
                     }
                 }
             }

@@ -13,6 +13,7 @@ Workloads:
                four quad lights and a gradient sky, 1920x1080.
 - glass_heavy: the plain scene with 60% glass.
 - serving_pool: eight of the above, 32 to 4096 primitives.
+- window_room: a room lit by the sky through a small ceiling window.
 """
 import math
 
@@ -168,6 +169,22 @@ def production(seed=0, count=512, width=1920, height=1080, samples=64, surfaces=
         out.extend(body[1:])
     out.append("paint(floor) plane([0, 0, 1], 0);")
     return "\n".join(out) + "\n"
+
+
+def window_room(width=64, height=48, samples=16, surfaces=6):
+    """A closed grey room lit only by a bright sky through a small square
+    window in its ceiling (no lamps): every pixel's light comes through
+    0.25 m^2 of opening that BSDF sampling finds rarely, the case path
+    guiding is for."""
+    return f"""import "standard.sdf";
+render {{ width = {width}; height = {height}; seed = 1; samples = {samples}; surfaces = {surfaces}; far = 4; }}
+camera {{ eye = [-1.6, -1.6, 1.1]; target = [0.6, 0.6, 0.7]; up = [0, 0, 1]; fov = 1.1; }}
+sky gradient {{ zenith = [12, 12, 12]; horizon = [6, 6, 6]; ground = [0, 0, 0]; }}
+let wall = matte([0.75, 0.75, 0.72]);
+let red = matte([0.75, 0.2, 0.18]);
+paint(wall) cut {{ at([0, 0, 1]) box([4.4, 4.4, 2.2]); at([0, 0, 1]) box([4, 4, 2]); at([0.8, 0.8, 2.05]) box([0.5, 0.5, 0.6]); }}
+paint(red) at([0.3, -0.2, 0.35]) sphere(0.35);
+"""
 
 
 SERVING_COUNTS = (32, 64, 128, 256, 512, 1024, 2048, 4096)

@@ -1,0 +1,216 @@
+"""The learned guide on the device, through the C ABI.
+
+- inference: the lobes the device's MFMA network gives for random queries
+  equal the oracle's (oracle/guide_ref.py, BF16 roundings where the device
+  rounds) within the spread BF16 rounding leaves;
+- gradients: the device's backward pass (weights and hash grid) equals
+  the oracle's within the same spread; the oracle's own gradients are
+  pinned by finite differences (tests/test_guide_oracle.py);
+- training: Adam steps on samples drawn from a known position-dependent
+  lobe bring the loss from a near-uniform mixture's to near the target's;
+- rendering: a guided render of a room lit through a small window keeps
+  the mean of the unguided render (one-sample MIS keeps it unbiased) and,
+  once trained, reaches a lower relative MSE at the same sample count.
+"""
+import numpy as np
+import pytest
+
+import kodanuki_amd as k
+from kodanuki_amd import scenes
+from oracle import guide_ref as R
+from tests.conftest import EMULATION
+
+pytestmark = pytest.mark.gpu
+
+LOWER, SIZE = np.array([-2, -2, -1], np.float32), np.array([4, 4, 4], np.float32)
+
+
+def note(msg):
+    print(msg, flush=True)
+
+
+def _unit(rng, n):
+    v = rng.standard_normal((n, 3))
+    return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+
+
+def _params(seed):
+    rng = np.random.default_rng(seed)
+    grid = (rng.standard_normal((R.LEVELS, R.TABLE, R.FEATURES)) * 0.5).astype(np.float32)
+    mlp = rng.uniform(-0.2, 0.2, (R.LAYERS, R.WIDTH, R.WIDTH)).astype(np.float32)
+    mlp[R.LAYERS - 1] *= 0.5
+    return grid, mlp
+
+
+def _queries(rng, n):
+    q = np.zeros((n, 10), np.float32)
+    q[:, 0:3] = rng.uniform(LOWER, LOWER + SIZE, (n, 3))
+    q[:, 3:6] = _unit(rng, n)
+    q[:, 6:9] = _unit(rng, n)
+    q[:, 9] = rng.uniform(380, 780, n)
+    return q
+
+
+@pytest.fixture(scope="module")
+def guide(device):
+    device.guide_reset(3, LOWER, SIZE)
+    grid, mlp = _params(17)
+    device.guide_set_params(grid, mlp)
+    return device, R.Guide(grid, mlp, LOWER, SIZE)
+
+
+def test_reset_is_deterministic_and_described(device):
+    device.guide_reset(9, LOWER, SIZE)
+    g1, m1 = device.guide_params()
+    device.guide_reset(9, LOWER, SIZE)
+    g2, m2 = device.guide_params()
+    assert g1.tobytes() == g2.tobytes() and m1.tobytes() == m2.tobytes()
+    assert np.abs(g1).max() <= 1e-4 and 0.2 < np.abs(m1[0]).max() <= 0.3062
+    info = device.guide_info()
+    res, dense = R.resolutions()
+    assert list(info.resolution) == res and info.dense_levels == dense
+    assert info.grid_params == R.LEVELS * R.TABLE * R.FEATURES and info.mlp_params == R.LAYERS * 64 * 64
+
+
+def test_inference_matches_the_oracle(guide):
+    device, G = guide
+    rng = np.random.default_rng(5)
+    q = _queries(rng, 512)
+    got = device.guide_infer(q)
+    X, _ = R.inputs(G, q[:, 0:3], q[:, 3:6], q[:, 6:9], q[:, 9])
+    _, O = R.forward(G, X, bf=True)
+    share, axis, sharp = R.decode(O.astype(np.float64))
+    d_share = np.abs(got[:, :, 4] - share)
+    big = share > 0.05
+    dots = np.sum(got[:, :, 0:3] * axis, axis=2)
+    rel_sharp = np.abs(got[:, :, 3] / sharp - 1)
+    note(f"[guide] infer: share |d| p99 {np.quantile(d_share, 0.99):.2e} max {d_share.max():.2e}; "
+         f"axis dot p1 {np.quantile(dots[big], 0.01):.6f}; sharpness rel p99 {np.quantile(rel_sharp, 0.99):.2e}")
+    assert np.allclose(got[:, :, 4].sum(axis=1), 1, atol=1e-5)
+    assert np.allclose(np.linalg.norm(got[:, :, 0:3], axis=2), 1, atol=1e-5)
+    assert np.quantile(d_share, 0.99) < 0.02 and d_share.max() < 0.1
+    assert np.quantile(dots[big], 0.01) > 0.995
+    assert np.quantile(rel_sharp, 0.99) < 0.05
+
+
+def test_gradients_match_the_oracle(guide):
+    device, G = guide
+    rng = np.random.default_rng(7)
+    n = 256
+    S = np.zeros((n, 16), np.float32)
+    S[:, 0:10] = _queries(rng, n)
+    S[:, 10:13] = _unit(rng, n)
+    S[:, 13] = rng.uniform(0.1, 2.0, n)
+    loss, grid_grad, mlp_grad = device.guide_gradients(S)
+    ref_loss, ref_grid, ref_mlp = R.gradients(G, S, bf=True)
+    note(f"[guide] loss device {loss:.6f} oracle {ref_loss:.6f}")
+    assert abs(loss / ref_loss - 1) < 2e-2
+    for layer in range(R.LAYERS):
+        a, b = mlp_grad[layer].ravel(), ref_mlp[layer].ravel()
+        err = np.linalg.norm(a - b) / np.linalg.norm(b)
+        cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b))
+        note(f"[guide] layer {layer}: relative L2 {err:.2e}, cosine {cos:.6f}")
+        assert err < 5e-2 and cos > 0.998
+    keys = list(ref_grid)
+    a = np.array([grid_grad[l, e] for l, e in keys]).ravel()
+    b = np.array([ref_grid[key] for key in keys]).ravel()
+    err = np.linalg.norm(a - b) / np.linalg.norm(b)
+    note(f"[guide] grid: {len(keys)} entries, relative L2 {err:.2e}")
+    assert err < 5e-2
+    # nothing outside the touched entries
+    touched = np.zeros(grid_grad.shape[:2], bool)
+    for l, e in keys:
+        touched[l, e] = True
+    assert np.all(grid_grad[~touched] == 0)
+
+
+def _target_samples(rng, n):
+    """Directions from a lobe whose axis turns with the position."""
+    S = np.zeros((n, 16), np.float32)
+    p = rng.uniform(LOWER, LOWER + SIZE, (n, 3))
+    S[:, 0:3] = p
+    S[:, 3:6] = _unit(rng, n)
+    S[:, 6:9] = [0, 0, 1]
+    S[:, 9] = 550
+    axis = np.stack([np.sin(1.5 * p[:, 0]), np.cos(1.5 * p[:, 1]), np.full(n, 0.8)], axis=1)
+    axis /= np.linalg.norm(axis, axis=1, keepdims=True)
+    for j in range(n):
+        S[j, 10:13] = R.mixture_sample(np.array([1.0]), axis[j:j + 1], np.array([20.0]), np.array([0.0]),
+                                       rng.random(1), rng.random(1))[0]
+    S[:, 13] = 1
+    return S
+
+
+def test_training_fits_a_known_lobe(device):
+    rng = np.random.default_rng(11)
+    n = 4096 if EMULATION else 32768
+    S = _target_samples(rng, n)
+    device.guide_reset(21, LOWER, SIZE)
+    steps = 12 if EMULATION else 400
+    losses = device.guide_train(S, steps, min(n, 4096))
+    first, last = np.mean(losses[:4]), np.mean(losses[-8:])
+    # a uniform mixture scores log(4 pi) = 2.53; the target lobe itself
+    # about -0.16 (sharpness 20)
+    note(f"[guide] training: loss {first:.3f} -> {last:.3f} over {steps} steps")
+    assert np.all(np.isfinite(losses))
+    assert abs(first - np.log(4 * np.pi)) < 0.6
+    assert last < (first - 0.2 if EMULATION else 0.8)
+
+
+def _render(device, text, depth, spp, passes, flags, reset=None):
+    s = k.Scene(text)
+    cam, rs = s.settings()
+    lower, size = s.region(depth)
+    device.upload(s)
+    device.build_octree(lower, size, depth)
+    if reset is not None:
+        device.guide_reset(reset)
+    stats = []
+    for p in range(passes):
+        stats.append(device.render_pass(rs.width, rs.height, spp, first_sample=p * spp, seed=rs.seed,
+                                        surfaces=rs.surfaces, keep=p > 0, flags=flags))
+    rgb, sq, n = device.accumulation(rs.width, rs.height)
+    return rgb / n[..., None], sq, n, stats
+
+
+def test_guided_passes_train_and_sample(device):
+    """A few small guided passes: the first trains the guide (no guided
+    sampling yet), the later ones sample with it; the image stays finite
+    and near the plain render's mean."""
+    text = scenes.window_room(24, 16)
+    plain, _, _, _ = _render(device, text, 6, 16, 3, 0)
+    guided, _, _, st = _render(device, text, 6, 16, 3, k.PASS_GUIDE, reset=2)
+    info = device.guide_info()
+    note(f"[guide] small guided passes: steps {info.steps}, samples {[x.guide_samples for x in st]}, "
+         f"means plain {plain.mean():.4f} guided {guided.mean():.4f}")
+    assert np.all(np.isfinite(guided))
+    assert info.steps > 0 and st[0].guide_steps > 0
+    assert abs(guided.mean() / plain.mean() - 1) < 0.25
+
+
+@pytest.mark.skipif(bool(EMULATION), reason="thousands of samples per pixel: device only")
+def test_guided_render_is_unbiased_and_gains(device):
+    text = scenes.window_room(64, 48)
+    ref, _, _, _ = _render(device, text, 6, 256, 8, 0)
+    plain, psq, pn, _ = _render(device, text, 6, 4, 16, 0)
+    guided, gsq, gn, gst = _render(device, text, 6, 4, 16, k.PASS_GUIDE, reset=1)
+    info = device.guide_info()
+
+    def rel_mse(img):
+        return float(np.mean((img - ref) ** 2 / (ref ** 2 + 1e-2)))
+
+    rp, rg = rel_mse(plain), rel_mse(guided)
+    note(f"[guide] window room 64x48 at 64 spp: relMSE plain {rp:.4e} guided {rg:.4e} (x{rp / rg:.2f}); "
+         f"guide steps {info.steps}, samples per pass {[st.guide_samples for st in gst[:3]]}, "
+         f"loss {info.last_loss:.3f}, guide {sum(st.guide_ms for st in gst):.1f} ms")
+    assert info.steps > 0 and all(st.guide_samples > 0 for st in gst)
+    # unbiased: block means of the guided render agree with the reference
+    def blocks(img):
+        return img.reshape(6, 8, 8, 8, 3).mean(axis=(1, 3))
+    var = gsq / gn[..., None] - guided ** 2
+    se = np.sqrt(np.maximum(var, 0).reshape(6, 8, 8, 8, 3).sum(axis=(1, 3)) / (64 * 64 * 64))
+    z = np.abs(blocks(guided) - blocks(ref)) / (se + 1e-6)
+    note(f"[guide] block |z| max {z.max():.2f} mean {z.mean():.2f}")
+    assert np.mean(z < 4) > 0.97
+    assert abs(guided.mean() / ref.mean() - 1) < 0.03
+    assert rg < 0.8 * rp
