@@ -137,6 +137,70 @@ def serving_stream(k, scenes, dev, jobs, rate, seed, max_paths, budget, sizes=No
             "mean_paths_per_round": paths_in_rounds / max(rounds, 1), "max_resident_bytes": max_resident}
 
 
+MLP_FLOPS_PER_QUERY = 2 * 64 * 64 * 4  # four 64x64 layers, multiply-adds as two
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def guided_leg(k, dev, label, text, depth, W, H, spp, passes):
+    """Plain and learned-guided renders of the same samples of one image.
+
+    Both estimators are unbiased (the guided one by one-sample MIS; its
+    mean is checked against a reference in the tests), so each one's MSE is
+    its variance: the relative MSE of a method is the mean over pixels of
+    (per-pixel sample variance / samples) / (mean^2 + 0.01), from its own
+    FP64 accumulators. Equal-MSE Msamples/s = samples / seconds x
+    (relMSE of the plain render / relMSE of the method): the plain
+    render's rate scaled by how many fewer samples the method needs for
+    the same error. The guided passes train the guide as they go (the
+    first pass samples with the BSDF only); their time includes inference
+    and training."""
+    import numpy as np
+    scene = k.Scene(text)
+    cam, rs = scene.settings()
+    lower, size = scene.region(depth)
+    dev.upload(scene)
+    dev.build_octree(lower, size, depth)
+    out = {}
+    for method, flags in (("plain", 0), ("guided", k.PASS_GUIDE)):
+        if flags:
+            dev.guide_reset(1)
+        stats = []
+        start = time.perf_counter()
+        for p in range(passes):
+            stats.append(dev.render_pass(W, H, spp, first_sample=p * spp, seed=rs.seed, surfaces=rs.surfaces, keep=p > 0,
+                                         flags=flags | k.PASS_TIME))
+        dt = time.perf_counter() - start
+        rgb, sq, n = dev.accumulation(W, H)
+        mean = rgb / n[..., None]
+        var = np.maximum(sq / n[..., None] - mean ** 2, 0) / n[..., None]
+        out[method] = dict(seconds=dt, mean=mean, var=var, stats=stats)
+    denom = out["plain"]["mean"] ** 2 + 1e-2
+    samples = W * H * spp * passes
+    res = {"workload": f"{label} {W}x{H}, {passes} passes of {spp} spp, octree depth {depth}, {rs.surfaces} surfaces"}
+    rel_plain = float(np.mean(out["plain"]["var"] / denom))
+    for method in ("plain", "guided"):
+        rel = float(np.mean(out[method]["var"] / denom))
+        rate = samples / out[method]["seconds"] / 1e6
+        res[method] = {"msamples_s": rate, "relmse": rel, "equal_mse_msamples_s": rate * rel_plain / rel,
+                       "seconds": out[method]["seconds"]}
+    g = out["guided"]["stats"]
+    infer_ms = sum(st.guide_infer_ms for st in g)
+    queries = sum(st.guide_queries for st in g)
+    res["guided"].update({"guide_ms": sum(st.guide_ms for st in g), "guide_infer_ms": infer_ms,
+                          "guide_queries": queries, "guide_train_steps": sum(st.guide_steps for st in g),
+                          "guide_samples": sum(st.guide_samples for st in g)})
+    if infer_ms > 0:
+        achieved = queries * MLP_FLOPS_PER_QUERY / (infer_ms * 1e-3) / 1e12
+        res["guided"]["mlp_mfma"] = {"kernel": "k_guide_infer (hash-grid encoding, 4 layers on BF16 MFMA, lobes)",
+                                     "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                     "frac": achieved / BF16_PEAK_TFLOPS}
+    res["gain"] = res["guided"]["equal_mse_msamples_s"] / res["plain"]["equal_mse_msamples_s"]
+    print(f"[guided] {label}: plain {res['plain']['msamples_s']:.2f} Ms/s relMSE {res['plain']['relmse']:.3e}; guided "
+          f"{res['guided']['msamples_s']:.2f} Ms/s relMSE {res['guided']['relmse']:.3e}; equal-MSE gain {res['gain']:.2f}",
+          file=sys.stderr, flush=True)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +212,8 @@ def main():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    ap.add_argument("--no-guide", action="store_true", help="skip the guided renders")
+    ap.add_argument("--guide-passes", type=int, default=8)
     ap.add_argument("--serve-jobs", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals, jobs/s")
     ap.add_argument("--serve-seed", type=int, default=7)
@@ -245,6 +311,13 @@ def main():
                                     "shadow": first_stats.shadow_ms, "finish": first_stats.finish_ms,
                                     "other": first_stats.other_ms}}
 
+    guided = None
+    if not args.no_guide:
+        guided = [guided_leg(k, dev, "production scene (512 primitives, glass as white matte)", text, args.depth, W, H, S,
+                             args.guide_passes),
+                  guided_leg(k, dev, "window room (sky through a ceiling window)", scenes.window_room(W, H), 6, W, H, S,
+                             args.guide_passes)]
+
     serving = None
     if not args.no_serve:
         serving = serving_stream(k, scenes, dev, args.serve_jobs, args.serve_rate, args.serve_seed, args.serve_paths,
@@ -272,6 +345,7 @@ def main():
                                f"materials painted as white mattes (glass scattering held off the device)",
                    "method": "plain", "segments_per_step": first_stats.segments, "shadow_rays_per_step": first_stats.shadow_rays},
         "roofline": roofline,
+        "guided": guided,
         "serving": serving,
         "cpu_baseline": cpu,
     }

@@ -335,7 +335,9 @@ typedef struct kdn_pass_stats {
     uint32_t shadow_launches;
     uint32_t finish_launches;
     double guide_ms;         /* the guide's inference and training kernels */
+    double guide_infer_ms;   /* its inference launches alone */
     double guide_loss;       /* its last training step's loss */
+    uint64_t guide_queries;  /* vertices its inference launches took */
     uint32_t guide_samples;  /* training samples the pass gave */
     uint32_t guide_steps;    /* Adam steps taken */
 } kdn_pass_stats;
@@ -401,8 +403,20 @@ typedef struct kdn_guide_info {
     float lower[3], size[3]; /* the region positions are scaled from */
 } kdn_guide_info;
 
-/* Initialise the network from a seed over a region (NULL: the octree's). */
+/* How a guided pass uses and trains the guide. */
+typedef struct kdn_guide_config {
+    float learning_rate;     /* Adam (default 1e-2) */
+    float share;             /* the guide's share of sampled directions (0.5) */
+    float epochs;            /* passes over a round's samples (2) */
+    uint32_t min_steps;      /* Adam steps a round, at least (4) */
+    uint32_t max_steps;      /* and at most (64, the limit) */
+    uint32_t pad;
+} kdn_guide_config;
+
+/* Initialise the network from a seed over a region (NULL: the octree's);
+ * the configuration returns to its defaults. */
 int kdn_guide_reset(kdn_ctx* ctx, uint32_t seed, const float* lower3, const float* size3);
+int kdn_guide_configure(kdn_ctx* ctx, const kdn_guide_config* config);
 int kdn_guide_describe(kdn_ctx* ctx, kdn_guide_info* info);
 /* FP32 parameters: grid[grid_params] (level, entry, feature) and
  * mlp[mlp_params] (layer, input, output); NULL skips one. */

@@ -102,8 +102,9 @@ class PassStats(C.Structure):
                 ("trace_launches", C.c_uint32), ("trace_ms", C.c_double), ("shadow_ms", C.c_double), ("shade_ms", C.c_double),
                 ("other_ms", C.c_double), ("trace_flops", C.c_double), ("shadow_flops", C.c_double),
                 ("finish_ms", C.c_double), ("finish_flops", C.c_double), ("shadow_launches", C.c_uint32),
-                ("finish_launches", C.c_uint32), ("guide_ms", C.c_double), ("guide_loss", C.c_double),
-                ("guide_samples", C.c_uint32), ("guide_steps", C.c_uint32)]
+                ("finish_launches", C.c_uint32), ("guide_ms", C.c_double), ("guide_infer_ms", C.c_double),
+                ("guide_loss", C.c_double), ("guide_queries", C.c_uint64), ("guide_samples", C.c_uint32),
+                ("guide_steps", C.c_uint32)]
 
 
 PASS_KEEP, PASS_TIME, PASS_COUNT, PASS_NO_FINISH, PASS_GUIDE = 1, 2, 4, 8, 16
@@ -116,6 +117,11 @@ class GuideInfo(C.Structure):
                 ("loss_scale", C.c_float), ("share", C.c_float), ("last_loss", C.c_double), ("last_samples", C.c_uint32),
                 ("pad", C.c_uint32), ("grid_params", C.c_uint64), ("mlp_params", C.c_uint64), ("lower", C.c_float * 3),
                 ("size", C.c_float * 3)]
+
+
+class GuideConfig(C.Structure):
+    _fields_ = [("learning_rate", C.c_float), ("share", C.c_float), ("epochs", C.c_float), ("min_steps", C.c_uint32),
+                ("max_steps", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class GuideQuery(C.Structure):
@@ -168,7 +174,7 @@ EXPORTS = [
     "kdn_scene_region", "kdn_open", "kdn_close", "kdn_last_error", "kdn_global_error", "kdn_upload_scene",
     "kdn_build_octree", "kdn_octree_sizes", "kdn_octree_download", "kdn_extend", "kdn_extend_device",
     "kdn_render_pass", "kdn_read_accumulation", "kdn_read_image",
-    "kdn_guide_reset", "kdn_guide_describe", "kdn_guide_params", "kdn_guide_set_params", "kdn_guide_infer",
+    "kdn_guide_reset", "kdn_guide_configure", "kdn_guide_describe", "kdn_guide_params", "kdn_guide_set_params", "kdn_guide_infer",
     "kdn_guide_gradients", "kdn_guide_train",
     "kdn_serve_open", "kdn_serve_submit", "kdn_serve_step", "kdn_serve_status", "kdn_serve_result",
     "kdn_serve_release", "kdn_serve_error", "kdn_serve_job_error", "kdn_serve_close",
@@ -222,6 +228,7 @@ def _bind(lib):
         "kdn_read_image": (C.c_int, [P, P]),
         "kdn_guide_reset": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "kdn_guide_describe": (C.c_int, [P, C.POINTER(GuideInfo)]),
+        "kdn_guide_configure": (C.c_int, [P, C.POINTER(GuideConfig)]),
         "kdn_guide_params": (C.c_int, [P, P, P]),
         "kdn_guide_set_params": (C.c_int, [P, P, P]),
         "kdn_guide_infer": (C.c_int, [P, P, C.c_uint32, P]),
@@ -428,6 +435,10 @@ class Device:
         lo = (C.c_float * 3)(*[float(x) for x in lower]) if lower is not None else None
         sz = (C.c_float * 3)(*[float(x) for x in size]) if size is not None else None
         _check(library().kdn_guide_reset(self._h, seed, lo, sz), self._h)
+
+    def guide_configure(self, learning_rate=1e-2, share=0.5, epochs=2.0, min_steps=4, max_steps=64):
+        cfg = GuideConfig(learning_rate, share, epochs, min_steps, max_steps, 0)
+        _check(library().kdn_guide_configure(self._h, C.byref(cfg)), self._h)
 
     def guide_info(self):
         info = GuideInfo()

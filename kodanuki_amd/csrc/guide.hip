@@ -30,9 +30,9 @@
 namespace kdn {
 
 constexpr uint32_t kGuideBatch = 1u << 16;
-constexpr uint32_t kGuideStepsPerRound = 16;
+constexpr uint32_t kGuideStepsPerRound = 64;  // Adam steps of one round, at most
 constexpr uint32_t kGuideMinSamples = 64;
-constexpr float kGuideLearningRate = 5e-3f;
+constexpr float kGuideLearningRate = 1e-2f;
 constexpr float kGuideBeta1 = 0.9f, kGuideBeta2 = 0.99f, kGuideEps = 1e-8f;
 
 struct GuideState {
@@ -46,6 +46,7 @@ struct GuideState {
     DevBuf partial, lobes, recs, samples, flags, offsets, scan_tmp;
     uint32_t steps = 0;
     float loss_scale = 128.0f;
+    kdn_guide_config config{kGuideLearningRate, kGuideShare, 2.0f, 4, kGuideStepsPerRound, 0};
     double last_loss = 0;
     uint32_t last_samples = 0, last_steps = 0;
     float* gridM() const { return grid + kGuideGridParams; }
@@ -57,7 +58,7 @@ struct GuideState {
 };
 
 constexpr int kScalarFlags = 2;
-constexpr int kScalarParts = 64;  // 256 doubles of weight sums from here
+constexpr int kScalarParts = kScalarFlags + 2 * kGuideStepsPerRound + 2;  // 256 doubles of weight sums from here
 constexpr int kScalarWords = kScalarParts + 2 * 256;
 
 namespace {
@@ -247,7 +248,7 @@ bool trainSteps(Ctx& c, GuideState& G, const float* samples, uint32_t n, uint32_
                            flags + k);
         hipLaunchKernelGGL(k_guide_check, dim3(2048), dim3(256), 0, st, (const float*)G.gridG(), kGuideGridParams, flags + k);
         const uint32_t t = G.steps + k + 1;
-        const AdamStep a{kGuideLearningRate, 1.0f / G.loss_scale, (float)(1 - std::pow((double)kGuideBeta1, t)),
+        const AdamStep a{G.config.learning_rate, 1.0f / G.loss_scale, (float)(1 - std::pow((double)kGuideBeta1, t)),
                          (float)(1 - std::pow((double)kGuideBeta2, t))};
         hipLaunchKernelGGL(k_guide_adam, dim3(kGuideMlpParams / 256), dim3(256), 0, st, G.mlp, G.mlpM(), G.mlpV(), G.mlpG(),
                            (uint64_t)kGuideMlpParams, a, (const uint32_t*)(flags + k), G.mlp_bf);
@@ -295,6 +296,7 @@ bool guideReset(Ctx& c, uint32_t seed, const float lower[3], const float size[3]
     KDN_HIP(hipGetLastError());
     KDN_HIP(hipStreamSynchronize(st));
     G.steps = 0;
+    G.config = kdn_guide_config{kGuideLearningRate, kGuideShare, 2.0f, 4, kGuideStepsPerRound, 0};
     G.loss_scale = 128.0f;
     G.last_loss = 0;
     return true;
@@ -329,7 +331,7 @@ bool guideBeginRound(Ctx& c, PassParams& P, uint32_t paths) {
     P.lobes = G.steps > 0 ? (const float*)G.lobes.ptr : nullptr;
     P.recs = (float*)G.recs.ptr;
     P.train_stride = stride;
-    P.guide_share = G.steps > 0 ? kGuideShare : 0.0f;
+    P.guide_share = G.steps > 0 ? G.config.share : 0.0f;
     return true;
 }
 
@@ -366,9 +368,10 @@ bool guideEndRound(Ctx& c, const PassParams& P, uint32_t paths, kdn_pass_stats* 
     G.last_samples = n;
     G.last_steps = 0;
     if (n >= kGuideMinSamples) {
-        // two passes over the samples, in 4 to 16 steps
+        // config.epochs passes over the samples, in min_steps to max_steps steps
         const uint32_t batch = std::min(kGuideBatch, n);
-        const uint32_t steps = std::max(4u, std::min(kGuideStepsPerRound, (uint32_t)(2ull * n / batch)));
+        const uint32_t want = (uint32_t)std::ceil(G.config.epochs * n / (double)batch);
+        const uint32_t steps = std::max(G.config.min_steps, std::min(G.config.max_steps, want));
         if (!trainSteps(c, G, packed, n, 0, steps, batch, nullptr)) return false;
     }
     if (stats) {
@@ -433,7 +436,7 @@ int kdn_guide_describe(kdn_ctx* ctx, kdn_guide_info* info) {
     info->dense_levels = G->view.dense;
     info->steps = G->steps;
     info->loss_scale = G->loss_scale;
-    info->share = kGuideShare;
+    info->share = G->config.share;
     info->last_loss = G->last_loss;
     info->last_samples = G->last_samples;
     info->grid_params = kGuideGridParams;
@@ -442,6 +445,20 @@ int kdn_guide_describe(kdn_ctx* ctx, kdn_guide_info* info) {
         info->lower[k] = G->lower[k];
         info->size[k] = G->size[k];
     }
+    return KDN_OK;
+}
+
+int kdn_guide_configure(kdn_ctx* ctx, const kdn_guide_config* config) {
+    if (!ctx || !config) return KDN_E_ARGUMENT;
+    GuideState* G = stateOf(ctx->c);
+    if (!G) return guideFail(ctx, "kdn_guide_configure: the guide is not initialised (kdn_guide_reset)", KDN_E_STATE);
+    const kdn_guide_config& k = *config;
+    if (!(k.learning_rate > 0 && k.learning_rate < 1) || !(k.share >= 0 && k.share <= 1) || !(k.epochs > 0 && k.epochs <= 64) ||
+        k.min_steps > k.max_steps || k.max_steps > kGuideStepsPerRound) {
+        return guideFail(ctx, "kdn_guide_configure: learning rate in (0, 1), share in [0, 1], epochs in (0, 64], "
+                              "min_steps <= max_steps <= 64", KDN_E_ARGUMENT);
+    }
+    G->config = k;
     return KDN_OK;
 }
 

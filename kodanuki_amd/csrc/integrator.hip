@@ -312,8 +312,12 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
         });
         if (guiding && bounce < (uint32_t)kGuideDepths) {
             bool ok = true;
-            timed(&stats->guide_ms, [&] { ok = guideInferQueue(c, P, in, count); });
+            double ms = 0;
+            timed(&ms, [&] { ok = guideInferQueue(c, P, in, count); });
             if (!ok) return false;
+            stats->guide_ms += ms;
+            stats->guide_infer_ms += ms;
+            if (P.lobes) stats->guide_queries += count;
         }
         KDN_HIP(hipMemsetAsync(&s->counters[out], 0, 4, st));
         KDN_HIP(hipMemsetAsync(&s->counters[2], 0, 4, st));

@@ -9,8 +9,9 @@
 - training: Adam steps on samples drawn from a known position-dependent
   lobe bring the loss from a near-uniform mixture's to near the target's;
 - rendering: a guided render of a room lit through a small window keeps
-  the mean of the unguided render (one-sample MIS keeps it unbiased) and,
-  once trained, reaches a lower relative MSE at the same sample count.
+  the mean of a high-sample reference (one-sample MIS keeps it unbiased)
+  and, training as it renders, reaches a relative MSE at least 1.3 times
+  lower than the plain render's at the same sample count (1.6 measured).
 """
 import numpy as np
 import pytest
@@ -157,7 +158,7 @@ def test_training_fits_a_known_lobe(device):
     assert last < (first - 0.2 if EMULATION else 0.8)
 
 
-def _render(device, text, depth, spp, passes, flags, reset=None):
+def _render(device, text, depth, spp, passes, flags, reset=None, first=0):
     s = k.Scene(text)
     cam, rs = s.settings()
     lower, size = s.region(depth)
@@ -167,7 +168,7 @@ def _render(device, text, depth, spp, passes, flags, reset=None):
         device.guide_reset(reset)
     stats = []
     for p in range(passes):
-        stats.append(device.render_pass(rs.width, rs.height, spp, first_sample=p * spp, seed=rs.seed,
+        stats.append(device.render_pass(rs.width, rs.height, spp, first_sample=first + p * spp, seed=rs.seed,
                                         surfaces=rs.surfaces, keep=p > 0, flags=flags))
     rgb, sq, n = device.accumulation(rs.width, rs.height)
     return rgb / n[..., None], sq, n, stats
@@ -190,8 +191,8 @@ def test_guided_passes_train_and_sample(device):
 
 @pytest.mark.skipif(bool(EMULATION), reason="thousands of samples per pixel: device only")
 def test_guided_render_is_unbiased_and_gains(device):
-    text = scenes.window_room(64, 48)
-    ref, _, _, _ = _render(device, text, 6, 256, 8, 0)
+    text = scenes.window_room(256, 192)
+    ref, _, _, _ = _render(device, text, 6, 256, 4, 0, first=1 << 20)
     plain, psq, pn, _ = _render(device, text, 6, 4, 16, 0)
     guided, gsq, gn, gst = _render(device, text, 6, 4, 16, k.PASS_GUIDE, reset=1)
     info = device.guide_info()
@@ -200,17 +201,17 @@ def test_guided_render_is_unbiased_and_gains(device):
         return float(np.mean((img - ref) ** 2 / (ref ** 2 + 1e-2)))
 
     rp, rg = rel_mse(plain), rel_mse(guided)
-    note(f"[guide] window room 64x48 at 64 spp: relMSE plain {rp:.4e} guided {rg:.4e} (x{rp / rg:.2f}); "
+    note(f"[guide] window room 256x192 at 64 spp: relMSE plain {rp:.4e} guided {rg:.4e} (x{rp / rg:.2f}); "
          f"guide steps {info.steps}, samples per pass {[st.guide_samples for st in gst[:3]]}, "
          f"loss {info.last_loss:.3f}, guide {sum(st.guide_ms for st in gst):.1f} ms")
     assert info.steps > 0 and all(st.guide_samples > 0 for st in gst)
     # unbiased: block means of the guided render agree with the reference
     def blocks(img):
-        return img.reshape(6, 8, 8, 8, 3).mean(axis=(1, 3))
+        return img.reshape(12, 16, 16, 16, 3).mean(axis=(1, 3))
     var = gsq / gn[..., None] - guided ** 2
-    se = np.sqrt(np.maximum(var, 0).reshape(6, 8, 8, 8, 3).sum(axis=(1, 3)) / (64 * 64 * 64))
+    se = np.sqrt(np.maximum(var, 0).reshape(12, 16, 16, 16, 3).sum(axis=(1, 3)) / (64 * 256 * 256))
     z = np.abs(blocks(guided) - blocks(ref)) / (se + 1e-6)
     note(f"[guide] block |z| max {z.max():.2f} mean {z.mean():.2f}")
     assert np.mean(z < 4) > 0.97
     assert abs(guided.mean() / ref.mean() - 1) < 0.03
-    assert rg < 0.8 * rp
+    assert rg < rp / 1.3
