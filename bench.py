@@ -147,8 +147,10 @@ def guided_leg(k, dev, label, text, depth, W, H, spp, passes):
     Both estimators are unbiased (the guided one by one-sample MIS; its
     mean is checked against a reference in the tests), so each one's MSE is
     its variance: the relative MSE of a method is the mean over pixels of
-    (per-pixel sample variance / samples) / (mean^2 + 0.01), from its own
-    FP64 accumulators. Equal-MSE Msamples/s = samples / seconds x
+    (per-pixel sample variance / samples) / (reference^2 + 0.01), the
+    variance from the method's own FP64 accumulators, the reference a plain
+    render of twice the samples on other sample indices (so that neither
+    method's own noise enters the normalisation). Equal-MSE Msamples/s = samples / seconds x
     (relMSE of the plain render / relMSE of the method): the plain
     render's rate scaled by how many fewer samples the method needs for
     the same error. The guided passes train the guide as they go (the
@@ -160,6 +162,10 @@ def guided_leg(k, dev, label, text, depth, W, H, spp, passes):
     lower, size = scene.region(depth)
     dev.upload(scene)
     dev.build_octree(lower, size, depth)
+    for p in range(2 * passes):
+        dev.render_pass(W, H, spp, first_sample=(1 << 20) + p * spp, seed=rs.seed, surfaces=rs.surfaces, keep=p > 0)
+    rgb, _, n = dev.accumulation(W, H)
+    denom = (rgb / n[..., None]) ** 2 + 1e-2
     out = {}
     for method, flags in (("plain", 0), ("guided", k.PASS_GUIDE)):
         if flags:
@@ -174,7 +180,6 @@ def guided_leg(k, dev, label, text, depth, W, H, spp, passes):
         mean = rgb / n[..., None]
         var = np.maximum(sq / n[..., None] - mean ** 2, 0) / n[..., None]
         out[method] = dict(seconds=dt, mean=mean, var=var, stats=stats)
-    denom = out["plain"]["mean"] ** 2 + 1e-2
     samples = W * H * spp * passes
     res = {"workload": f"{label} {W}x{H}, {passes} passes of {spp} spp, octree depth {depth}, {rs.surfaces} surfaces"}
     rel_plain = float(np.mean(out["plain"]["var"] / denom))

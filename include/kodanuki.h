@@ -352,7 +352,8 @@ int kdn_read_image(kdn_ctx* ctx, float* rgb);
 /* ---- the learned guide ------------------------------------------------------ */
 /*
  * A network that learns, per scene, where light arrives from: a hash grid
- * of the position (16 levels of 2^19 entries, 2 features each), real
+ * of the position (16 levels of 2^19 entries, 2 features each, 16 to 128
+ * cells a side by default), real
  * spherical harmonics of the outgoing direction, the normal and a code of
  * the hero wavelength feed three ReLU layers of 64 (BF16 weights, MFMA
  * with FP32 accumulation) whose outputs decode to a mixture of 8 von
@@ -408,13 +409,16 @@ typedef struct kdn_guide_config {
     float learning_rate;     /* Adam (default 1e-2) */
     float share;             /* the guide's share of sampled directions (0.5) */
     float epochs;            /* passes over a round's samples (2) */
-    uint32_t min_steps;      /* Adam steps a round, at least (4) */
+    uint32_t min_steps;      /* Adam steps a round, at least (16) */
     uint32_t max_steps;      /* and at most (64, the limit) */
+    uint32_t max_resolution; /* the finest grid level's cells per side (128) */
+    uint32_t train_paths;    /* paths of a round that keep training records, at most (2^20) */
     uint32_t pad;
 } kdn_guide_config;
 
 /* Initialise the network from a seed over a region (NULL: the octree's);
- * the configuration returns to its defaults. */
+ * the configuration returns to its defaults. A new max_resolution
+ * re-spaces the grid's levels (configure before training). */
 int kdn_guide_reset(kdn_ctx* ctx, uint32_t seed, const float* lower3, const float* size3);
 int kdn_guide_configure(kdn_ctx* ctx, const kdn_guide_config* config);
 int kdn_guide_describe(kdn_ctx* ctx, kdn_guide_info* info);

@@ -29,11 +29,11 @@ constexpr int kGuideLayers = 4;           // weight matrices: 3 hidden ReLU laye
 constexpr int kGuideLobes = 8;
 constexpr int kGuideOutputs = 5 * kGuideLobes;  // logit, axis (3), log sharpness per lobe
 constexpr int kGuideDepths = 4;           // guided vertices: path depths 0 .. 3
-constexpr uint32_t kGuideTrainPaths = 1u << 18;  // paths of a round that keep training records, at most
+constexpr uint32_t kGuideTrainPaths = 1u << 20;  // paths of a round that keep training records, at most (default)
 constexpr float kGuideShare = 0.5f;       // the guide's share of directions (one-sample MIS)
 constexpr float kGuideLogSharpMin = -3.0f;  // sharpness within [e^-3, e^6.2]
 constexpr float kGuideLogSharpMax = 6.2f;
-constexpr int kGuideMinRes = 16, kGuideMaxRes = 2048;
+constexpr int kGuideMinRes = 16, kGuideMaxRes = 128;  // grid resolutions (the finest by default)
 constexpr uint64_t kGuideGridParams = (uint64_t)kGuideLevels * kGuideTable * kGuideFeatures;
 constexpr int kGuideMlpParams = kGuideLayers * kGuideWidth * kGuideWidth;
 // input layout: grid features, SH of the outgoing direction, the normal,

@@ -5,7 +5,7 @@
 // the folds) and the C ABI of the guide.
 //
 // Training: each round keeps records of the guided vertices of one path in
-// a stride (at most 2^18 paths a round; point, outgoing direction, normal, hero wavelength, the
+// a stride (at most 2^20 paths a round by default; point, outgoing direction, normal, hero wavelength, the
 // direction taken and its density, the throughput after the vertex, the
 // radiance the path held when its next vertex began). After the round a
 // record's sample weighs the radiance that came back along the direction
@@ -13,7 +13,8 @@
 // the live lanes) by its density: the Monte Carlo estimate of the
 // gradient of the KL divergence from the incident radiance to the
 // mixture (Neural Importance Sampling, Mueller et al. 2019). Adam steps
-// on batches of 65536 samples, weights normalised by their mean, a loss
+// (16 to 64 a round) on batches of up to 65536 samples, weights normalised
+// by their mean, a loss
 // scale on the backward pass (halved when a step meets a non-finite
 // gradient, which skips the step).
 #include <hip/hip_runtime.h>
@@ -35,6 +36,10 @@ constexpr uint32_t kGuideMinSamples = 64;
 constexpr float kGuideLearningRate = 1e-2f;
 constexpr float kGuideBeta1 = 0.9f, kGuideBeta2 = 0.99f, kGuideEps = 1e-8f;
 
+constexpr kdn_guide_config defaultConfig() {
+    return kdn_guide_config{kGuideLearningRate, kGuideShare, 2.0f, 16, kGuideStepsPerRound, kGuideMaxRes, kGuideTrainPaths, 0};
+}
+
 struct GuideState {
     GuideView view{};
     uint32_t seed = 0;
@@ -46,7 +51,7 @@ struct GuideState {
     DevBuf partial, lobes, recs, samples, flags, offsets, scan_tmp;
     uint32_t steps = 0;
     float loss_scale = 128.0f;
-    kdn_guide_config config{kGuideLearningRate, kGuideShare, 2.0f, 4, kGuideStepsPerRound, 0};
+    kdn_guide_config config = defaultConfig();
     double last_loss = 0;
     uint32_t last_samples = 0, last_steps = 0;
     float* gridM() const { return grid + kGuideGridParams; }
@@ -187,10 +192,10 @@ __global__ void k_guide_set_scale(float* scale, float value) {
 
 GuideState* stateOf(Ctx& c) { return (GuideState*)c.guide; }
 
-void viewOf(GuideState& G) {
+void viewOf(GuideState& G, uint32_t maxRes) {
     G.view.grid = G.grid;
     G.view.mlp = G.mlp_bf;
-    const double growth = std::exp((std::log((double)kGuideMaxRes) - std::log((double)kGuideMinRes)) / (kGuideLevels - 1));
+    const double growth = std::exp((std::log((double)maxRes) - std::log((double)kGuideMinRes)) / (kGuideLevels - 1));
     G.view.dense = 0;
     for (int l = 0; l < kGuideLevels; ++l) {
         const uint32_t res = (uint32_t)std::floor(kGuideMinRes * std::pow(growth, l) + 0.5);
@@ -289,14 +294,14 @@ bool guideReset(Ctx& c, uint32_t seed, const float lower[3], const float size[3]
         G.lower[k] = lower[k];
         G.size[k] = size[k];
     }
-    viewOf(G);
+    viewOf(G, G.config.max_resolution);
     KDN_HIP(hipMemsetAsync(G.gridM(), 0, 3 * kGuideGridParams * sizeof(float), st));
     KDN_HIP(hipMemsetAsync(G.mlpM(), 0, 3 * (size_t)kGuideMlpParams * sizeof(float), st));
     hipLaunchKernelGGL(k_guide_init, dim3((uint32_t)((kGuideGridParams + 255) / 256)), dim3(256), 0, st, G.grid, G.mlp, G.mlp_bf, seed);
     KDN_HIP(hipGetLastError());
     KDN_HIP(hipStreamSynchronize(st));
     G.steps = 0;
-    G.config = kdn_guide_config{kGuideLearningRate, kGuideShare, 2.0f, 4, kGuideStepsPerRound, 0};
+    G.config = defaultConfig();
     G.loss_scale = 128.0f;
     G.last_loss = 0;
     return true;
@@ -324,7 +329,7 @@ bool guideBeginRound(Ctx& c, PassParams& P, uint32_t paths) {
     const size_t N = c.paths->capacity;
     if (!ensure(G.lobes, N * kGuideOutputs * sizeof(float), c.error, false)) return false;
     uint32_t stride = 1;
-    while ((uint64_t)stride * kGuideTrainPaths < paths) stride *= 2;
+    while ((uint64_t)stride * G.config.train_paths < paths) stride *= 2;
     const size_t nRec = (size_t)((paths + stride - 1) / stride) * kGuideDepths;
     if (!ensure(G.recs, nRec * kRecFloats * sizeof(float), c.error, false)) return false;
     KDN_HIP(hipMemsetAsync(G.recs.ptr, 0, nRec * kRecFloats * sizeof(float), c.stream));
@@ -431,7 +436,7 @@ int kdn_guide_describe(kdn_ctx* ctx, kdn_guide_info* info) {
     info->layers = kGuideLayers;
     info->lobes = kGuideLobes;
     info->depths = kGuideDepths;
-    info->train_paths = kGuideTrainPaths;
+    info->train_paths = G->config.train_paths;
     for (int l = 0; l < kGuideLevels; ++l) info->resolution[l] = G->view.res[l];
     info->dense_levels = G->view.dense;
     info->steps = G->steps;
@@ -454,11 +459,14 @@ int kdn_guide_configure(kdn_ctx* ctx, const kdn_guide_config* config) {
     if (!G) return guideFail(ctx, "kdn_guide_configure: the guide is not initialised (kdn_guide_reset)", KDN_E_STATE);
     const kdn_guide_config& k = *config;
     if (!(k.learning_rate > 0 && k.learning_rate < 1) || !(k.share >= 0 && k.share <= 1) || !(k.epochs > 0 && k.epochs <= 64) ||
-        k.min_steps > k.max_steps || k.max_steps > kGuideStepsPerRound) {
+        k.min_steps > k.max_steps || k.max_steps > kGuideStepsPerRound || k.max_resolution < kGuideMinRes ||
+        k.max_resolution > 65536 || k.train_paths < 1024 || k.train_paths > (1u << 22)) {
         return guideFail(ctx, "kdn_guide_configure: learning rate in (0, 1), share in [0, 1], epochs in (0, 64], "
-                              "min_steps <= max_steps <= 64", KDN_E_ARGUMENT);
+                              "min_steps <= max_steps <= 64, max_resolution in [16, 65536], train_paths in [2^10, 2^22]",
+                         KDN_E_ARGUMENT);
     }
     G->config = k;
+    viewOf(*G, k.max_resolution);
     return KDN_OK;
 }
 

@@ -6,8 +6,8 @@ product (libkodanuki_hip.so) never calls it.
 A numpy restatement of the guide the library runs (the reference has no
 learned guide; it joins the direction sampling of
 plugins/transport/paths/sampler.h:23,357):
-- the input encoding: a hash grid of 16 levels (resolutions 16 .. 2048 in
-  a geometric series, rounded to nearest; 2^19 entries a level, two
+- the input encoding: a hash grid of 16 levels (resolutions 16 .. 128 by
+  default in a geometric series, rounded to nearest; 2^19 entries a level, two
   features; levels whose (res + 1)^3 corners fit indexed densely, the
   others by the spatial hash
   x * 1 ^ y * 2654435761 ^ z * 805459861 of Instant NGP, Mueller et al.
@@ -35,13 +35,13 @@ LEVELS, FEATURES, TABLE, WIDTH, LAYERS, LOBES = 16, 2, 1 << 19, 64, 4, 8
 OUTPUTS = 5 * LOBES
 DEPTHS, SHARE = 4, 0.5
 LOG_SHARP_MIN, LOG_SHARP_MAX = -3.0, 6.2
-MIN_RES, MAX_RES = 16, 2048
+MIN_RES, MAX_RES = 16, 128  # the device's default; 2048 and others by argument
 IN_GRID, IN_SH, IN_NORMAL, IN_WAVE, IN_ONE = 0, 32, 48, 51, 59
 
 
-def resolutions():
+def resolutions(max_res=MAX_RES):
     """(res per level, dense mask)."""
-    growth = math.exp((math.log(MAX_RES) - math.log(MIN_RES)) / (LEVELS - 1))
+    growth = math.exp((math.log(max_res) - math.log(MIN_RES)) / (LEVELS - 1))
     res, dense = [], 0
     for level in range(LEVELS):
         r = int(math.floor(MIN_RES * growth ** level + 0.5))
@@ -65,12 +65,12 @@ class Guide:
     """Parameters of a guide: grid [LEVELS, TABLE, FEATURES] and
     mlp [LAYERS, WIDTH, WIDTH] (input, output), the region."""
 
-    def __init__(self, grid, mlp, lower, size):
+    def __init__(self, grid, mlp, lower, size, max_res=MAX_RES):
         self.grid = np.asarray(grid).reshape(LEVELS, TABLE, FEATURES)
         self.mlp = np.asarray(mlp).reshape(LAYERS, WIDTH, WIDTH)
         self.lower = np.asarray(lower, np.float32)
         self.inv_size = (np.float32(1) / np.asarray(size, np.float32)).astype(np.float32)
-        self.res, self.dense = resolutions()
+        self.res, self.dense = resolutions(max_res)
 
 
 def _corners(g, level, p, dt):

@@ -52,12 +52,13 @@ def _queries(rng, n):
     return q
 
 
-@pytest.fixture(scope="module")
-def guide(device):
+@pytest.fixture(scope="module", params=[128, 2048], ids=["finest128", "finest2048"])
+def guide(device, request):
     device.guide_reset(3, LOWER, SIZE)
+    device.guide_configure(max_resolution=request.param)
     grid, mlp = _params(17)
     device.guide_set_params(grid, mlp)
-    return device, R.Guide(grid, mlp, LOWER, SIZE)
+    return device, R.Guide(grid, mlp, LOWER, SIZE, max_res=request.param)
 
 
 def test_reset_is_deterministic_and_described(device):

@@ -14,11 +14,11 @@ import pytest
 from oracle import guide_ref as R
 
 
-def _guide(seed, grid_scale=0.05):
+def _guide(seed, grid_scale=0.05, max_res=2048):
     rng = np.random.default_rng(seed)
     grid = (rng.standard_normal((R.LEVELS, R.TABLE, R.FEATURES)) * grid_scale).astype(np.float64)
     mlp = rng.uniform(-0.3, 0.3, (R.LAYERS, R.WIDTH, R.WIDTH))
-    return R.Guide(grid, mlp, lower=[-2, -2, -1], size=[4, 4, 4])
+    return R.Guide(grid, mlp, lower=[-2, -2, -1], size=[4, 4, 4], max_res=max_res)
 
 
 def _unit(rng, n):
@@ -46,8 +46,9 @@ def _fib(n):
 
 
 def test_resolutions_and_dense_levels():
-    res, dense = R.resolutions()
+    res, dense = R.resolutions(2048)
     assert res[0] == 16 and res[-1] == 2048 and res == sorted(res)
+    assert R.resolutions()[0][-1] == 128
     for level in range(R.LEVELS):
         side = res[level] + 1
         if dense >> level & 1:
@@ -56,6 +57,7 @@ def test_resolutions_and_dense_levels():
             assert side ** 3 > R.TABLE
     # a dense level maps its corners one to one
     g = _guide(1)
+    g.res, g.dense = res, dense
     level = 0
     n = res[level]
     pts = np.stack(np.meshgrid(*[np.arange(n + 1)] * 3, indexing="ij"), -1).reshape(-1, 3)
