@@ -353,17 +353,18 @@ int kdn_read_image(kdn_ctx* ctx, float* rgb);
 /*
  * A network that learns, per scene, where light arrives from: a hash grid
  * of the position (16 levels of 2^19 entries, 2 features each, 16 to 128
- * cells a side by default), real
- * spherical harmonics of the outgoing direction, the normal and a code of
- * the hero wavelength feed three ReLU layers of 64 (BF16 weights, MFMA
- * with FP32 accumulation) whose outputs decode to a mixture of 8 von
- * Mises-Fisher lobes. A guided pass (kdn_render_pass flag bit 4) samples
- * half of the directions at the first four vertices of a path from the
- * mixture and half from the BSDF (one-sample MIS, so the estimate stays
- * unbiased) and trains the network online, with Adam, on the radiance its
- * own paths bring back. The reference has no learned guide; it joins the
- * direction sampling the reference's Sampler dispatches
- * (plugins/transport/paths/sampler.h:23,357). The calls below expose the
+ * cells a side by default), real spherical harmonics of the outgoing
+ * direction, the normal and a code of the hero wavelength feed three ReLU
+ * layers of 64 (BF16 weights, MFMA with FP32 accumulation) whose outputs
+ * decode to a mixture of 8 von Mises-Fisher lobes and a selection
+ * probability. A guided pass (kdn_render_pass flag bit 4) samples a
+ * direction at each of the first four vertices of a path from the mixture
+ * with that probability and from the BSDF otherwise (one-sample MIS, so
+ * the estimate stays unbiased), and trains the network online, with Adam,
+ * on the radiance its own paths bring back. The reference has no learned
+ * guide; it joins the direction sampling the reference's Sampler
+ * dispatches (plugins/transport/paths/sampler.h:23,357). The calls below
+ * expose the
  * network for tests and tools.
  */
 typedef struct kdn_guide_query {
@@ -380,7 +381,10 @@ typedef struct kdn_guide_sample {
     float nm;
     float in[3];       /* the direction light came back along */
     float weight;      /* radiance over the density it was sampled with */
-    float pad[2];
+    float bsdf_pdf;    /* the BSDF's density of in: the loss is then on the
+                          one-sample MIS density a q + (1 - a) bsdf_pdf, which
+                          trains the selection a too; negative: on q alone */
+    float pad;
 } kdn_guide_sample;
 
 typedef struct kdn_guide_lobe {
@@ -407,7 +411,9 @@ typedef struct kdn_guide_info {
 /* How a guided pass uses and trains the guide. */
 typedef struct kdn_guide_config {
     float learning_rate;     /* Adam (default 1e-2) */
-    float share;             /* the guide's share of sampled directions (0.5) */
+    float share;             /* the most of the sampled directions the guide may
+                                take: its selection probability is share times
+                                the sigmoid of a learned output (0.9) */
     float epochs;            /* passes over a round's samples (2) */
     uint32_t min_steps;      /* Adam steps a round, at least (16) */
     uint32_t max_steps;      /* and at most (64, the limit) */
@@ -426,8 +432,9 @@ int kdn_guide_describe(kdn_ctx* ctx, kdn_guide_info* info);
  * mlp[mlp_params] (layer, input, output); NULL skips one. */
 int kdn_guide_params(kdn_ctx* ctx, float* grid, float* mlp);
 int kdn_guide_set_params(kdn_ctx* ctx, const float* grid, const float* mlp);
-/* The mixture at each query: lobes[8 * n]. */
-int kdn_guide_infer(kdn_ctx* ctx, const kdn_guide_query* queries, uint32_t n, kdn_guide_lobe* lobes);
+/* The mixture at each query: lobes[8 * n], and the probability of
+ * sampling it rather than the BSDF, select[n] (may be NULL). */
+int kdn_guide_infer(kdn_ctx* ctx, const kdn_guide_query* queries, uint32_t n, kdn_guide_lobe* lobes, float* select);
 /* Loss sum over samples of -weight * log q(in) and its gradients (no
  * normalisation, no loss scale); grid_grad may be NULL. */
 int kdn_guide_gradients(kdn_ctx* ctx, const kdn_guide_sample* samples, uint32_t n, float* grid_grad, float* mlp_grad, double* loss);

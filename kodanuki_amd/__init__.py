@@ -131,7 +131,7 @@ class GuideQuery(C.Structure):
 
 class GuideSample(C.Structure):
     _fields_ = [("point", C.c_float * 3), ("out", C.c_float * 3), ("normal", C.c_float * 3), ("nm", C.c_float),
-                ("in_", C.c_float * 3), ("weight", C.c_float), ("pad", C.c_float * 2)]
+                ("in_", C.c_float * 3), ("weight", C.c_float), ("bsdf_pdf", C.c_float), ("pad", C.c_float)]
 
 
 class GuideLobe(C.Structure):
@@ -232,7 +232,7 @@ def _bind(lib):
         "kdn_guide_configure": (C.c_int, [P, C.POINTER(GuideConfig)]),
         "kdn_guide_params": (C.c_int, [P, P, P]),
         "kdn_guide_set_params": (C.c_int, [P, P, P]),
-        "kdn_guide_infer": (C.c_int, [P, P, C.c_uint32, P]),
+        "kdn_guide_infer": (C.c_int, [P, P, C.c_uint32, P, P]),
         "kdn_guide_gradients": (C.c_int, [P, P, C.c_uint32, P, P, C.POINTER(C.c_double)]),
         "kdn_guide_train": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_uint32, P]),
         "kdn_serve_open": (C.c_int, [P, C.c_uint64, C.c_uint32, C.POINTER(P)]),
@@ -437,7 +437,7 @@ class Device:
         sz = (C.c_float * 3)(*[float(x) for x in size]) if size is not None else None
         _check(library().kdn_guide_reset(self._h, seed, lo, sz), self._h)
 
-    def guide_configure(self, learning_rate=1e-2, share=0.5, epochs=2.0, min_steps=16, max_steps=64, max_resolution=128,
+    def guide_configure(self, learning_rate=1e-2, share=0.9, epochs=2.0, min_steps=16, max_steps=64, max_resolution=128,
                         train_paths=1 << 20):
         cfg = GuideConfig(learning_rate, share, epochs, min_steps, max_steps, max_resolution, train_paths, 0)
         _check(library().kdn_guide_configure(self._h, C.byref(cfg)), self._h)
@@ -462,12 +462,13 @@ class Device:
                                               _ptr(m) if m is not None else None), self._h)
 
     def guide_infer(self, queries):
-        """queries [n, 10] (point, out, normal, nm) -> lobes [n, 8, 5]
-        (axis, sharpness, share)."""
+        """queries [n, 10] (point, out, normal, nm) -> (lobes [n, 8, 5]
+        (axis, sharpness, share), selection probabilities [n])."""
         q = np.ascontiguousarray(queries, np.float32)
         out = np.zeros((len(q), 8, 5), np.float32)
-        _check(library().kdn_guide_infer(self._h, _ptr(q), len(q), _ptr(out)), self._h)
-        return out
+        sel = np.zeros(len(q), np.float32)
+        _check(library().kdn_guide_infer(self._h, _ptr(q), len(q), _ptr(out), _ptr(sel)), self._h)
+        return out, sel
 
     def guide_gradients(self, samples, grid=True):
         """samples [n, 16] -> (loss, grid grad [16, 2^19, 2] or None, mlp grad [4, 64, 64])."""

@@ -27,10 +27,11 @@ constexpr uint32_t kGuideTable = 1u << 19;  // entries per level
 constexpr int kGuideWidth = 64;           // inputs, hidden width and padded outputs
 constexpr int kGuideLayers = 4;           // weight matrices: 3 hidden ReLU layers, then the outputs
 constexpr int kGuideLobes = 8;
-constexpr int kGuideOutputs = 5 * kGuideLobes;  // logit, axis (3), log sharpness per lobe
+constexpr int kGuideOutputs = 5 * kGuideLobes + 1;  // logit, axis (3), log sharpness per lobe; the selection logit
+constexpr int kGuideSelect = 5 * kGuideLobes;       // output of the selection logit
 constexpr int kGuideDepths = 4;           // guided vertices: path depths 0 .. 3
 constexpr uint32_t kGuideTrainPaths = 1u << 20;  // paths of a round that keep training records, at most (default)
-constexpr float kGuideShare = 0.5f;       // the guide's share of directions (one-sample MIS)
+constexpr float kGuideShare = 0.9f;       // the most of the directions the guide may take (one-sample MIS)
 constexpr float kGuideLogSharpMin = -3.0f;  // sharpness within [e^-3, e^6.2]
 constexpr float kGuideLogSharpMax = 6.2f;
 constexpr int kGuideMinRes = 16, kGuideMaxRes = 128;  // grid resolutions (the finest by default)
@@ -43,13 +44,15 @@ constexpr int kInGrid = 0, kInSh = 32, kInNormal = 48, kInWave = 51, kInOne = 59
 // A training record of one guided vertex (floats): point, outgoing
 // direction, normal, hero wavelength, the direction taken, its density,
 // the throughput after the vertex (4 lanes), the path's radiance when the
-// next vertex begins (4 lanes), and a state word (bit 0 recorded, bit 1
-// snapshot taken).
+// next vertex begins (4 lanes), a state word (1 recorded, 3 snapshot
+// taken) and the BSDF's density of the direction.
 constexpr int kRecPoint = 0, kRecOut = 3, kRecNormal = 6, kRecNm = 9, kRecIn = 10, kRecPdf = 13, kRecThr = 14,
-              kRecSnap = 18, kRecState = 22, kRecFloats = 24;
+              kRecSnap = 18, kRecState = 22, kRecBsdf = 23, kRecFloats = 24;
 // A training sample (floats): point, outgoing direction, normal, hero
-// wavelength, the direction taken, its weight (radiance over density).
-constexpr int kSamPoint = 0, kSamOut = 3, kSamNormal = 6, kSamNm = 9, kSamIn = 10, kSamWeight = 13, kSamFloats = 16;
+// wavelength, the direction taken, its weight (radiance over density),
+// the BSDF's density of the direction (negative: train the lobes alone).
+constexpr int kSamPoint = 0, kSamOut = 3, kSamNormal = 6, kSamNm = 9, kSamIn = 10, kSamWeight = 13, kSamBsdf = 14,
+              kSamFloats = 16;
 
 // The guide as the kernels read it.
 struct GuideView {
@@ -58,6 +61,7 @@ struct GuideView {
     uint32_t res[kGuideLevels];
     uint32_t dense;          // bit l: level l is indexed densely
     float lower[3], inv_size[3];
+    float share_cap;         // the selection probability is share_cap * sigmoid(logit)
 };
 
 // ---- BF16 (round to nearest even; NaN kept quiet) ---------------------------
@@ -162,6 +166,7 @@ struct GuideLobes {
     float axis[kGuideLobes][3];
     float sharp[kGuideLobes];
     float share[kGuideLobes];
+    float select;  // the probability of sampling the mixture rather than the BSDF
 };
 
 // The vMF normalisation sharpness / (2 pi (1 - e^(-2 sharpness))).
@@ -169,7 +174,9 @@ KDN_D float vmf_norm(float k) { return k / (2 * kPi * (1 - expf(-2 * k))); }
 
 KDN_D float clamp_log_sharp(float c) { return c < kGuideLogSharpMin ? kGuideLogSharpMin : (c > kGuideLogSharpMax ? kGuideLogSharpMax : c); }
 
-KDN_D void decode_lobes(const float* o, GuideLobes& L) {
+KDN_D float sigmoid(float x) { return 1 / (1 + expf(-x)); }
+
+KDN_D void decode_lobes(const float* o, float cap, GuideLobes& L) {
     float top = o[0];
     for (int k = 1; k < kGuideLobes; ++k) top = fmax_(top, o[5 * k]);
     float sum = 0;
@@ -190,6 +197,7 @@ KDN_D void decode_lobes(const float* o, GuideLobes& L) {
         L.sharp[k] = expf(clamp_log_sharp(o[5 * k + 4]));
     }
     for (int k = 0; k < kGuideLobes; ++k) L.share[k] /= sum;
+    L.select = cap * sigmoid(o[kGuideSelect]);
 }
 
 KDN_D float mixture_pdf(const GuideLobes& L, V3 w) {
@@ -214,14 +222,17 @@ KDN_D V3 mixture_sample(const GuideLobes& L, float uLobe, float u, float v) {
     return to_world(B, v3(r * f_cos(phi), r * f_sin(phi), w));
 }
 
-// The training loss of one sample, -weight * log q(in), and its gradient
-// with respect to the network's 40 outputs (zero where the sample carries
-// nothing or its density underflows).
-KDN_D float lobe_loss_gradient(const float* o, const float in[3], float weight, float* g) {
+// The training loss of one sample and its gradient with respect to the
+// network's outputs (zero where the sample carries nothing or its density
+// underflows). With the BSDF's density pB of the direction (bsdf >= 0)
+// the loss is -weight * log p of the one-sample MIS density
+// p = a q + (1 - a) pB, which trains the lobes and the selection a
+// together (Neural Parametric Mixtures); without it, -weight * log q.
+KDN_D float lobe_loss_gradient(const float* o, const float in[3], float weight, float bsdf, float cap, float* g) {
     for (int c = 0; c < kGuideOutputs; ++c) g[c] = 0;
     if (!(weight > 0)) return 0.0f;
     GuideLobes L;
-    decode_lobes(o, L);
+    decode_lobes(o, cap, L);
     float v[kGuideLobes], dots[kGuideLobes];
     float q = 0;
     for (int k = 0; k < kGuideLobes; ++k) {
@@ -230,6 +241,18 @@ KDN_D float lobe_loss_gradient(const float* o, const float in[3], float weight, 
         q += L.share[k] * v[k];
     }
     if (!(q > 1e-30f)) return 0.0f;
+    float loss;
+    if (bsdf >= 0) {
+        const float a = L.select;
+        const float p = a * q + (1 - a) * bsdf;
+        if (!(p > 1e-30f)) return 0.0f;
+        const float s = sigmoid(o[kGuideSelect]);
+        g[kGuideSelect] = -weight * (q - bsdf) / p * cap * s * (1 - s);
+        loss = -weight * logf(p);
+        weight *= a * q / p;  // the lobes' share of the gradient
+    } else {
+        loss = -weight * logf(q);
+    }
     for (int k = 0; k < kGuideLobes; ++k) {
         const float gamma = L.share[k] * v[k] / q;
         g[5 * k] = -weight * (gamma - L.share[k]);
@@ -246,7 +269,7 @@ KDN_D float lobe_loss_gradient(const float* o, const float in[3], float weight, 
             g[5 * k + 4] = -weight * gamma * dlogv * s;
         }
     }
-    return -weight * logf(q);
+    return loss;
 }
 
 }  // namespace kdn

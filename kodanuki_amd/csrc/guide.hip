@@ -153,7 +153,8 @@ __global__ void k_guide_collect(const float* recs, uint32_t nRec, uint32_t strid
             float* s = samples + (size_t)r * kSamFloats;
             for (int k = 0; k < 13; ++k) s[k] = rec[k];
             s[kSamWeight] = w;
-            s[14] = s[15] = 0;
+            s[kSamBsdf] = rec[kRecBsdf];
+            s[15] = 0;
             ok = 1;
         }
     }
@@ -295,6 +296,7 @@ bool guideReset(Ctx& c, uint32_t seed, const float lower[3], const float size[3]
         G.size[k] = size[k];
     }
     viewOf(G, G.config.max_resolution);
+    G.view.share_cap = G.config.share;
     KDN_HIP(hipMemsetAsync(G.gridM(), 0, 3 * kGuideGridParams * sizeof(float), st));
     KDN_HIP(hipMemsetAsync(G.mlpM(), 0, 3 * (size_t)kGuideMlpParams * sizeof(float), st));
     hipLaunchKernelGGL(k_guide_init, dim3((uint32_t)((kGuideGridParams + 255) / 256)), dim3(256), 0, st, G.grid, G.mlp, G.mlp_bf, seed);
@@ -336,7 +338,6 @@ bool guideBeginRound(Ctx& c, PassParams& P, uint32_t paths) {
     P.lobes = G.steps > 0 ? (const float*)G.lobes.ptr : nullptr;
     P.recs = (float*)G.recs.ptr;
     P.train_stride = stride;
-    P.guide_share = G.steps > 0 ? G.config.share : 0.0f;
     return true;
 }
 
@@ -467,6 +468,7 @@ int kdn_guide_configure(kdn_ctx* ctx, const kdn_guide_config* config) {
     }
     G->config = k;
     viewOf(*G, k.max_resolution);
+    G->view.share_cap = k.share;
     return KDN_OK;
 }
 
@@ -513,7 +515,7 @@ int kdn_guide_set_params(kdn_ctx* ctx, const float* grid, const float* mlp) {
     return KDN_OK;
 }
 
-int kdn_guide_infer(kdn_ctx* ctx, const kdn_guide_query* queries, uint32_t n, kdn_guide_lobe* lobes) {
+int kdn_guide_infer(kdn_ctx* ctx, const kdn_guide_query* queries, uint32_t n, kdn_guide_lobe* lobes, float* select) {
     if (!ctx || (n && (!queries || !lobes))) return KDN_E_ARGUMENT;
     Ctx& c = ctx->c;
     GuideState* G = stateOf(c);
@@ -521,13 +523,16 @@ int kdn_guide_infer(kdn_ctx* ctx, const kdn_guide_query* queries, uint32_t n, kd
     if (n > (1u << 24)) return guideFail(ctx, "kdn_guide_infer: at most 2^24 queries a call", KDN_E_ARGUMENT);
     if (n == 0) return KDN_OK;
     hipSetDevice(c.device);
-    if (!ensure(c.scratch[6], (size_t)n * (sizeof(kdn_guide_query) + kGuideOutputs * sizeof(float)), c.error, false)) return KDN_E_MEMORY;
+    const size_t lobeBytes = (size_t)n * kGuideLobes * sizeof(kdn_guide_lobe);
+    if (!ensure(c.scratch[6], (size_t)n * (sizeof(kdn_guide_query) + sizeof(float)) + lobeBytes, c.error, false)) return KDN_E_MEMORY;
     char* base = (char*)c.scratch[6].ptr;
     float* q = (float*)base;
     float* out = (float*)(base + (size_t)n * sizeof(kdn_guide_query));
+    float* sel = (float*)((char*)out + lobeBytes);
     if (hipMemcpyAsync(q, queries, (size_t)n * sizeof(kdn_guide_query), hipMemcpyHostToDevice, c.stream) != hipSuccess ||
-        !launchGuideInferQueries(G->view, q, n, out, c.stream) ||
-        hipMemcpyAsync(lobes, out, (size_t)n * kGuideOutputs * sizeof(float), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+        !launchGuideInferQueries(G->view, q, n, out, sel, c.stream) ||
+        hipMemcpyAsync(lobes, out, lobeBytes, hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+        (select && hipMemcpyAsync(select, sel, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c.stream) != hipSuccess) ||
         hipStreamSynchronize(c.stream) != hipSuccess) {
         return guideFail(ctx, "kdn_guide_infer: device call failed", KDN_E_DEVICE);
     }

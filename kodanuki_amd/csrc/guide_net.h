@@ -7,8 +7,9 @@
 
 namespace kdn {
 
-// lobes of queries (kdn_guide_query each) into kGuideOutputs floats each
-bool launchGuideInferQueries(const GuideView& g, const float* queries, uint32_t n, float* lobes, hipStream_t st);
+// lobes of queries (kdn_guide_query each): 5 floats a lobe, and the
+// selection probabilities (select may be null)
+bool launchGuideInferQueries(const GuideView& g, const float* queries, uint32_t n, float* lobes, float* select, hipStream_t st);
 // lobes of the guided vertices among the first n paths of a queue, into
 // the per-path lobe buffer (lane-major)
 bool launchGuideInferPaths(const GuideView& g, const PassParams& P, uint32_t queue, uint32_t n, float* lobes, hipStream_t st);

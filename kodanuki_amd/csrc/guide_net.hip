@@ -94,7 +94,8 @@ __device__ __forceinline__ void store_f32(const f32x4 acc[16], float* O) {
 // the queries of a host-given array
 struct ArrayQueries {
     const float* q;  // kdn_guide_query: point, out, normal, nm
-    float* lobes;    // per query kGuideOutputs floats: axis, sharpness, share per lobe
+    float* lobes;    // per query kGuideLobes x (axis, sharpness, share)
+    float* select;   // per query the selection probability, or null
     uint32_t n;
     __device__ bool fetch(const GuideView& g, uint32_t item, float in[kGuideWidth]) const {
         if (item >= n) return false;
@@ -103,7 +104,7 @@ struct ArrayQueries {
         return true;
     }
     __device__ void write(uint32_t item, const GuideLobes& L) const {
-        float* o = lobes + (size_t)item * kGuideOutputs;
+        float* o = lobes + (size_t)item * 5 * kGuideLobes;
         for (int k = 0; k < kGuideLobes; ++k) {
             o[5 * k] = L.axis[k][0];
             o[5 * k + 1] = L.axis[k][1];
@@ -111,6 +112,7 @@ struct ArrayQueries {
             o[5 * k + 3] = L.sharp[k];
             o[5 * k + 4] = L.share[k];
         }
+        if (select) select[item] = L.select;
     }
 };
 
@@ -167,7 +169,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideView g, Q
             float o[kGuideOutputs];
             for (int c = 0; c < kGuideOutputs; ++c) o[c] = O[lane * kOutLd + c];
             GuideLobes L;
-            decode_lobes(o, L);
+            decode_lobes(o, g.share_cap, L);
             src.write(item, L);
         }
         __syncthreads();
@@ -212,7 +214,7 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideView g, const float* sa
         float o[kGuideOutputs], grad[kGuideOutputs];
         for (int c = 0; c < kGuideOutputs; ++c) o[c] = O[lane * kOutLd + c];
         const float w = live ? smp[kSamWeight] * *scale : 0.0f;
-        const float l = lobe_loss_gradient(o, smp + kSamIn, w, grad);
+        const float l = lobe_loss_gradient(o, smp + kSamIn, w, live ? smp[kSamBsdf] : -1.0f, g.share_cap, grad);
         for (int c = 0; c < kGuideWidth; ++c) G[0][lane * kLd + c] = c < kGuideOutputs ? to_bf16(grad[c]) : (uint16_t)0;
         if (l != 0) atomicAdd(loss, l);
     }
@@ -253,11 +255,11 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideView g, const float* sa
 
 }  // namespace
 
-bool launchGuideInferQueries(const GuideView& g, const float* queries, uint32_t n, float* lobes, hipStream_t st) {
+bool launchGuideInferQueries(const GuideView& g, const float* queries, uint32_t n, float* lobes, float* select, hipStream_t st) {
     if (n == 0) return true;
     const uint32_t tiles = (n + 63) / 64;
     const uint32_t blocks = std::min<uint32_t>((tiles + kInferWaves - 1) / kInferWaves, 4096);
-    hipLaunchKernelGGL(k_guide_infer<ArrayQueries>, dim3(blocks), dim3(64 * kInferWaves), 0, st, g, ArrayQueries{queries, lobes, n});
+    hipLaunchKernelGGL(k_guide_infer<ArrayQueries>, dim3(blocks), dim3(64 * kInferWaves), 0, st, g, ArrayQueries{queries, lobes, select, n});
     return hipGetLastError() == hipSuccess;
 }
 

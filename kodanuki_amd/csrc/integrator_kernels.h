@@ -67,14 +67,13 @@ struct PassParams {
     uint32_t paths;        // paths of the round (the last slice's end)
     const ColourTables* colour;
     PathState s;
-    // the learned guide: lobes of the guided vertices (lane-major,
-    // kGuideOutputs per path; null: BSDF sampling only), training records
-    // (kGuideDepths for each path of train_stride; null: none kept) and
-    // the guide's share of sampled directions
+    // the learned guide: lobes and selection probability of the guided
+    // vertices (lane-major, kGuideOutputs per path; null: BSDF sampling
+    // only) and training records (kGuideDepths for each path of
+    // train_stride; null: none kept)
     const float* lobes;
     float* recs;
     uint32_t train_stride;
-    float guide_share;
 };
 
 KDN_D const JobDev& job_of(const PassParams& P, uint32_t i) { return P.jobs[P.slices[P.s.slice[i]].job]; }
@@ -210,6 +209,7 @@ KDN_D void store_lobes(float* lobes, size_t N, uint32_t i, const GuideLobes& L) 
         lobes[(5 * k + 3) * N + i] = L.sharp[k];
         lobes[(5 * k + 4) * N + i] = L.share[k];
     }
+    lobes[(size_t)kGuideSelect * N + i] = L.select;
 }
 
 KDN_D void load_lobes(const float* lobes, size_t N, uint32_t i, GuideLobes& L) {
@@ -220,6 +220,7 @@ KDN_D void load_lobes(const float* lobes, size_t N, uint32_t i, GuideLobes& L) {
         L.sharp[k] = lobes[(5 * k + 3) * N + i];
         L.share[k] = lobes[(5 * k + 4) * N + i];
     }
+    L.select = lobes[(size_t)kGuideSelect * N + i];
 }
 
 // numbers of the guided sampling, apart from the reference's dimensions
@@ -340,7 +341,7 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
         const bool guided = P.lobes != nullptr && guidable;
         GuideLobes lobes;
         if (guided) load_lobes(P.lobes, N, i, lobes);
-        const float share = P.guide_share;
+        const float share = guided ? lobes.select : 0.0f;
         // aim at one lamp (aimedFrom, trace.cpp:896-947), MIS against meeting
         if (!exact && J.n_lamps > 0) {
             const float uPick = number(stream, sample, depth, Pick);
@@ -437,6 +438,7 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
                             const float q[14] = {x.x, x.y, x.z, out.x, out.y, out.z, n.x, n.y, n.z, w.nm[0], sc.in.x, sc.in.y, sc.in.z, sc.density};
                             for (int k = 0; k < 14; ++k) rec[k] = q[k];
                             for (int l = 0; l < kLanes; ++l) rec[kRecThr + l] = carried[l];
+                            rec[kRecBsdf] = rough_density(m, n, out, sc.in);
                             rec[kRecState] = 1.0f;
                         }
                     }

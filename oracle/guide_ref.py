@@ -17,10 +17,14 @@ plugins/transport/paths/sampler.h:23,357):
 - four 64x64 layers, ReLU between them, no biases;
 - the outputs as 8 von Mises-Fisher lobes: softmax shares, normalised
   axes, sharpness exp(clamp(c, -3, 6.2)); density
-  sum share * k / (2 pi (1 - e^-2k)) e^(k (axis . w - 1));
-- the loss -weight * log q(in) and its gradient through the layers and the
-  grid (Neural Parametric Mixtures, Dong et al. 2023; the KL gradient
-  estimate of Neural Importance Sampling, Mueller et al. 2019).
+  sum share * k / (2 pi (1 - e^-2k)) e^(k (axis . w - 1)); and the
+  selection probability a = cap * sigmoid(output 40) of sampling the
+  mixture rather than the BSDF;
+- the loss -weight * log p of the one-sample MIS density
+  p = a q + (1 - a) pB (pB the BSDF's density of the direction), or
+  -weight * log q for samples without one, and its gradient through the
+  layers and the grid (Neural Parametric Mixtures, Dong et al. 2023; the
+  KL gradient estimate of Neural Importance Sampling, Mueller et al. 2019).
 
 Two modes: `bf16=True` rounds where the device rounds (the network's
 inputs, each ReLU activation, the gradient into each layer; FP32 sums), for
@@ -32,8 +36,9 @@ import math
 import numpy as np
 
 LEVELS, FEATURES, TABLE, WIDTH, LAYERS, LOBES = 16, 2, 1 << 19, 64, 4, 8
-OUTPUTS = 5 * LOBES
-DEPTHS, SHARE = 4, 0.5
+SELECT = 5 * LOBES  # the output of the selection logit
+OUTPUTS = 5 * LOBES + 1
+DEPTHS, SHARE_CAP = 4, 0.9
 LOG_SHARP_MIN, LOG_SHARP_MAX = -3.0, 6.2
 MIN_RES, MAX_RES = 16, 128  # the device's default; 2048 and others by argument
 IN_GRID, IN_SH, IN_NORMAL, IN_WAVE, IN_ONE = 0, 32, 48, 51, 59
@@ -65,12 +70,13 @@ class Guide:
     """Parameters of a guide: grid [LEVELS, TABLE, FEATURES] and
     mlp [LAYERS, WIDTH, WIDTH] (input, output), the region."""
 
-    def __init__(self, grid, mlp, lower, size, max_res=MAX_RES):
+    def __init__(self, grid, mlp, lower, size, max_res=MAX_RES, cap=SHARE_CAP):
         self.grid = np.asarray(grid).reshape(LEVELS, TABLE, FEATURES)
         self.mlp = np.asarray(mlp).reshape(LAYERS, WIDTH, WIDTH)
         self.lower = np.asarray(lower, np.float32)
         self.inv_size = (np.float32(1) / np.asarray(size, np.float32)).astype(np.float32)
         self.res, self.dense = resolutions(max_res)
+        self.cap = cap
 
 
 def _corners(g, level, p, dt):
@@ -164,16 +170,21 @@ def forward(g, X, bf=True):
     return H, O
 
 
+def select(O, cap=SHARE_CAP):
+    """The selection probability [B]."""
+    return cap / (1 + np.exp(-np.asarray(O)[:, SELECT]))
+
+
 def decode(O):
     """shares [B, 8], axes [B, 8, 3], sharpness [B, 8]."""
     O = np.asarray(O)
-    logits = O[:, 0:OUTPUTS:5]
+    logits = O[:, 0:5 * LOBES:5]
     e = np.exp(logits - logits.max(axis=1, keepdims=True))
     share = e / e.sum(axis=1, keepdims=True)
     raw = np.stack([O[:, 5 * k + 1:5 * k + 4] for k in range(LOBES)], axis=1)
     length = np.linalg.norm(raw, axis=2, keepdims=True)
     axis = np.where(length > 1e-6, raw / np.maximum(length, 1e-30), np.array([0, 0, 1.0]))
-    sharp = np.exp(np.clip(O[:, 4:OUTPUTS:5], LOG_SHARP_MIN, LOG_SHARP_MAX))
+    sharp = np.exp(np.clip(O[:, 4:5 * LOBES:5], LOG_SHARP_MIN, LOG_SHARP_MAX))
     return share, axis, sharp
 
 
@@ -215,32 +226,43 @@ def _basis(n):
     return t, np.cross(n, t)
 
 
-def loss_gradient(O, w_in, weight):
-    """Per-sample loss -weight log q(in) [B] and its gradient dL/dO [B, 64]
-    (zero for samples whose density underflows or that carry nothing)."""
+def loss_gradient(O, w_in, weight, bsdf=None, cap=SHARE_CAP):
+    """Per-sample loss [B] and its gradient dL/dO [B, 64]: -weight log p of
+    p = a q + (1 - a) bsdf where bsdf >= 0, -weight log q where it is
+    negative or not given (zero for samples whose density underflows or
+    that carry nothing)."""
     O = np.asarray(O)
     dt = O.dtype
+    n = len(O)
+    bsdf = np.full(n, -1.0) if bsdf is None else np.asarray(bsdf, np.float64)
     share, axis, sharp = decode(O)
     dots = np.einsum("bkj,bj->bk", axis, w_in)
     v = vmf_norm(sharp) * np.exp(sharp * (dots - 1))
     q = np.sum(share * v, axis=1)
-    ok = (q > 1e-30) & (weight > 0)
+    s = 1 / (1 + np.exp(-O[:, SELECT]))
+    a = cap * s
+    mixed = bsdf >= 0
+    p = np.where(mixed, a * q + (1 - a) * np.maximum(bsdf, 0), q)
+    ok = (q > 1e-30) & (weight > 0) & (p > 1e-30)
     qs = np.where(ok, q, 1)
+    ps = np.where(ok, p, 1)
     gamma = share * v / qs[:, None]
-    wgt = np.where(ok, weight, 0).astype(dt)
-    G = np.zeros((len(O), WIDTH), dt)
-    G[:, 0:OUTPUTS:5] = -wgt[:, None] * (gamma - share)
+    w0 = np.where(ok, weight, 0).astype(dt)
+    loss = np.where(ok, -w0 * np.log(ps), 0)
+    G = np.zeros((n, WIDTH), dt)
+    G[:, SELECT] = np.where(mixed & ok, -w0 * (q - np.maximum(bsdf, 0)) / ps * cap * s * (1 - s), 0)
+    wgt = np.where(mixed, w0 * a * q / ps, w0).astype(dt)
+    G[:, 0:5 * LOBES:5] = -wgt[:, None] * (gamma - share)
     raw = np.stack([O[:, 5 * k + 1:5 * k + 4] for k in range(LOBES)], axis=1)
     length = np.linalg.norm(raw, axis=2)
     s = np.where(length > 1e-6, -wgt[:, None] * gamma * sharp / np.maximum(length, 1e-30), 0)
     dax = s[:, :, None] * (w_in[:, None, :] - dots[:, :, None] * axis)
     for k in range(LOBES):
         G[:, 5 * k + 1:5 * k + 4] = dax[:, k]
-    c = O[:, 4:OUTPUTS:5]
+    c = O[:, 4:5 * LOBES:5]
     inside = (c > LOG_SHARP_MIN) & (c < LOG_SHARP_MAX)
     dlogv = 1 / sharp - 2 / (np.exp(2 * sharp) - 1) + (dots - 1)
-    G[:, 4:OUTPUTS:5] = np.where(inside, -wgt[:, None] * gamma * dlogv * sharp, 0)
-    loss = np.where(ok, -wgt * np.log(qs), 0)
+    G[:, 4:5 * LOBES:5] = np.where(inside, -wgt[:, None] * gamma * dlogv * sharp, 0)
     return loss, G
 
 
@@ -251,7 +273,7 @@ def gradients(g, samples, bf=True):
     S = np.asarray(samples, np.float64 if not bf else np.float32)
     X, corners = inputs(g, S[:, 0:3], S[:, 3:6], S[:, 6:9], S[:, 9], dt)
     H, O = forward(g, X, bf)
-    loss, G = loss_gradient(O, S[:, 10:13].astype(dt), S[:, 13].astype(dt))
+    loss, G = loss_gradient(O, S[:, 10:13].astype(dt), S[:, 13].astype(dt), S[:, 14], g.cap)
     W = g.mlp.astype(dt)
     if bf:
         W = bf16(W)
@@ -281,5 +303,5 @@ def loss_of(g, samples):
     S = np.asarray(samples, np.float64)
     X, _ = inputs(g, S[:, 0:3], S[:, 3:6], S[:, 6:9], S[:, 9], np.float64)
     _, O = forward(g, X, bf=False)
-    loss, _ = loss_gradient(O, S[:, 10:13], S[:, 13])
+    loss, _ = loss_gradient(O, S[:, 10:13], S[:, 13], S[:, 14], g.cap)
     return float(np.sum(loss))

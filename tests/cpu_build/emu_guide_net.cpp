@@ -31,11 +31,11 @@ void forward(const GuideView& g, const float in[kGuideWidth], uint16_t H[kGuideL
     }
 }
 
-void decode(const float O[kGuideWidth], GuideLobes& L) { decode_lobes(O, L); }
+void decode(const GuideView& g, const float O[kGuideWidth], GuideLobes& L) { decode_lobes(O, g.share_cap, L); }
 
 }  // namespace
 
-bool launchGuideInferQueries(const GuideView& g, const float* queries, uint32_t n, float* lobes, hipStream_t) {
+bool launchGuideInferQueries(const GuideView& g, const float* queries, uint32_t n, float* lobes, float* select, hipStream_t) {
     for (uint32_t item = 0; item < n; ++item) {
         const float* e = queries + (size_t)item * 10;
         float in[kGuideWidth], O[kGuideWidth];
@@ -43,8 +43,9 @@ bool launchGuideInferQueries(const GuideView& g, const float* queries, uint32_t 
         guide_inputs(g, e, e + 3, e + 6, e[9], in);
         forward(g, in, H, O);
         GuideLobes L;
-        decode(O, L);
-        float* o = lobes + (size_t)item * kGuideOutputs;
+        decode(g, O, L);
+        if (select) select[item] = L.select;
+        float* o = lobes + (size_t)item * 5 * kGuideLobes;
         for (int k = 0; k < kGuideLobes; ++k) {
             o[5 * k] = L.axis[k][0];
             o[5 * k + 1] = L.axis[k][1];
@@ -67,7 +68,7 @@ bool launchGuideInferPaths(const GuideView& g, const PassParams& P, uint32_t que
         guide_inputs(g, x, out, normal, nm, in);
         forward(g, in, H, O);
         GuideLobes L;
-        decode(O, L);
+        decode(g, O, L);
         store_lobes(lobes, P.s.capacity, i, L);
     }
     return true;
@@ -88,7 +89,7 @@ bool launchGuideTrain(const GuideView& g, const float* samples, uint32_t n, uint
             guide_inputs(g, smp + kSamPoint, smp + kSamOut, smp + kSamNormal, smp[kSamNm], in);
             forward(g, in, H, O);
             float grad[kGuideOutputs];
-            const float l = lobe_loss_gradient(O, smp + kSamIn, smp[kSamWeight] * *scale, grad);
+            const float l = lobe_loss_gradient(O, smp + kSamIn, smp[kSamWeight] * *scale, smp[kSamBsdf], g.share_cap, grad);
             if (l != 0) *loss += l;
             uint16_t G[kGuideWidth];
             for (int c = 0; c < kGuideWidth; ++c) G[c] = c < kGuideOutputs ? to_bf16(grad[c]) : (uint16_t)0;

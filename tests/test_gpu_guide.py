@@ -78,10 +78,12 @@ def test_inference_matches_the_oracle(guide):
     device, G = guide
     rng = np.random.default_rng(5)
     q = _queries(rng, 512)
-    got = device.guide_infer(q)
+    got, sel = device.guide_infer(q)
     X, _ = R.inputs(G, q[:, 0:3], q[:, 3:6], q[:, 6:9], q[:, 9])
     _, O = R.forward(G, X, bf=True)
     share, axis, sharp = R.decode(O.astype(np.float64))
+    ref_sel = R.select(O.astype(np.float64), G.cap)
+    assert np.max(np.abs(sel - ref_sel)) < 0.02 and np.all((sel >= 0) & (sel <= G.cap))
     d_share = np.abs(got[:, :, 4] - share)
     big = share > 0.05
     dots = np.sum(got[:, :, 0:3] * axis, axis=2)
@@ -103,6 +105,7 @@ def test_gradients_match_the_oracle(guide):
     S[:, 0:10] = _queries(rng, n)
     S[:, 10:13] = _unit(rng, n)
     S[:, 13] = rng.uniform(0.1, 2.0, n)
+    S[:, 14] = np.where(np.arange(n) % 2 == 0, rng.uniform(0.05, 0.6, n), -1.0)
     loss, grid_grad, mlp_grad = device.guide_gradients(S)
     ref_loss, ref_grid, ref_mlp = R.gradients(G, S, bf=True)
     note(f"[guide] loss device {loss:.6f} oracle {ref_loss:.6f}")
@@ -140,6 +143,7 @@ def _target_samples(rng, n):
         S[j, 10:13] = R.mixture_sample(np.array([1.0]), axis[j:j + 1], np.array([20.0]), np.array([0.0]),
                                        rng.random(1), rng.random(1))[0]
     S[:, 13] = 1
+    S[:, 14] = -1  # the lobes alone
     return S
 
 

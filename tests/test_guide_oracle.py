@@ -1,7 +1,8 @@
 """The learned guide's CPU oracle, pinned by mathematics rather than by its
 own formulas:
-- its gradients (through the loss, the four layers and the hash grid)
-  equal central finite differences of its FP64 loss;
+- its gradients (through the loss on the lobes alone and on the one-sample
+  MIS density with the selection probability, the four layers and the
+  hash grid) equal central finite differences of its FP64 loss;
 - each vMF lobe, and so the mixture, integrates to one over the sphere
   (a fine spherical quadrature), and sampling by Jakob's inversion draws
   directions in proportion to the density (binned counts within 5 sigma);
@@ -34,6 +35,9 @@ def _samples(rng, n):
     S[:, 9] = rng.uniform(380, 780, n)
     S[:, 10:13] = _unit(rng, n)
     S[:, 13] = rng.uniform(0.1, 2.0, n)
+    # half with the BSDF's density (the loss on the MIS density, the
+    # selection trained too), half without (the loss on the lobes alone)
+    S[:, 14] = np.where(np.arange(n) % 2 == 0, rng.uniform(0.05, 0.6, n), -1.0)
     return S
 
 
@@ -71,14 +75,15 @@ def test_loss_gradient_equals_finite_differences_of_the_outputs():
     O = rng.standard_normal((6, R.WIDTH))
     w_in = _unit(rng, 6)
     wgt = rng.uniform(0.5, 2, 6)
-    loss, G = R.loss_gradient(O, w_in, wgt)
+    bsdf = np.array([0.3, -1, 0.05, -1, 1.2, 0.0])
+    loss, G = R.loss_gradient(O, w_in, wgt, bsdf)
     h = 1e-6
     for b in range(6):
         for c in list(range(R.OUTPUTS)) + [R.OUTPUTS + 3]:
             Op, Om = O.copy(), O.copy()
             Op[b, c] += h
             Om[b, c] -= h
-            fd = (R.loss_gradient(Op, w_in, wgt)[0][b] - R.loss_gradient(Om, w_in, wgt)[0][b]) / (2 * h)
+            fd = (R.loss_gradient(Op, w_in, wgt, bsdf)[0][b] - R.loss_gradient(Om, w_in, wgt, bsdf)[0][b]) / (2 * h)
             assert abs(fd - G[b, c]) <= 1e-6 + 1e-5 * abs(fd), (b, c, fd, G[b, c])
 
 
@@ -123,10 +128,10 @@ def test_lobes_integrate_to_one(log_sharp):
     dirs = _fib(400000)
     rng = np.random.default_rng(11)
     O = np.zeros((1, R.WIDTH))
-    O[0, 0:R.OUTPUTS:5] = rng.standard_normal(R.LOBES)
+    O[0, 0:5 * R.LOBES:5] = rng.standard_normal(R.LOBES)
     for k in range(R.LOBES):
         O[0, 5 * k + 1:5 * k + 4] = rng.standard_normal(3)
-    O[0, 4:R.OUTPUTS:5] = log_sharp
+    O[0, 4:5 * R.LOBES:5] = log_sharp
     share, axis, sharp = R.decode(O)
     assert abs(share.sum() - 1) < 1e-12
     q = R.mixture_pdf(share, axis, sharp, dirs[None])[0]
@@ -136,7 +141,7 @@ def test_lobes_integrate_to_one(log_sharp):
 def test_sampling_follows_the_density():
     rng = np.random.default_rng(13)
     O = np.zeros((1, R.WIDTH))
-    O[0, 0:R.OUTPUTS:5] = rng.standard_normal(R.LOBES)
+    O[0, 0:5 * R.LOBES:5] = rng.standard_normal(R.LOBES)
     for k in range(R.LOBES):
         O[0, 5 * k + 1:5 * k + 4] = rng.standard_normal(3)
         O[0, 5 * k + 4] = rng.uniform(0, 4)
