@@ -141,7 +141,7 @@ MLP_FLOPS_PER_QUERY = 2 * 64 * 64 * 4  # four 64x64 layers, multiply-adds as two
 BF16_PEAK_TFLOPS = 2500.0
 
 
-def guided_leg(k, dev, label, text, depth, W, H, spp, passes):
+def guided_leg(k, dev, label, text, depth, W, H, spp, passes, config=None):
     """Plain and learned-guided renders of the same samples of one image.
 
     Both estimators are unbiased (the guided one by one-sample MIS; its
@@ -170,6 +170,8 @@ def guided_leg(k, dev, label, text, depth, W, H, spp, passes):
     for method, flags in (("plain", 0), ("guided", k.PASS_GUIDE)):
         if flags:
             dev.guide_reset(1)
+            if config:
+                dev.guide_configure(**config)
         stats = []
         start = time.perf_counter()
         for p in range(passes):

@@ -419,7 +419,8 @@ typedef struct kdn_guide_config {
     uint32_t max_steps;      /* and at most (64, the limit) */
     uint32_t max_resolution; /* the finest grid level's cells per side (128) */
     uint32_t train_paths;    /* paths of a round that keep training records, at most (2^20) */
-    uint32_t pad;
+    uint32_t learn_select;   /* 1: the selection probability is learned (up to
+                                share); 0: it is share itself (1) */
 } kdn_guide_config;
 
 /* Initialise the network from a seed over a region (NULL: the octree's);

@@ -61,7 +61,8 @@ struct GuideView {
     uint32_t res[kGuideLevels];
     uint32_t dense;          // bit l: level l is indexed densely
     float lower[3], inv_size[3];
-    float share_cap;         // the selection probability is share_cap * sigmoid(logit)
+    float share_cap;         // the selection probability: share_cap * sigmoid(logit) if learned
+    uint32_t learn_select;   // 0: the selection probability is share_cap itself
 };
 
 // ---- BF16 (round to nearest even; NaN kept quiet) ---------------------------
@@ -176,7 +177,7 @@ KDN_D float clamp_log_sharp(float c) { return c < kGuideLogSharpMin ? kGuideLogS
 
 KDN_D float sigmoid(float x) { return 1 / (1 + expf(-x)); }
 
-KDN_D void decode_lobes(const float* o, float cap, GuideLobes& L) {
+KDN_D void decode_lobes(const float* o, float cap, bool learned, GuideLobes& L) {
     float top = o[0];
     for (int k = 1; k < kGuideLobes; ++k) top = fmax_(top, o[5 * k]);
     float sum = 0;
@@ -197,7 +198,7 @@ KDN_D void decode_lobes(const float* o, float cap, GuideLobes& L) {
         L.sharp[k] = expf(clamp_log_sharp(o[5 * k + 4]));
     }
     for (int k = 0; k < kGuideLobes; ++k) L.share[k] /= sum;
-    L.select = cap * sigmoid(o[kGuideSelect]);
+    L.select = learned ? cap * sigmoid(o[kGuideSelect]) : cap;
 }
 
 KDN_D float mixture_pdf(const GuideLobes& L, V3 w) {
@@ -228,11 +229,11 @@ KDN_D V3 mixture_sample(const GuideLobes& L, float uLobe, float u, float v) {
 // the loss is -weight * log p of the one-sample MIS density
 // p = a q + (1 - a) pB, which trains the lobes and the selection a
 // together (Neural Parametric Mixtures); without it, -weight * log q.
-KDN_D float lobe_loss_gradient(const float* o, const float in[3], float weight, float bsdf, float cap, float* g) {
+KDN_D float lobe_loss_gradient(const float* o, const float in[3], float weight, float bsdf, float cap, bool learned, float* g) {
     for (int c = 0; c < kGuideOutputs; ++c) g[c] = 0;
     if (!(weight > 0)) return 0.0f;
     GuideLobes L;
-    decode_lobes(o, cap, L);
+    decode_lobes(o, cap, learned, L);
     float v[kGuideLobes], dots[kGuideLobes];
     float q = 0;
     for (int k = 0; k < kGuideLobes; ++k) {
@@ -246,8 +247,10 @@ KDN_D float lobe_loss_gradient(const float* o, const float in[3], float weight, 
         const float a = L.select;
         const float p = a * q + (1 - a) * bsdf;
         if (!(p > 1e-30f)) return 0.0f;
-        const float s = sigmoid(o[kGuideSelect]);
-        g[kGuideSelect] = -weight * (q - bsdf) / p * cap * s * (1 - s);
+        if (learned) {
+            const float s = sigmoid(o[kGuideSelect]);
+            g[kGuideSelect] = -weight * (q - bsdf) / p * cap * s * (1 - s);
+        }
         loss = -weight * logf(p);
         weight *= a * q / p;  // the lobes' share of the gradient
     } else {

@@ -37,7 +37,7 @@ constexpr float kGuideLearningRate = 1e-2f;
 constexpr float kGuideBeta1 = 0.9f, kGuideBeta2 = 0.99f, kGuideEps = 1e-8f;
 
 constexpr kdn_guide_config defaultConfig() {
-    return kdn_guide_config{kGuideLearningRate, kGuideShare, 2.0f, 16, kGuideStepsPerRound, kGuideMaxRes, kGuideTrainPaths, 0};
+    return kdn_guide_config{kGuideLearningRate, kGuideShare, 2.0f, 16, kGuideStepsPerRound, kGuideMaxRes, kGuideTrainPaths, 1};
 }
 
 struct GuideState {
@@ -297,6 +297,7 @@ bool guideReset(Ctx& c, uint32_t seed, const float lower[3], const float size[3]
     }
     viewOf(G, G.config.max_resolution);
     G.view.share_cap = G.config.share;
+    G.view.learn_select = G.config.learn_select;
     KDN_HIP(hipMemsetAsync(G.gridM(), 0, 3 * kGuideGridParams * sizeof(float), st));
     KDN_HIP(hipMemsetAsync(G.mlpM(), 0, 3 * (size_t)kGuideMlpParams * sizeof(float), st));
     hipLaunchKernelGGL(k_guide_init, dim3((uint32_t)((kGuideGridParams + 255) / 256)), dim3(256), 0, st, G.grid, G.mlp, G.mlp_bf, seed);
@@ -461,7 +462,7 @@ int kdn_guide_configure(kdn_ctx* ctx, const kdn_guide_config* config) {
     const kdn_guide_config& k = *config;
     if (!(k.learning_rate > 0 && k.learning_rate < 1) || !(k.share >= 0 && k.share <= 1) || !(k.epochs > 0 && k.epochs <= 64) ||
         k.min_steps > k.max_steps || k.max_steps > kGuideStepsPerRound || k.max_resolution < kGuideMinRes ||
-        k.max_resolution > 65536 || k.train_paths < 1024 || k.train_paths > (1u << 22)) {
+        k.max_resolution > 65536 || k.train_paths < 1024 || k.train_paths > (1u << 22) || k.learn_select > 1) {
         return guideFail(ctx, "kdn_guide_configure: learning rate in (0, 1), share in [0, 1], epochs in (0, 64], "
                               "min_steps <= max_steps <= 64, max_resolution in [16, 65536], train_paths in [2^10, 2^22]",
                          KDN_E_ARGUMENT);
@@ -469,6 +470,7 @@ int kdn_guide_configure(kdn_ctx* ctx, const kdn_guide_config* config) {
     G->config = k;
     viewOf(*G, k.max_resolution);
     G->view.share_cap = k.share;
+    G->view.learn_select = k.learn_select;
     return KDN_OK;
 }
 

@@ -169,7 +169,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideView g, Q
             float o[kGuideOutputs];
             for (int c = 0; c < kGuideOutputs; ++c) o[c] = O[lane * kOutLd + c];
             GuideLobes L;
-            decode_lobes(o, g.share_cap, L);
+            decode_lobes(o, g.share_cap, g.learn_select != 0, L);
             src.write(item, L);
         }
         __syncthreads();
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideView g, const float* sa
         float o[kGuideOutputs], grad[kGuideOutputs];
         for (int c = 0; c < kGuideOutputs; ++c) o[c] = O[lane * kOutLd + c];
         const float w = live ? smp[kSamWeight] * *scale : 0.0f;
-        const float l = lobe_loss_gradient(o, smp + kSamIn, w, live ? smp[kSamBsdf] : -1.0f, g.share_cap, grad);
+        const float l = lobe_loss_gradient(o, smp + kSamIn, w, live ? smp[kSamBsdf] : -1.0f, g.share_cap, g.learn_select != 0, grad);
         for (int c = 0; c < kGuideWidth; ++c) G[0][lane * kLd + c] = c < kGuideOutputs ? to_bf16(grad[c]) : (uint16_t)0;
         if (l != 0) atomicAdd(loss, l);
     }

@@ -291,16 +291,7 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
     const float tHit = hit.kind == KDN_HIT_MISSED ? kInf : hit.distance;
     const bool exactBefore = (meta & FlagExact) != 0;
     const float pdfBefore = s.pdf[i];
-    // training records: the radiance the path holds as this vertex begins
-    // closes the record of the vertex before
     const bool training = P.recs != nullptr && i % P.train_stride == 0;
-    if (training && depth >= 1 && depth - 1 < kGuideDepths) {
-        float* rec = P.recs + ((size_t)(i / P.train_stride) * kGuideDepths + depth - 1) * kRecFloats;
-        if (rec[kRecState] == 1.0f) {
-            for (int l = 0; l < kLanes; ++l) rec[kRecSnap + l] = rad[l];
-            rec[kRecState] = 3.0f;
-        }
-    }
 
     // lamps met along the ray (metAlong, trace.cpp:352-374), MIS against aiming
     if (!inside) {
@@ -313,6 +304,17 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
             const float pdfL = L.pick * lh.t * lh.t / (lh.cosine * L.area);
             const float wt = exactBefore ? 1.0f : power_weight(pdfBefore, pdfL);
             for (int l = 0; l < kLanes; ++l) rad[l] += thr[l] * Le[l] * wt;
+        }
+    }
+    // training records: the radiance the path holds as this vertex begins
+    // (lamps met on the way counted in: lamps are aimed at from every
+    // guided vertex, so the guide learns the light that aiming misses)
+    // closes the record of the vertex before
+    if (training && depth >= 1 && depth - 1 < kGuideDepths) {
+        float* rec = P.recs + ((size_t)(i / P.train_stride) * kGuideDepths + depth - 1) * kRecFloats;
+        if (rec[kRecState] == 1.0f) {
+            for (int l = 0; l < kLanes; ++l) rec[kRecSnap + l] = rad[l];
+            rec[kRecState] = 3.0f;
         }
     }
     bool alive = false;

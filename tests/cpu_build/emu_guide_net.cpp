@@ -31,7 +31,7 @@ void forward(const GuideView& g, const float in[kGuideWidth], uint16_t H[kGuideL
     }
 }
 
-void decode(const GuideView& g, const float O[kGuideWidth], GuideLobes& L) { decode_lobes(O, g.share_cap, L); }
+void decode(const GuideView& g, const float O[kGuideWidth], GuideLobes& L) { decode_lobes(O, g.share_cap, g.learn_select != 0, L); }
 
 }  // namespace
 
@@ -89,7 +89,7 @@ bool launchGuideTrain(const GuideView& g, const float* samples, uint32_t n, uint
             guide_inputs(g, smp + kSamPoint, smp + kSamOut, smp + kSamNormal, smp[kSamNm], in);
             forward(g, in, H, O);
             float grad[kGuideOutputs];
-            const float l = lobe_loss_gradient(O, smp + kSamIn, smp[kSamWeight] * *scale, smp[kSamBsdf], g.share_cap, grad);
+            const float l = lobe_loss_gradient(O, smp + kSamIn, smp[kSamWeight] * *scale, smp[kSamBsdf], g.share_cap, g.learn_select != 0, grad);
             if (l != 0) *loss += l;
             uint16_t G[kGuideWidth];
             for (int c = 0; c < kGuideWidth; ++c) G[c] = c < kGuideOutputs ? to_bf16(grad[c]) : (uint16_t)0;
