@@ -411,16 +411,18 @@ typedef struct kdn_guide_info {
 /* How a guided pass uses and trains the guide. */
 typedef struct kdn_guide_config {
     float learning_rate;     /* Adam (default 1e-2) */
-    float share;             /* the most of the sampled directions the guide may
-                                take: its selection probability is share times
-                                the sigmoid of a learned output (0.9) */
+    float share;             /* the guide's share of the sampled directions (0.3);
+                                with learn_select, the most of them it may take */
     float epochs;            /* passes over a round's samples (2) */
     uint32_t min_steps;      /* Adam steps a round, at least (16) */
     uint32_t max_steps;      /* and at most (64, the limit) */
     uint32_t max_resolution; /* the finest grid level's cells per side (128) */
     uint32_t train_paths;    /* paths of a round that keep training records, at most (2^20) */
-    uint32_t learn_select;   /* 1: the selection probability is learned (up to
-                                share); 0: it is share itself (1) */
+    uint32_t learn_select;   /* 1: the selection probability is share times the
+                                sigmoid of a learned output; 0: share itself (0) */
+    uint32_t product;        /* 1: the lobes learn the integrand f cos L;
+                                0: the incident radiance L (1) */
+    uint32_t pad;
 } kdn_guide_config;
 
 /* Initialise the network from a seed over a region (NULL: the octree's);

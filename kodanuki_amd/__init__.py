@@ -122,7 +122,7 @@ class GuideInfo(C.Structure):
 class GuideConfig(C.Structure):
     _fields_ = [("learning_rate", C.c_float), ("share", C.c_float), ("epochs", C.c_float), ("min_steps", C.c_uint32),
                 ("max_steps", C.c_uint32), ("max_resolution", C.c_uint32), ("train_paths", C.c_uint32),
-                ("learn_select", C.c_uint32)]
+                ("learn_select", C.c_uint32), ("product", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class GuideQuery(C.Structure):
@@ -437,10 +437,10 @@ class Device:
         sz = (C.c_float * 3)(*[float(x) for x in size]) if size is not None else None
         _check(library().kdn_guide_reset(self._h, seed, lo, sz), self._h)
 
-    def guide_configure(self, learning_rate=1e-2, share=0.9, epochs=2.0, min_steps=16, max_steps=64, max_resolution=128,
-                        train_paths=1 << 20, learn_select=True):
+    def guide_configure(self, learning_rate=1e-2, share=0.3, epochs=2.0, min_steps=16, max_steps=64, max_resolution=128,
+                        train_paths=1 << 20, learn_select=False, product=True):
         cfg = GuideConfig(learning_rate, share, epochs, min_steps, max_steps, max_resolution, train_paths,
-                          1 if learn_select else 0)
+                          1 if learn_select else 0, 1 if product else 0, 0)
         _check(library().kdn_guide_configure(self._h, C.byref(cfg)), self._h)
 
     def guide_info(self):

@@ -31,7 +31,7 @@ constexpr int kGuideOutputs = 5 * kGuideLobes + 1;  // logit, axis (3), log shar
 constexpr int kGuideSelect = 5 * kGuideLobes;       // output of the selection logit
 constexpr int kGuideDepths = 4;           // guided vertices: path depths 0 .. 3
 constexpr uint32_t kGuideTrainPaths = 1u << 20;  // paths of a round that keep training records, at most (default)
-constexpr float kGuideShare = 0.9f;       // the most of the directions the guide may take (one-sample MIS)
+constexpr float kGuideShare = 0.3f;       // the guide's share of directions (one-sample MIS; the most if learned)
 constexpr float kGuideLogSharpMin = -3.0f;  // sharpness within [e^-3, e^6.2]
 constexpr float kGuideLogSharpMax = 6.2f;
 constexpr int kGuideMinRes = 16, kGuideMaxRes = 128;  // grid resolutions (the finest by default)
@@ -45,9 +45,10 @@ constexpr int kInGrid = 0, kInSh = 32, kInNormal = 48, kInWave = 51, kInOne = 59
 // direction, normal, hero wavelength, the direction taken, its density,
 // the throughput after the vertex (4 lanes), the path's radiance when the
 // next vertex begins (4 lanes), a state word (1 recorded, 3 snapshot
-// taken) and the BSDF's density of the direction.
+// taken), the BSDF's density of the direction and the throughput before
+// the vertex (4 lanes).
 constexpr int kRecPoint = 0, kRecOut = 3, kRecNormal = 6, kRecNm = 9, kRecIn = 10, kRecPdf = 13, kRecThr = 14,
-              kRecSnap = 18, kRecState = 22, kRecBsdf = 23, kRecFloats = 24;
+              kRecSnap = 18, kRecState = 22, kRecBsdf = 23, kRecBefore = 24, kRecFloats = 28;
 // A training sample (floats): point, outgoing direction, normal, hero
 // wavelength, the direction taken, its weight (radiance over density),
 // the BSDF's density of the direction (negative: train the lobes alone).

@@ -37,7 +37,7 @@ constexpr float kGuideLearningRate = 1e-2f;
 constexpr float kGuideBeta1 = 0.9f, kGuideBeta2 = 0.99f, kGuideEps = 1e-8f;
 
 constexpr kdn_guide_config defaultConfig() {
-    return kdn_guide_config{kGuideLearningRate, kGuideShare, 2.0f, 16, kGuideStepsPerRound, kGuideMaxRes, kGuideTrainPaths, 1};
+    return kdn_guide_config{kGuideLearningRate, kGuideShare, 2.0f, 16, kGuideStepsPerRound, kGuideMaxRes, kGuideTrainPaths, 0, 1, 0};
 }
 
 struct GuideState {
@@ -130,8 +130,8 @@ __global__ void k_guide_adam(float* p, float* m, float* v, float* g, uint64_t n,
 // the samples of a round's records: weight = radiance back along the
 // direction (final less snapshot, over the throughput, mean of the live
 // lanes) over its density; valid[r] = 1 for a sample that carries some
-__global__ void k_guide_collect(const float* recs, uint32_t nRec, uint32_t stride, const float* rad, size_t N, float* samples,
-                                uint32_t* valid) {
+__global__ void k_guide_collect(const float* recs, uint32_t nRec, uint32_t stride, const float* rad, size_t N, uint32_t product,
+                                float* samples, uint32_t* valid) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nRec) return;
     const float* rec = recs + (size_t)r * kRecFloats;
@@ -140,15 +140,18 @@ __global__ void k_guide_collect(const float* recs, uint32_t nRec, uint32_t strid
         const size_t i = (size_t)(r / kGuideDepths) * stride;
         float sum = 0;
         int live = 0;
+        // the incident radiance over the density, or (product) the
+        // integrand f cos L over the density: radiance back over the
+        // throughput before the vertex
         for (int l = 0; l < kLanes; ++l) {
-            const float t = rec[kRecThr + l];
-            if (!(t > 0)) continue;
+            const float t = rec[(product ? kRecBefore : kRecThr) + l];
+            if (!(t > 0) || !(rec[kRecThr + l] > 0)) continue;
             const float back = rad[l * N + i] - rec[kRecSnap + l];
             sum += (back > 0 ? back : 0.0f) / t;
             ++live;
         }
         const float pdf = rec[kRecPdf];
-        const float w = live > 0 && pdf > 0 ? sum / live / pdf : 0.0f;
+        const float w = live > 0 && pdf > 0 ? (product ? sum / live : sum / live / pdf) : 0.0f;
         if (w > 0 && w <= 1e30f) {
             float* s = samples + (size_t)r * kSamFloats;
             for (int k = 0; k < 13; ++k) s[k] = rec[k];
@@ -365,7 +368,7 @@ bool guideEndRound(Ctx& c, const PassParams& P, uint32_t paths, kdn_pass_stats* 
     float* raw = (float*)G.samples.ptr;
     float* packed = raw + (size_t)nRec * kSamFloats;
     hipLaunchKernelGGL(k_guide_collect, dim3((nRec + 255) / 256), dim3(256), 0, st, (const float*)G.recs.ptr, nRec, stride, (const float*)P.s.rad,
-                       P.s.capacity, raw, (uint32_t*)G.flags.ptr);
+                       P.s.capacity, G.config.product, raw, (uint32_t*)G.flags.ptr);
     uint64_t total = 0;
     if (!scanU32((const uint32_t*)G.flags.ptr, (uint32_t*)G.offsets.ptr, nRec, &total, G.scan_tmp, st, c.error)) return false;
     const uint32_t n = (uint32_t)total;
@@ -462,7 +465,7 @@ int kdn_guide_configure(kdn_ctx* ctx, const kdn_guide_config* config) {
     const kdn_guide_config& k = *config;
     if (!(k.learning_rate > 0 && k.learning_rate < 1) || !(k.share >= 0 && k.share <= 1) || !(k.epochs > 0 && k.epochs <= 64) ||
         k.min_steps > k.max_steps || k.max_steps > kGuideStepsPerRound || k.max_resolution < kGuideMinRes ||
-        k.max_resolution > 65536 || k.train_paths < 1024 || k.train_paths > (1u << 22) || k.learn_select > 1) {
+        k.max_resolution > 65536 || k.train_paths < 1024 || k.train_paths > (1u << 22) || k.learn_select > 1 || k.product > 1) {
         return guideFail(ctx, "kdn_guide_configure: learning rate in (0, 1), share in [0, 1], epochs in (0, 64], "
                               "min_steps <= max_steps <= 64, max_resolution in [16, 65536], train_paths in [2^10, 2^22]",
                          KDN_E_ARGUMENT);

@@ -431,7 +431,6 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
                         nr.inside = nowInside ? 1u : 0u;
                         nr.within = nowInside ? mi : KDN_NO_MATERIAL;
                         s.rays[i] = nr;
-                        for (int l = 0; l < kLanes; ++l) thr[l] = carried[l];
                         s.pdf[i] = sc.exact ? 0.0f : sc.density;
                         meta = (meta & ~(0xFFu | FlagExact)) | (uint32_t)(depth + 1) | (sc.exact ? FlagExact : 0u);
                         alive = true;
@@ -439,10 +438,14 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
                             float* rec = P.recs + ((size_t)(i / P.train_stride) * kGuideDepths + depth) * kRecFloats;
                             const float q[14] = {x.x, x.y, x.z, out.x, out.y, out.z, n.x, n.y, n.z, w.nm[0], sc.in.x, sc.in.y, sc.in.z, sc.density};
                             for (int k = 0; k < 14; ++k) rec[k] = q[k];
-                            for (int l = 0; l < kLanes; ++l) rec[kRecThr + l] = carried[l];
+                            for (int l = 0; l < kLanes; ++l) {
+                                rec[kRecThr + l] = carried[l];
+                                rec[kRecBefore + l] = thr[l];
+                            }
                             rec[kRecBsdf] = rough_density(m, n, out, sc.in);
                             rec[kRecState] = 1.0f;
                         }
+                        for (int l = 0; l < kLanes; ++l) thr[l] = carried[l];
                     }
                 }
             }

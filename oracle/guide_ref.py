@@ -18,8 +18,8 @@ plugins/transport/paths/sampler.h:23,357):
 - the outputs as 8 von Mises-Fisher lobes: softmax shares, normalised
   axes, sharpness exp(clamp(c, -3, 6.2)); density
   sum share * k / (2 pi (1 - e^-2k)) e^(k (axis . w - 1)); and the
-  selection probability a = cap * sigmoid(output 40) of sampling the
-  mixture rather than the BSDF;
+  probability a of sampling the mixture rather than the BSDF: a fixed
+  share, or the share times sigmoid(output 40) when it is learned;
 - the loss -weight * log p of the one-sample MIS density
   p = a q + (1 - a) pB (pB the BSDF's density of the direction), or
   -weight * log q for samples without one, and its gradient through the
@@ -38,7 +38,7 @@ import numpy as np
 LEVELS, FEATURES, TABLE, WIDTH, LAYERS, LOBES = 16, 2, 1 << 19, 64, 4, 8
 SELECT = 5 * LOBES  # the output of the selection logit
 OUTPUTS = 5 * LOBES + 1
-DEPTHS, SHARE_CAP = 4, 0.9
+DEPTHS, SHARE = 4, 0.3
 LOG_SHARP_MIN, LOG_SHARP_MAX = -3.0, 6.2
 MIN_RES, MAX_RES = 16, 128  # the device's default; 2048 and others by argument
 IN_GRID, IN_SH, IN_NORMAL, IN_WAVE, IN_ONE = 0, 32, 48, 51, 59
@@ -70,13 +70,14 @@ class Guide:
     """Parameters of a guide: grid [LEVELS, TABLE, FEATURES] and
     mlp [LAYERS, WIDTH, WIDTH] (input, output), the region."""
 
-    def __init__(self, grid, mlp, lower, size, max_res=MAX_RES, cap=SHARE_CAP):
+    def __init__(self, grid, mlp, lower, size, max_res=MAX_RES, cap=SHARE, learned=False):
         self.grid = np.asarray(grid).reshape(LEVELS, TABLE, FEATURES)
         self.mlp = np.asarray(mlp).reshape(LAYERS, WIDTH, WIDTH)
         self.lower = np.asarray(lower, np.float32)
         self.inv_size = (np.float32(1) / np.asarray(size, np.float32)).astype(np.float32)
         self.res, self.dense = resolutions(max_res)
         self.cap = cap
+        self.learned = learned
 
 
 def _corners(g, level, p, dt):
@@ -170,9 +171,10 @@ def forward(g, X, bf=True):
     return H, O
 
 
-def select(O, cap=SHARE_CAP):
-    """The selection probability [B]."""
-    return cap / (1 + np.exp(-np.asarray(O)[:, SELECT]))
+def select(O, cap=SHARE, learned=False):
+    """The selection probability [B]: cap, or cap * sigmoid(output 40)."""
+    O = np.asarray(O)
+    return cap / (1 + np.exp(-O[:, SELECT])) if learned else np.full(len(O), cap)
 
 
 def decode(O):
@@ -226,7 +228,7 @@ def _basis(n):
     return t, np.cross(n, t)
 
 
-def loss_gradient(O, w_in, weight, bsdf=None, cap=SHARE_CAP):
+def loss_gradient(O, w_in, weight, bsdf=None, cap=SHARE, learned=True):
     """Per-sample loss [B] and its gradient dL/dO [B, 64]: -weight log p of
     p = a q + (1 - a) bsdf where bsdf >= 0, -weight log q where it is
     negative or not given (zero for samples whose density underflows or
@@ -240,7 +242,7 @@ def loss_gradient(O, w_in, weight, bsdf=None, cap=SHARE_CAP):
     v = vmf_norm(sharp) * np.exp(sharp * (dots - 1))
     q = np.sum(share * v, axis=1)
     s = 1 / (1 + np.exp(-O[:, SELECT]))
-    a = cap * s
+    a = cap * s if learned else np.full(n, cap)
     mixed = bsdf >= 0
     p = np.where(mixed, a * q + (1 - a) * np.maximum(bsdf, 0), q)
     ok = (q > 1e-30) & (weight > 0) & (p > 1e-30)
@@ -250,7 +252,8 @@ def loss_gradient(O, w_in, weight, bsdf=None, cap=SHARE_CAP):
     w0 = np.where(ok, weight, 0).astype(dt)
     loss = np.where(ok, -w0 * np.log(ps), 0)
     G = np.zeros((n, WIDTH), dt)
-    G[:, SELECT] = np.where(mixed & ok, -w0 * (q - np.maximum(bsdf, 0)) / ps * cap * s * (1 - s), 0)
+    if learned:
+        G[:, SELECT] = np.where(mixed & ok, -w0 * (q - np.maximum(bsdf, 0)) / ps * cap * s * (1 - s), 0)
     wgt = np.where(mixed, w0 * a * q / ps, w0).astype(dt)
     G[:, 0:5 * LOBES:5] = -wgt[:, None] * (gamma - share)
     raw = np.stack([O[:, 5 * k + 1:5 * k + 4] for k in range(LOBES)], axis=1)
@@ -273,7 +276,7 @@ def gradients(g, samples, bf=True):
     S = np.asarray(samples, np.float64 if not bf else np.float32)
     X, corners = inputs(g, S[:, 0:3], S[:, 3:6], S[:, 6:9], S[:, 9], dt)
     H, O = forward(g, X, bf)
-    loss, G = loss_gradient(O, S[:, 10:13].astype(dt), S[:, 13].astype(dt), S[:, 14], g.cap)
+    loss, G = loss_gradient(O, S[:, 10:13].astype(dt), S[:, 13].astype(dt), S[:, 14], g.cap, g.learned)
     W = g.mlp.astype(dt)
     if bf:
         W = bf16(W)
@@ -303,5 +306,5 @@ def loss_of(g, samples):
     S = np.asarray(samples, np.float64)
     X, _ = inputs(g, S[:, 0:3], S[:, 3:6], S[:, 6:9], S[:, 9], np.float64)
     _, O = forward(g, X, bf=False)
-    loss, _ = loss_gradient(O, S[:, 10:13], S[:, 13], S[:, 14], g.cap)
+    loss, _ = loss_gradient(O, S[:, 10:13], S[:, 13], S[:, 14], g.cap, g.learned)
     return float(np.sum(loss))

@@ -52,13 +52,14 @@ def _queries(rng, n):
     return q
 
 
-@pytest.fixture(scope="module", params=[128, 2048], ids=["finest128", "finest2048"])
+@pytest.fixture(scope="module", params=[(128, False), (2048, True)], ids=["finest128-fixed", "finest2048-learned"])
 def guide(device, request):
+    res, learned = request.param
     device.guide_reset(3, LOWER, SIZE)
-    device.guide_configure(max_resolution=request.param)
+    device.guide_configure(max_resolution=res, learn_select=learned, share=0.9 if learned else 0.3)
     grid, mlp = _params(17)
     device.guide_set_params(grid, mlp)
-    return device, R.Guide(grid, mlp, LOWER, SIZE, max_res=request.param)
+    return device, R.Guide(grid, mlp, LOWER, SIZE, max_res=res, cap=0.9 if learned else 0.3, learned=learned)
 
 
 def test_reset_is_deterministic_and_described(device):
@@ -82,7 +83,7 @@ def test_inference_matches_the_oracle(guide):
     X, _ = R.inputs(G, q[:, 0:3], q[:, 3:6], q[:, 6:9], q[:, 9])
     _, O = R.forward(G, X, bf=True)
     share, axis, sharp = R.decode(O.astype(np.float64))
-    ref_sel = R.select(O.astype(np.float64), G.cap)
+    ref_sel = R.select(O.astype(np.float64), G.cap, G.learned)
     assert np.max(np.abs(sel - ref_sel)) < 0.02 and np.all((sel >= 0) & (sel <= G.cap))
     d_share = np.abs(got[:, :, 4] - share)
     big = share > 0.05

@@ -15,11 +15,11 @@ import pytest
 from oracle import guide_ref as R
 
 
-def _guide(seed, grid_scale=0.05, max_res=2048):
+def _guide(seed, grid_scale=0.05, max_res=2048, learned=True):
     rng = np.random.default_rng(seed)
     grid = (rng.standard_normal((R.LEVELS, R.TABLE, R.FEATURES)) * grid_scale).astype(np.float64)
     mlp = rng.uniform(-0.3, 0.3, (R.LAYERS, R.WIDTH, R.WIDTH))
-    return R.Guide(grid, mlp, lower=[-2, -2, -1], size=[4, 4, 4], max_res=max_res)
+    return R.Guide(grid, mlp, lower=[-2, -2, -1], size=[4, 4, 4], max_res=max_res, cap=0.9, learned=learned)
 
 
 def _unit(rng, n):
@@ -76,19 +76,20 @@ def test_loss_gradient_equals_finite_differences_of_the_outputs():
     w_in = _unit(rng, 6)
     wgt = rng.uniform(0.5, 2, 6)
     bsdf = np.array([0.3, -1, 0.05, -1, 1.2, 0.0])
-    loss, G = R.loss_gradient(O, w_in, wgt, bsdf)
+    loss, G = R.loss_gradient(O, w_in, wgt, bsdf, cap=0.9)
     h = 1e-6
     for b in range(6):
         for c in list(range(R.OUTPUTS)) + [R.OUTPUTS + 3]:
             Op, Om = O.copy(), O.copy()
             Op[b, c] += h
             Om[b, c] -= h
-            fd = (R.loss_gradient(Op, w_in, wgt, bsdf)[0][b] - R.loss_gradient(Om, w_in, wgt, bsdf)[0][b]) / (2 * h)
+            fd = (R.loss_gradient(Op, w_in, wgt, bsdf, cap=0.9)[0][b] - R.loss_gradient(Om, w_in, wgt, bsdf, cap=0.9)[0][b]) / (2 * h)
             assert abs(fd - G[b, c]) <= 1e-6 + 1e-5 * abs(fd), (b, c, fd, G[b, c])
 
 
-def test_network_gradients_equal_finite_differences():
-    g = _guide(5)
+@pytest.mark.parametrize("learned", [True, False])
+def test_network_gradients_equal_finite_differences(learned):
+    g = _guide(5, learned=learned)
     rng = np.random.default_rng(7)
     S = _samples(rng, 24)
     loss, grid, dW = R.gradients(g, S, bf=False)
