@@ -84,8 +84,8 @@ __global__ void k_guide_init(float* grid, float* mlp, uint16_t* bf, uint32_t see
     }
 }
 
-// the sum of each weight's gradient over the tiles, in tile order; a
-// non-finite sum raises the step's flag
+// the sum of each weight's gradient over the training blocks' partial
+// sums, in block order; a non-finite sum raises the step's flag
 __global__ void k_guide_reduce(const float* partial, uint32_t tiles, float* grad, uint32_t* flag) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (uint32_t)kGuideMlpParams) return;
@@ -239,7 +239,7 @@ bool trainSteps(Ctx& c, GuideState& G, const float* samples, uint32_t n, uint32_
     batch = std::min(batch, n);
     steps = std::min(steps, kGuideStepsPerRound);
     if (steps == 0 || batch == 0) return true;
-    const uint32_t tiles = (batch + 63) / 64;
+    const uint32_t tiles = guideTrainPartials(batch);
     if (!ensure(G.partial, (size_t)tiles * kGuideMlpParams * sizeof(float), c.error, false)) return false;
     KDN_HIP(hipMemsetAsync(G.scalars, 0, kScalarWords * sizeof(float), st));
     double* parts = (double*)(G.scalars + kScalarParts);
@@ -552,7 +552,7 @@ int kdn_guide_gradients(kdn_ctx* ctx, const kdn_guide_sample* samples, uint32_t 
     if (n > (1u << 20)) return guideFail(ctx, "kdn_guide_gradients: at most 2^20 samples a call", KDN_E_ARGUMENT);
     hipSetDevice(c.device);
     hipStream_t st = c.stream;
-    const uint32_t tiles = (n + 63) / 64;
+    const uint32_t tiles = guideTrainPartials(n);
     if (!ensure(c.scratch[6], (size_t)n * sizeof(kdn_guide_sample), c.error, false) ||
         !ensure(G->partial, (size_t)tiles * kGuideMlpParams * sizeof(float), c.error, false)) {
         return KDN_E_MEMORY;

@@ -13,9 +13,11 @@ bool launchGuideInferQueries(const GuideView& g, const float* queries, uint32_t 
 // lobes of the guided vertices among the first n paths of a queue, into
 // the per-path lobe buffer (lane-major)
 bool launchGuideInferPaths(const GuideView& g, const PassParams& P, uint32_t queue, uint32_t n, float* lobes, hipStream_t st);
-// forward and backward of samples [first, first + count) (modulo n): each
-// tile of 64 writes its weight gradients to partial[tile], adds its grid
-// gradients to gridGrad and its loss to *loss; sample weights times *scale
+// forward and backward of samples [first, first + count) (modulo n):
+// weight gradients summed into guideTrainPartials(count) partial sums of
+// kGuideMlpParams floats, grid gradients added to gridGrad, the loss to
+// *loss; sample weights times *scale
+uint32_t guideTrainPartials(uint32_t count);
 bool launchGuideTrain(const GuideView& g, const float* samples, uint32_t n, uint32_t first, uint32_t count, const float* scale,
                       float* partial, float* gridGrad, float* loss, hipStream_t st);
 

@@ -9,6 +9,8 @@
 // as BF16 rows, outputs as FP32.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "ctx.h"
 #include "guide_net.h"
 
@@ -21,6 +23,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kLd = 72;         // LDS row stride of a 64-wide BF16 matrix (elements)
 constexpr int kOutLd = 65;      // LDS row stride of FP32 outputs
 constexpr int kInferWaves = 4;  // waves per inference block
+constexpr uint32_t kTrainBlocks = 256;  // training blocks (one a CU: each holds 142 KB of LDS)
 
 // acc = A · B for a 64x64 tile: A row-major (rows = samples, lda), B given
 // transposed (Bt[n][k], ldb) so that a lane's eight consecutive k are
@@ -91,6 +94,21 @@ __device__ __forceinline__ void store_f32(const f32x4 acc[16], float* O) {
         for (int j = 0; j < 4; ++j) O[d_row(t, j) * kOutLd + d_col(t)] = acc[t][j];
 }
 
+// The layers' BF16 weights into LDS, eight at a time: Wt[layer][out][in]
+// for the forward products, and (Wn not null) Wn[layer][in][out] for the
+// backward ones.
+__device__ __forceinline__ void load_weights(const uint16_t* mlp, uint16_t (*Wt)[64 * kLd], uint16_t (*Wn)[64 * kLd], int threads) {
+    const uint4* src = (const uint4*)mlp;
+    for (int c = threadIdx.x; c < kGuideMlpParams / 8; c += threads) {
+        const uint4 v = src[c];
+        const int layer = c >> 9, in = (c >> 3) & 63, out0 = (c & 7) * 8;
+        uint16_t e[8];
+        __builtin_memcpy(e, &v, sizeof e);
+        for (int j = 0; j < 8; ++j) Wt[layer][(out0 + j) * kLd + in] = e[j];
+        if (Wn) *(uint4*)&Wn[layer][in * kLd + out0] = v;
+    }
+}
+
 // the queries of a host-given array
 struct ArrayQueries {
     const float* q;  // kdn_guide_query: point, out, normal, nm
@@ -140,10 +158,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideView g, Q
     __shared__ __attribute__((aligned(16))) uint16_t Wt[kGuideLayers][64 * kLd];
     __shared__ __attribute__((aligned(16))) uint16_t act[kInferWaves][64 * kLd];
     __shared__ float fo[kInferWaves][64 * kOutLd];
-    for (int e = threadIdx.x; e < kGuideMlpParams; e += blockDim.x) {
-        const int layer = e >> 12, in = (e >> 6) & 63, out = e & 63;
-        Wt[layer][out * kLd + in] = g.mlp[e];
-    }
+    load_weights(g.mlp, Wt, nullptr, 64 * kInferWaves);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint16_t* A = act[wave];
@@ -176,10 +191,11 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideView g, Q
     }
 }
 
-// One tile of 64 training samples per block (one wave): forward, the
-// loss gradient of each sample (its weight times *scale), backward.
-// Writes the tile's weight gradients to partial[block][layer][in][out],
-// adds the grid gradients to grid_grad, the tile's loss to *loss.
+// Training, one wave a block, persistent: each block takes tiles of 64
+// samples in turn (forward, the loss gradient of each sample, its weight
+// times *scale, backward) and sums its tiles' weight gradients into
+// partial[block][layer][in][out]; grid gradients are added to grid_grad
+// and each tile's loss to *loss.
 __global__ __launch_bounds__(64) void k_guide_train(GuideView g, const float* samples, uint32_t n, uint32_t first, uint32_t count,
                                                     const float* scale, float* partial, float* grid_grad, float* loss) {
     __shared__ __attribute__((aligned(16))) uint16_t Wt[kGuideLayers][64 * kLd];  // [out][in]
@@ -187,69 +203,76 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideView g, const float* sa
     __shared__ __attribute__((aligned(16))) uint16_t H[kGuideLayers][64 * kLd];   // inputs, h1, h2, h3
     __shared__ __attribute__((aligned(16))) uint16_t G[2][64 * kLd];              // gradients, ping-pong
     __shared__ float O[64 * kOutLd];
-    for (int e = threadIdx.x; e < kGuideMlpParams; e += 64) {
-        const int layer = e >> 12, in = (e >> 6) & 63, out = e & 63;
-        Wt[layer][out * kLd + in] = g.mlp[e];
-        Wn[layer][in * kLd + out] = g.mlp[e];
-    }
+    load_weights(g.mlp, Wt, Wn, 64);
     const int lane = threadIdx.x & 63;
-    const uint32_t b = blockIdx.x * 64 + lane;
-    const bool live = b < count && n > 0;
-    const float* smp = samples + (size_t)(live ? (first + b) % n : 0) * kSamFloats;
-    float in[kGuideWidth];
-    if (live) guide_inputs(g, smp + kSamPoint, smp + kSamOut, smp + kSamNormal, smp[kSamNm], in);
-    for (int k = 0; k < kGuideWidth; ++k) H[0][lane * kLd + k] = live ? to_bf16(in[k]) : (uint16_t)0;
-    __syncthreads();
-    f32x4 acc[16];
-    for (int layer = 0; layer < kGuideLayers - 1; ++layer) {
-        wave_gemm(H[layer], kLd, Wt[layer], kLd, acc);
-        store_relu(acc, H[layer + 1]);
-        __syncthreads();
-    }
-    wave_gemm(H[kGuideLayers - 1], kLd, Wt[kGuideLayers - 1], kLd, acc);
-    store_f32(acc, O);
-    __syncthreads();
-    // the loss gradient of each output
-    {
-        float o[kGuideOutputs], grad[kGuideOutputs];
-        for (int c = 0; c < kGuideOutputs; ++c) o[c] = O[lane * kOutLd + c];
-        const float w = live ? smp[kSamWeight] * *scale : 0.0f;
-        const float l = lobe_loss_gradient(o, smp + kSamIn, w, live ? smp[kSamBsdf] : -1.0f, g.share_cap, g.learn_select != 0, grad);
-        for (int c = 0; c < kGuideWidth; ++c) G[0][lane * kLd + c] = c < kGuideOutputs ? to_bf16(grad[c]) : (uint16_t)0;
-        if (l != 0) atomicAdd(loss, l);
-    }
-    __syncthreads();
-    // backward, last layer first: dW = H^T dZ, dH = dZ W^T masked by ReLU
-    int cur = 0;
-    for (int layer = kGuideLayers - 1; layer >= 0; --layer) {
-        wave_gemm_tn(H[layer], kLd, G[cur], kLd, acc);
-        float* P = partial + ((size_t)blockIdx.x * kGuideLayers + layer) * 4096;
-        for (int t = 0; t < 16; ++t)
-            for (int j = 0; j < 4; ++j) P[d_row(t, j) * 64 + d_col(t)] = acc[t][j];
-        wave_gemm(G[cur], kLd, Wn[layer], kLd, acc);
-        if (layer > 0) {
-            store_masked(acc, H[layer], G[cur ^ 1]);
-        } else {
-            store_f32(acc, O);
+    float* P = partial + (size_t)blockIdx.x * kGuideMlpParams;
+    const uint32_t tiles = (count + 63) / 64;
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const bool firstTile = tile == blockIdx.x;
+        const uint32_t b = tile * 64 + lane;
+        const bool live = b < count && n > 0;
+        const float* smp = samples + (size_t)(live ? (first + b) % n : 0) * kSamFloats;
+        {
+            float in[kGuideWidth];
+            if (live) guide_inputs(g, smp + kSamPoint, smp + kSamOut, smp + kSamNormal, smp[kSamNm], in);
+            for (int k = 0; k < kGuideWidth; ++k) H[0][lane * kLd + k] = live ? to_bf16(in[k]) : (uint16_t)0;
         }
         __syncthreads();
-        cur ^= 1;
-    }
-    // the grid's share: each level's two features to its eight corners
-    if (live) {
-        float p[3];
-        guide_position(g, smp + kSamPoint, p);
-        for (int level = 0; level < kGuideLevels; ++level) {
-            uint32_t idx[8];
-            float wt[8];
-            grid_corners(g, level, p, idx, wt);
-            const float d0 = O[lane * kOutLd + kInGrid + 2 * level], d1 = O[lane * kOutLd + kInGrid + 2 * level + 1];
-            float* table = grid_grad + (size_t)level * kGuideTable * kGuideFeatures;
-            for (int c = 0; c < 8; ++c) {
-                if (d0 != 0) atomicAdd(&table[(size_t)idx[c] * 2], wt[c] * d0);
-                if (d1 != 0) atomicAdd(&table[(size_t)idx[c] * 2 + 1], wt[c] * d1);
+        f32x4 acc[16];
+        for (int layer = 0; layer < kGuideLayers - 1; ++layer) {
+            wave_gemm(H[layer], kLd, Wt[layer], kLd, acc);
+            store_relu(acc, H[layer + 1]);
+            __syncthreads();
+        }
+        wave_gemm(H[kGuideLayers - 1], kLd, Wt[kGuideLayers - 1], kLd, acc);
+        store_f32(acc, O);
+        __syncthreads();
+        // the loss gradient of each output
+        {
+            float o[kGuideOutputs], grad[kGuideOutputs];
+            for (int c = 0; c < kGuideOutputs; ++c) o[c] = O[lane * kOutLd + c];
+            const float w = live ? smp[kSamWeight] * *scale : 0.0f;
+            const float l = lobe_loss_gradient(o, smp + kSamIn, w, live ? smp[kSamBsdf] : -1.0f, g.share_cap, g.learn_select != 0, grad);
+            for (int c = 0; c < kGuideWidth; ++c) G[0][lane * kLd + c] = c < kGuideOutputs ? to_bf16(grad[c]) : (uint16_t)0;
+            if (l != 0) atomicAdd(loss, l);
+        }
+        __syncthreads();
+        // backward, last layer first: dW = H^T dZ, dH = dZ W^T masked by ReLU
+        int cur = 0;
+        for (int layer = kGuideLayers - 1; layer >= 0; --layer) {
+            wave_gemm_tn(H[layer], kLd, G[cur], kLd, acc);
+            float* PW = P + layer * 4096;
+            for (int t = 0; t < 16; ++t)
+                for (int j = 0; j < 4; ++j) {
+                    float* e = PW + d_row(t, j) * 64 + d_col(t);
+                    *e = firstTile ? acc[t][j] : *e + acc[t][j];
+                }
+            wave_gemm(G[cur], kLd, Wn[layer], kLd, acc);
+            if (layer > 0) {
+                store_masked(acc, H[layer], G[cur ^ 1]);
+            } else {
+                store_f32(acc, O);
+            }
+            __syncthreads();
+            cur ^= 1;
+        }
+        // the grid's share: each level's two features to its eight corners
+        if (live) {
+            float p[3];
+            guide_position(g, smp + kSamPoint, p);
+            for (int level = 0; level < kGuideLevels; ++level) {
+                uint32_t idx[8];
+                float wt[8];
+                grid_corners(g, level, p, idx, wt);
+                const float d0 = O[lane * kOutLd + kInGrid + 2 * level], d1 = O[lane * kOutLd + kInGrid + 2 * level + 1];
+                float* table = grid_grad + (size_t)level * kGuideTable * kGuideFeatures;
+                for (int c = 0; c < 8; ++c) {
+                    if (d0 != 0) atomicAdd(&table[(size_t)idx[c] * 2], wt[c] * d0);
+                    if (d1 != 0) atomicAdd(&table[(size_t)idx[c] * 2 + 1], wt[c] * d1);
+                }
             }
         }
+        __syncthreads();
     }
 }
 
@@ -271,10 +294,13 @@ bool launchGuideInferPaths(const GuideView& g, const PassParams& P, uint32_t que
     return hipGetLastError() == hipSuccess;
 }
 
+uint32_t guideTrainPartials(uint32_t count) { return std::min<uint32_t>((count + 63) / 64, kTrainBlocks); }
+
 bool launchGuideTrain(const GuideView& g, const float* samples, uint32_t n, uint32_t first, uint32_t count, const float* scale,
                       float* partial, float* gridGrad, float* loss, hipStream_t st) {
     if (count == 0) return true;
-    hipLaunchKernelGGL(k_guide_train, dim3((count + 63) / 64), dim3(64), 0, st, g, samples, n, first, count, scale, partial, gridGrad, loss);
+    hipLaunchKernelGGL(k_guide_train, dim3(guideTrainPartials(count)), dim3(64), 0, st, g, samples, n, first, count, scale, partial,
+                       gridGrad, loss);
     return hipGetLastError() == hipSuccess;
 }
 

@@ -74,6 +74,8 @@ bool launchGuideInferPaths(const GuideView& g, const PassParams& P, uint32_t que
     return true;
 }
 
+uint32_t guideTrainPartials(uint32_t count) { return (count + 63) / 64; }  // one partial sum a tile here
+
 bool launchGuideTrain(const GuideView& g, const float* samples, uint32_t n, uint32_t first, uint32_t count, const float* scale,
                       float* partial, float* gridGrad, float* loss, hipStream_t) {
     const uint32_t tiles = (count + 63) / 64;
