@@ -242,11 +242,16 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideView g, const float* sa
         for (int layer = kGuideLayers - 1; layer >= 0; --layer) {
             wave_gemm_tn(H[layer], kLd, G[cur], kLd, acc);
             float* PW = P + layer * 4096;
+            if (!firstTile) {
+                // the block's sums so far, all loads issued before any store
+                float before[64];
+                for (int t = 0; t < 16; ++t)
+                    for (int j = 0; j < 4; ++j) before[t * 4 + j] = PW[d_row(t, j) * 64 + d_col(t)];
+                for (int t = 0; t < 16; ++t)
+                    for (int j = 0; j < 4; ++j) acc[t][j] += before[t * 4 + j];
+            }
             for (int t = 0; t < 16; ++t)
-                for (int j = 0; j < 4; ++j) {
-                    float* e = PW + d_row(t, j) * 64 + d_col(t);
-                    *e = firstTile ? acc[t][j] : *e + acc[t][j];
-                }
+                for (int j = 0; j < 4; ++j) PW[d_row(t, j) * 64 + d_col(t)] = acc[t][j];
             wave_gemm(G[cur], kLd, Wn[layer], kLd, acc);
             if (layer > 0) {
                 store_masked(acc, H[layer], G[cur ^ 1]);
