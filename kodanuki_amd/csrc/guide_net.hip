@@ -94,6 +94,39 @@ __device__ __forceinline__ void store_f32(const f32x4 acc[16], float* O) {
         for (int j = 0; j < 4; ++j) O[d_row(t, j) * kOutLd + d_col(t)] = acc[t][j];
 }
 
+// Adds (v0, v1) to base[2 idx], base[2 idx + 1] for the active lanes of
+// a wavefront (all 64 lanes call it). Lanes with the same entry are summed
+// first, a leader adding the sum, while the first remaining lane's entry
+// is shared by another; the rest add their own. The global atomics are
+// executed far from the CU: the nearby samples of a tile share the coarse
+// levels' entries, and one atomic a group instead of one a lane is what
+// keeps training from waiting on them.
+__device__ __forceinline__ void wave_scatter_add(float* base, uint32_t idx, float v0, float v1, bool active) {
+    const int lane = threadIdx.x & 63;
+    uint64_t pending = __ballot(active);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t a = __shfl(idx, leader);
+        const uint64_t same = __ballot(active && idx == a) & pending;
+        if (__popcll(same) < 2) break;
+        const bool in = (same >> lane) & 1;
+        float s0 = in ? v0 : 0.0f, s1 = in ? v1 : 0.0f;
+        for (int off = 32; off > 0; off >>= 1) {
+            s0 += __shfl_xor(s0, off);
+            s1 += __shfl_xor(s1, off);
+        }
+        if (lane == leader) {
+            atomicAdd(&base[(size_t)a * 2], s0);
+            atomicAdd(&base[(size_t)a * 2 + 1], s1);
+        }
+        pending &= ~same;
+    }
+    if ((pending >> lane) & 1) {
+        atomicAdd(&base[(size_t)idx * 2], v0);
+        atomicAdd(&base[(size_t)idx * 2 + 1], v1);
+    }
+}
+
 // The layers' BF16 weights into LDS, eight at a time: Wt[layer][out][in]
 // for the forward products, and (Wn not null) Wn[layer][in][out] for the
 // backward ones.
@@ -262,19 +295,17 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideView g, const float* sa
             cur ^= 1;
         }
         // the grid's share: each level's two features to its eight corners
-        if (live) {
-            float p[3];
-            guide_position(g, smp + kSamPoint, p);
+        {
+            float p[3] = {0, 0, 0};
+            if (live) guide_position(g, smp + kSamPoint, p);
             for (int level = 0; level < kGuideLevels; ++level) {
                 uint32_t idx[8];
                 float wt[8];
                 grid_corners(g, level, p, idx, wt);
                 const float d0 = O[lane * kOutLd + kInGrid + 2 * level], d1 = O[lane * kOutLd + kInGrid + 2 * level + 1];
+                const bool active = live && (d0 != 0 || d1 != 0);
                 float* table = grid_grad + (size_t)level * kGuideTable * kGuideFeatures;
-                for (int c = 0; c < 8; ++c) {
-                    if (d0 != 0) atomicAdd(&table[(size_t)idx[c] * 2], wt[c] * d0);
-                    if (d1 != 0) atomicAdd(&table[(size_t)idx[c] * 2 + 1], wt[c] * d1);
-                }
+                for (int c = 0; c < 8; ++c) wave_scatter_add(table, idx[c], wt[c] * d0, wt[c] * d1, active);
             }
         }
         __syncthreads();
