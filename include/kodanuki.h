@@ -349,6 +349,50 @@ int kdn_read_accumulation(kdn_ctx* ctx, double* rgb_sum, double* sq_sum, uint32_
 /* The mean image as linear RGB floats (width*height*3). */
 int kdn_read_image(kdn_ctx* ctx, float* rgb);
 
+/* ---- the denoiser ------------------------------------------------------------ */
+/*
+ * An edge-avoiding à-trous filter guided by the variance (SVGF-like), the
+ * reference's denoisedFrame (plugins/reconstruction/atrous/atrous.h:462,
+ * atrous.cpp:18-500) on the device, to its bits: the light divided by the
+ * albedo, a variance per pixel (from its history, or from the 7x7 pixels
+ * around it for histories under 4 samples), 4 passes of a 5x5 B3 spline
+ * at steps 1, 2, 4, 8 with normal, plane and luminance edge stops, the
+ * albedo multiplied back and the filtered light faded into the held light
+ * by 1 / (1 + length / 8).
+ */
+/* A pixel of the temporal history (TemporalPixel, temporal/temporal.h:18-70). */
+typedef struct kdn_denoise_pixel {
+    double colour[3];    /* its light (its lighting where demodulated) */
+    double moment;       /* the blend of the luminance squared */
+    float length;        /* samples the blend holds */
+    float distance;      /* from the lens; infinite on the sky */
+    double normal[3];    /* unit shading normal; zero on the sky */
+    uint32_t leaf;       /* KDN_NO_LEAF on the sky */
+    uint32_t pad;
+    double albedo[3];
+} kdn_denoise_pixel;
+
+/* The lens at the history's size (DenoiseCamera, atrous.h:386-404). */
+typedef struct kdn_denoise_frame {
+    int32_t width, height;
+    double forward[3], right[3], up[3];  /* unit, the lens's frame */
+    double reach;                        /* tan(field of view / 2) */
+    double aspect;                       /* width / height */
+    double centre_x, centre_y;           /* where in its pixel a ray passes (0.5) */
+    uint32_t demodulated;                /* the history holds lighting */
+    uint32_t pad;
+} kdn_denoise_frame;
+
+/* Denoise a frame: pixels[width * height] by rows from the top left ->
+ * rgb[width * height * 3] (host buffers). */
+int kdn_denoise(kdn_ctx* ctx, const kdn_denoise_frame* frame, const kdn_denoise_pixel* pixels, float* rgb);
+/* The same on device buffers, on a stream (NULL: the context's). */
+int kdn_denoise_device(kdn_ctx* ctx, const kdn_denoise_frame* frame, const void* pixels, void* rgb, void* stream);
+/* Denoise the context's accumulated image (kdn_render_pass): its FP64
+ * sums are the history (mean, moment, samples), a pass of primary rays
+ * through the pixel centres gives distance, normal, leaf and albedo. */
+int kdn_denoise_accumulation(kdn_ctx* ctx, float* rgb);
+
 /* ---- the learned guide ------------------------------------------------------ */
 /*
  * A network that learns, per scene, where light arrives from: a hash grid

@@ -119,6 +119,17 @@ class GuideInfo(C.Structure):
                 ("size", C.c_float * 3)]
 
 
+DENOISE_PIXEL_DTYPE = np.dtype([("colour", "<f8", 3), ("moment", "<f8"), ("length", "<f4"), ("distance", "<f4"),
+                                ("normal", "<f8", 3), ("leaf", "<u4"), ("pad", "<u4"), ("albedo", "<f8", 3)])
+assert DENOISE_PIXEL_DTYPE.itemsize == 96
+
+
+class DenoiseFrame(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("forward", C.c_double * 3), ("right", C.c_double * 3),
+                ("up", C.c_double * 3), ("reach", C.c_double), ("aspect", C.c_double), ("centre_x", C.c_double),
+                ("centre_y", C.c_double), ("demodulated", C.c_uint32), ("pad", C.c_uint32)]
+
+
 class GuideConfig(C.Structure):
     _fields_ = [("learning_rate", C.c_float), ("share", C.c_float), ("epochs", C.c_float), ("min_steps", C.c_uint32),
                 ("max_steps", C.c_uint32), ("max_resolution", C.c_uint32), ("train_paths", C.c_uint32),
@@ -175,6 +186,7 @@ EXPORTS = [
     "kdn_scene_region", "kdn_open", "kdn_close", "kdn_last_error", "kdn_global_error", "kdn_upload_scene",
     "kdn_build_octree", "kdn_octree_sizes", "kdn_octree_download", "kdn_extend", "kdn_extend_device",
     "kdn_render_pass", "kdn_read_accumulation", "kdn_read_image",
+    "kdn_denoise", "kdn_denoise_device", "kdn_denoise_accumulation",
     "kdn_guide_reset", "kdn_guide_configure", "kdn_guide_describe", "kdn_guide_params", "kdn_guide_set_params", "kdn_guide_infer",
     "kdn_guide_gradients", "kdn_guide_train",
     "kdn_serve_open", "kdn_serve_submit", "kdn_serve_step", "kdn_serve_status", "kdn_serve_result",
@@ -227,6 +239,9 @@ def _bind(lib):
         "kdn_render_pass": (C.c_int, [P, C.POINTER(PassDesc), C.POINTER(PassStats)]),
         "kdn_read_accumulation": (C.c_int, [P, P, P, P]),
         "kdn_read_image": (C.c_int, [P, P]),
+        "kdn_denoise": (C.c_int, [P, C.POINTER(DenoiseFrame), P, P]),
+        "kdn_denoise_device": (C.c_int, [P, C.POINTER(DenoiseFrame), P, P, P]),
+        "kdn_denoise_accumulation": (C.c_int, [P, P]),
         "kdn_guide_reset": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "kdn_guide_describe": (C.c_int, [P, C.POINTER(GuideInfo)]),
         "kdn_guide_configure": (C.c_int, [P, C.POINTER(GuideConfig)]),
@@ -430,6 +445,22 @@ class Device:
         n = np.zeros((height, width), np.uint32)
         _check(library().kdn_read_accumulation(self._h, _ptr(rgb), _ptr(sq), _ptr(n)), self._h)
         return rgb, sq, n
+
+    # ---- the denoiser ----
+    def denoise(self, pixels, forward, right, up, reach, centre=(0.5, 0.5), demodulated=False):
+        """pixels [height, width] of DENOISE_PIXEL_DTYPE -> rgb [height, width, 3]."""
+        px = np.ascontiguousarray(pixels, DENOISE_PIXEL_DTYPE)
+        h, w = px.shape
+        f = DenoiseFrame(w, h, (C.c_double * 3)(*forward), (C.c_double * 3)(*right), (C.c_double * 3)(*up), reach, w / h,
+                         centre[0], centre[1], 1 if demodulated else 0, 0)
+        rgb = np.zeros((h, w, 3), np.float32)
+        _check(library().kdn_denoise(self._h, C.byref(f), _ptr(px), _ptr(rgb)), self._h)
+        return rgb
+
+    def denoise_accumulation(self, width, height):
+        rgb = np.zeros((height, width, 3), np.float32)
+        _check(library().kdn_denoise_accumulation(self._h, _ptr(rgb)), self._h)
+        return rgb
 
     # ---- the learned guide ----
     def guide_reset(self, seed=1, lower=None, size=None):

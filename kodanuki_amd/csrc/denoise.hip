@@ -1,0 +1,254 @@
+// The à-trous denoiser on the device: one kernel a stage, one thread a
+// pixel (prepare, variance estimate, 4 filter passes, remodulate), and the
+// G-buffer pass that turns a context's accumulated image into the
+// denoiser's history. The per-pixel arithmetic is denoise_dev.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+
+#include "ctx.h"
+#include "denoise_dev.h"
+#include "integrator_kernels.h"
+
+namespace kdn {
+namespace {
+
+constexpr int kDenoiseBlock = 256;
+
+__global__ void k_dn_prepare(DenoiseBuffers B, DenoiseLensDev L, const kdn_denoise_pixel* pixels, uint32_t demodulated) {
+    const uint32_t at = blockIdx.x * blockDim.x + threadIdx.x;
+    if (at >= (uint32_t)(B.width * B.height)) return;
+    prepare_pixel(B, L, pixels[at], (int)(at % B.width), (int)(at / B.width), demodulated != 0);
+}
+
+__global__ void k_dn_estimate(DenoiseBuffers B, int src) {
+    const uint32_t at = blockIdx.x * blockDim.x + threadIdx.x;
+    if (at >= (uint32_t)(B.width * B.height)) return;
+    estimate_pixel(B, src, (int)(at % B.width), (int)(at / B.width));
+}
+
+__global__ void k_dn_filter(DenoiseBuffers B, int src, int step) {
+    const uint32_t at = blockIdx.x * blockDim.x + threadIdx.x;
+    if (at >= (uint32_t)(B.width * B.height)) return;
+    filter_pixel(B, src, step, (int)(at % B.width), (int)(at / B.width));
+}
+
+__global__ void k_dn_remodulate(DenoiseBuffers B, int src, const kdn_denoise_pixel* pixels, uint32_t demodulated, float* rgb) {
+    const uint32_t at = blockIdx.x * blockDim.x + threadIdx.x;
+    if (at >= (uint32_t)(B.width * B.height)) return;
+    remodulate_pixel(B, src, pixels[at], demodulated != 0, at, rgb);
+}
+
+// primary rays through the pixel centres of a lens
+__global__ void k_dn_rays(DenoiseLensDev L, int width, int height, float ex, float ey, float ez, kdn_ray* rays) {
+    const uint32_t at = blockIdx.x * blockDim.x + threadIdx.x;
+    if (at >= (uint32_t)(width * height)) return;
+    const int x = (int)(at % width), y = (int)(at / width);
+    double h[3];
+    for (int k = 0; k < 3; ++k) h[k] = L.corner[k] + L.across[k] * ((double)x + L.cx) + L.down[k] * ((double)y + L.cy);
+    const double len = __builtin_sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+    kdn_ray r;
+    r.origin[0] = ex;
+    r.origin[1] = ey;
+    r.origin[2] = ez;
+    for (int k = 0; k < 3; ++k) r.direction[k] = (float)(h[k] / len);
+    r.far_ = kInf;
+    r.inside = 0;
+    r.within = KDN_NO_MATERIAL;
+    rays[at] = r;
+}
+
+// the history of a pixel from its FP64 sums and its primary hit: the mean,
+// the luminance's second moment (the channels' spreads taken as fully
+// correlated: the sums hold squares per channel, not of the luminance),
+// the samples, and distance, facing normal, leaf and albedo of the hit
+__global__ void k_dn_history(const double* sum, const double* sq, const uint32_t* count, const kdn_hit* hits, const kdn_ray* rays,
+                             const uint32_t* leafMaterial, const MaterialDev* materials, uint32_t nMaterials, uint32_t n,
+                             kdn_denoise_pixel* out) {
+    const uint32_t at = blockIdx.x * blockDim.x + threadIdx.x;
+    if (at >= n) return;
+    kdn_denoise_pixel px;
+    const double k = count[at] > 0 ? (double)count[at] : 1.0;
+    double spread = 0;
+    const double w[3] = {0.2126, 0.7152, 0.0722};
+    for (int c = 0; c < 3; ++c) {
+        const double mean = sum[3 * (size_t)at + c] / k;
+        px.colour[c] = mean;
+        spread += w[c] * __builtin_sqrt(fmax_d(sq[3 * (size_t)at + c] / k - mean * mean, 0.0));
+    }
+    const double lum = luminance_of(px.colour[0], px.colour[1], px.colour[2]);
+    px.moment = lum * lum + spread * spread;
+    px.length = (float)count[at];
+    px.pad = 0;
+    const kdn_hit h = hits[at];
+    if (h.kind != KDN_HIT_SURFACE) {
+        px.distance = kInf;
+        px.leaf = KDN_NO_LEAF;
+        for (int c = 0; c < 3; ++c) px.normal[c] = px.albedo[c] = 0;
+    } else {
+        px.distance = h.distance;
+        px.leaf = h.leaf;
+        const kdn_ray& r = rays[at];
+        double nv[3] = {h.normal[0], h.normal[1], h.normal[2]};
+        if (nv[0] * r.direction[0] + nv[1] * r.direction[1] + nv[2] * r.direction[2] > 0) {
+            for (int c = 0; c < 3; ++c) nv[c] = -nv[c];
+        }
+        const double nl = __builtin_sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+        for (int c = 0; c < 3; ++c) px.normal[c] = nl > 0 ? nv[c] / nl : 0.0;
+        const uint32_t mi = h.leaf != KDN_NO_LEAF ? leafMaterial[h.leaf] : nMaterials - 1;
+        const MaterialDev& m = materials[mi < nMaterials ? mi : nMaterials - 1];
+        for (int c = 0; c < 3; ++c) px.albedo[c] = m.kind == KDN_GLASS ? 1.0 : (double)m.albedo[c];
+    }
+    out[at] = px;
+}
+
+DenoiseLensDev lensOf(const kdn_denoise_frame& f) {
+    DenoiseLensDev L;
+    const double sideways = f.reach * f.aspect;
+    for (int k = 0; k < 3; ++k) {
+        L.corner[k] = f.forward[k] - f.right[k] * sideways + f.up[k] * f.reach;
+        L.across[k] = f.right[k] * (2 * sideways / (double)f.width);
+        L.down[k] = f.up[k] * (-2 * f.reach / (double)f.height);
+    }
+    L.cx = f.centre_x;
+    L.cy = f.centre_y;
+    return L;
+}
+
+// The filter on device buffers: pixels in, rgb out, scratch from the
+// context (19 floats a pixel).
+bool denoiseDevice(Ctx& c, const kdn_denoise_frame& f, const kdn_denoise_pixel* pixels, float* rgb, hipStream_t st) {
+    const size_t n = (size_t)f.width * f.height;
+    if (!ensure(c.scratch[4], n * 19 * sizeof(float) + 256, c.error, false)) return false;
+    float* base = (float*)c.scratch[4].ptr;
+    DenoiseBuffers B;
+    B.width = f.width;
+    B.height = f.height;
+    float** fields[] = {&B.x, &B.y, &B.z, &B.nx, &B.ny, &B.nz, &B.offset, &B.inverseReach, &B.sky, &B.red[0], &B.green[0], &B.blue[0],
+                        &B.luminance[0], &B.variance[0], &B.red[1], &B.green[1], &B.blue[1], &B.luminance[1], &B.variance[1]};
+    for (int k = 0; k < 19; ++k) *fields[k] = base + (size_t)k * n;
+    const DenoiseLensDev L = lensOf(f);
+    const dim3 grid((uint32_t)((n + kDenoiseBlock - 1) / kDenoiseBlock)), block(kDenoiseBlock);
+    hipLaunchKernelGGL(k_dn_prepare, grid, block, 0, st, B, L, pixels, f.demodulated);
+    hipLaunchKernelGGL(k_dn_estimate, grid, block, 0, st, B, 0);
+    int src = 1;
+    for (int k = 0; k < kDenoisePasses; ++k) {
+        hipLaunchKernelGGL(k_dn_filter, grid, block, 0, st, B, src, 1 << k);
+        src ^= 1;
+    }
+    hipLaunchKernelGGL(k_dn_remodulate, grid, block, 0, st, B, src, pixels, f.demodulated, rgb);
+    KDN_HIP(hipGetLastError());
+    return true;
+}
+
+bool checkFrame(Ctx& c, const kdn_denoise_frame* f) {
+    if (!f || f->width <= 0 || f->height <= 0 || f->width > 16384 || f->height > 16384) {
+        c.error = "kdn_denoise: frame size out of 1..16384";
+        return false;
+    }
+    return true;
+}
+
+}  // namespace
+}  // namespace kdn
+
+using namespace kdn;
+
+extern "C" {
+
+int kdn_denoise_device(kdn_ctx* ctx, const kdn_denoise_frame* frame, const void* pixels, void* rgb, void* stream) {
+    if (!ctx || !pixels || !rgb) return KDN_E_ARGUMENT;
+    Ctx& c = ctx->c;
+    if (!checkFrame(c, frame)) return KDN_E_ARGUMENT;
+    hipSetDevice(c.device);
+    hipStream_t st = stream ? (hipStream_t)stream : c.stream;
+    return denoiseDevice(c, *frame, (const kdn_denoise_pixel*)pixels, (float*)rgb, st) ? KDN_OK : KDN_E_DEVICE;
+}
+
+int kdn_denoise(kdn_ctx* ctx, const kdn_denoise_frame* frame, const kdn_denoise_pixel* pixels, float* rgb) {
+    if (!ctx || !pixels || !rgb) return KDN_E_ARGUMENT;
+    Ctx& c = ctx->c;
+    if (!checkFrame(c, frame)) return KDN_E_ARGUMENT;
+    hipSetDevice(c.device);
+    const size_t n = (size_t)frame->width * frame->height;
+    if (!ensure(c.scratch[5], n * (sizeof(kdn_denoise_pixel) + 3 * sizeof(float)) + 256, c.error, false)) return KDN_E_MEMORY;
+    kdn_denoise_pixel* dp = (kdn_denoise_pixel*)c.scratch[5].ptr;
+    float* drgb = (float*)(dp + n);
+    if (hipMemcpyAsync(dp, pixels, n * sizeof(kdn_denoise_pixel), hipMemcpyHostToDevice, c.stream) != hipSuccess ||
+        !denoiseDevice(c, *frame, dp, drgb, c.stream) ||
+        hipMemcpyAsync(rgb, drgb, n * 3 * sizeof(float), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+        hipStreamSynchronize(c.stream) != hipSuccess) {
+        if (c.error.empty()) c.error = "kdn_denoise: device call failed";
+        return KDN_E_DEVICE;
+    }
+    return KDN_OK;
+}
+
+int kdn_denoise_accumulation(kdn_ctx* ctx, float* rgb) {
+    if (!ctx || !rgb) return KDN_E_ARGUMENT;
+    Ctx& c = ctx->c;
+    if (!c.accum || !c.tree.built || !c.integrator_ready) {
+        c.error = "kdn_denoise_accumulation: no accumulated image (kdn_render_pass first)";
+        return KDN_E_STATE;
+    }
+    hipSetDevice(c.device);
+    hipStream_t st = c.stream;
+    const int W = c.width, H = c.height;
+    const size_t n = (size_t)W * H;
+    // the lens of the scene's camera (core/camera.cpp:9-25)
+    const kdn_camera& cam = c.scene.camera;
+    double fwd[3], rt[3], upv[3], len = 0;
+    for (int k = 0; k < 3; ++k) {
+        fwd[k] = (double)cam.target[k] - cam.eye[k];
+        len += fwd[k] * fwd[k];
+    }
+    len = std::sqrt(len);
+    for (int k = 0; k < 3; ++k) fwd[k] /= len;
+    rt[0] = fwd[1] * cam.up[2] - fwd[2] * cam.up[1];
+    rt[1] = fwd[2] * cam.up[0] - fwd[0] * cam.up[2];
+    rt[2] = fwd[0] * cam.up[1] - fwd[1] * cam.up[0];
+    len = std::sqrt(rt[0] * rt[0] + rt[1] * rt[1] + rt[2] * rt[2]);
+    for (int k = 0; k < 3; ++k) rt[k] /= len;
+    upv[0] = rt[1] * fwd[2] - rt[2] * fwd[1];
+    upv[1] = rt[2] * fwd[0] - rt[0] * fwd[2];
+    upv[2] = rt[0] * fwd[1] - rt[1] * fwd[0];
+    kdn_denoise_frame f{};
+    f.width = W;
+    f.height = H;
+    for (int k = 0; k < 3; ++k) {
+        f.forward[k] = fwd[k];
+        f.right[k] = rt[k];
+        f.up[k] = upv[k];
+    }
+    f.reach = std::tan((double)cam.fov / 2);
+    f.aspect = (double)W / H;
+    f.centre_x = f.centre_y = 0.5;
+    f.demodulated = 0;
+    const size_t bytes = n * (sizeof(kdn_ray) + sizeof(kdn_hit) + sizeof(kdn_denoise_pixel) + 3 * sizeof(float)) + 1024;
+    if (!ensure(c.scratch[5], bytes, c.error, false)) return KDN_E_MEMORY;
+    char* at = (char*)c.scratch[5].ptr;
+    kdn_denoise_pixel* px = (kdn_denoise_pixel*)at;
+    at += (n * sizeof(kdn_denoise_pixel) + 255) & ~size_t(255);
+    kdn_ray* rays = (kdn_ray*)at;
+    at += (n * sizeof(kdn_ray) + 255) & ~size_t(255);
+    kdn_hit* hits = (kdn_hit*)at;
+    at += (n * sizeof(kdn_hit) + 255) & ~size_t(255);
+    float* drgb = (float*)at;
+    const DenoiseLensDev L = lensOf(f);
+    const dim3 grid((uint32_t)((n + kDenoiseBlock - 1) / kDenoiseBlock)), block(kDenoiseBlock);
+    hipLaunchKernelGGL(k_dn_rays, grid, block, 0, st, L, W, H, cam.eye[0], cam.eye[1], cam.eye[2], rays);
+    if (!extendRays(c, rays, (uint32_t)n, hits, st)) return KDN_E_DEVICE;
+    hipLaunchKernelGGL(k_dn_history, grid, block, 0, st, (const double*)c.accum, (const double*)c.accum_sq, (const uint32_t*)c.accum_n,
+                       (const kdn_hit*)hits, (const kdn_ray*)rays, (const uint32_t*)c.scene.leaf_material,
+                       (const MaterialDev*)c.materials_dev, c.n_materials_dev, (uint32_t)n, px);
+    if (!denoiseDevice(c, f, px, drgb, st) ||
+        hipMemcpyAsync(rgb, drgb, n * 3 * sizeof(float), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        if (c.error.empty()) c.error = "kdn_denoise_accumulation: device call failed";
+        return KDN_E_DEVICE;
+    }
+    return KDN_OK;
+}
+
+}  // extern "C"
