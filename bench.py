@@ -325,6 +325,31 @@ def main():
                   guided_leg(k, dev, "window room (sky through a ceiling window)", scenes.window_room(W, H), 6, W, H, S,
                              args.guide_passes)]
 
+    # the denoiser on the 1920x1080 history of the timed steps (their FP64
+    # sums): a G-buffer pass of primary rays, then the filter
+    dev.upload(scene)
+    dev.build_octree(lower, size, args.depth)
+    for p in range(2):
+        dev.render_pass(W, H, S, first_sample=p * S, seed=rs.seed, surfaces=surfaces, keep=p > 0)
+    times = []
+    for _ in range(5):
+        _, dst = dev.denoise_accumulation(W, H, stats=True)
+        times.append((dst.gbuffer_ms, dst.filter_ms))
+    gb, fl = (sorted(t[i] for t in times)[len(times) // 2] for i in range(2))
+    # unique bytes a pixel: the history in (96) and guides and light out
+    # (14 floats), the estimate (read 12, write 5 floats), each filter pass
+    # (read 14, write 5 floats), remodulation (read 5 floats, the history,
+    # write 3 floats)
+    px_bytes = 96 + 14 * 4 + 17 * 4 + 4 * 19 * 4 + 5 * 4 + 96 + 12
+    achieved = W * H * px_bytes / (fl * 1e-3) / 1e9
+    denoise = {"workload": f"{W}x{H} history of the production scene at {2 * S} spp (its FP64 sums), the reference's "
+                           f"à-trous filter: variance estimate, 4 passes of 5x5 B3 taps, remodulation",
+               "filter_ms": fl, "gbuffer_ms": gb,
+               "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
+                            "bytes_per_pixel": px_bytes}}
+    print(f"[denoise] filter {fl:.2f} ms, G-buffer {gb:.2f} ms ({achieved:.0f} GB/s of unique bytes)", file=sys.stderr,
+          flush=True)
+
     serving = None
     if not args.no_serve:
         serving = serving_stream(k, scenes, dev, args.serve_jobs, args.serve_rate, args.serve_seed, args.serve_paths,
@@ -353,6 +378,7 @@ def main():
                    "method": "plain", "segments_per_step": first_stats.segments, "shadow_rays_per_step": first_stats.shadow_rays},
         "roofline": roofline,
         "guided": guided,
+        "denoise": denoise,
         "serving": serving,
         "cpu_baseline": cpu,
     }

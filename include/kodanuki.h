@@ -388,10 +388,15 @@ typedef struct kdn_denoise_frame {
 int kdn_denoise(kdn_ctx* ctx, const kdn_denoise_frame* frame, const kdn_denoise_pixel* pixels, float* rgb);
 /* The same on device buffers, on a stream (NULL: the context's). */
 int kdn_denoise_device(kdn_ctx* ctx, const kdn_denoise_frame* frame, const void* pixels, void* rgb, void* stream);
+typedef struct kdn_denoise_stats {
+    double gbuffer_ms;   /* primary rays and the history, HIP events */
+    double filter_ms;    /* the filter's kernels */
+} kdn_denoise_stats;
 /* Denoise the context's accumulated image (kdn_render_pass): its FP64
  * sums are the history (mean, moment, samples), a pass of primary rays
- * through the pixel centres gives distance, normal, leaf and albedo. */
-int kdn_denoise_accumulation(kdn_ctx* ctx, float* rgb);
+ * through the pixel centres gives distance, normal, leaf and albedo.
+ * stats may be NULL. */
+int kdn_denoise_accumulation(kdn_ctx* ctx, float* rgb, kdn_denoise_stats* stats);
 
 /* ---- the learned guide ------------------------------------------------------ */
 /*

@@ -130,6 +130,10 @@ class DenoiseFrame(C.Structure):
                 ("centre_y", C.c_double), ("demodulated", C.c_uint32), ("pad", C.c_uint32)]
 
 
+class DenoiseStats(C.Structure):
+    _fields_ = [("gbuffer_ms", C.c_double), ("filter_ms", C.c_double)]
+
+
 class GuideConfig(C.Structure):
     _fields_ = [("learning_rate", C.c_float), ("share", C.c_float), ("epochs", C.c_float), ("min_steps", C.c_uint32),
                 ("max_steps", C.c_uint32), ("max_resolution", C.c_uint32), ("train_paths", C.c_uint32),
@@ -241,7 +245,7 @@ def _bind(lib):
         "kdn_read_image": (C.c_int, [P, P]),
         "kdn_denoise": (C.c_int, [P, C.POINTER(DenoiseFrame), P, P]),
         "kdn_denoise_device": (C.c_int, [P, C.POINTER(DenoiseFrame), P, P, P]),
-        "kdn_denoise_accumulation": (C.c_int, [P, P]),
+        "kdn_denoise_accumulation": (C.c_int, [P, P, C.POINTER(DenoiseStats)]),
         "kdn_guide_reset": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "kdn_guide_describe": (C.c_int, [P, C.POINTER(GuideInfo)]),
         "kdn_guide_configure": (C.c_int, [P, C.POINTER(GuideConfig)]),
@@ -457,10 +461,11 @@ class Device:
         _check(library().kdn_denoise(self._h, C.byref(f), _ptr(px), _ptr(rgb)), self._h)
         return rgb
 
-    def denoise_accumulation(self, width, height):
+    def denoise_accumulation(self, width, height, stats=False):
         rgb = np.zeros((height, width, 3), np.float32)
-        _check(library().kdn_denoise_accumulation(self._h, _ptr(rgb)), self._h)
-        return rgb
+        st = DenoiseStats()
+        _check(library().kdn_denoise_accumulation(self._h, _ptr(rgb), C.byref(st) if stats else None), self._h)
+        return (rgb, st) if stats else rgb
 
     # ---- the learned guide ----
     def guide_reset(self, seed=1, lower=None, size=None):

@@ -185,7 +185,7 @@ int kdn_denoise(kdn_ctx* ctx, const kdn_denoise_frame* frame, const kdn_denoise_
     return KDN_OK;
 }
 
-int kdn_denoise_accumulation(kdn_ctx* ctx, float* rgb) {
+int kdn_denoise_accumulation(kdn_ctx* ctx, float* rgb, kdn_denoise_stats* stats) {
     if (!ctx || !rgb) return KDN_E_ARGUMENT;
     Ctx& c = ctx->c;
     if (!c.accum || !c.tree.built || !c.integrator_ready) {
@@ -237,12 +237,31 @@ int kdn_denoise_accumulation(kdn_ctx* ctx, float* rgb) {
     float* drgb = (float*)at;
     const DenoiseLensDev L = lensOf(f);
     const dim3 grid((uint32_t)((n + kDenoiseBlock - 1) / kDenoiseBlock)), block(kDenoiseBlock);
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    if (stats) {
+        for (hipEvent_t& e : ev) {
+            if (hipEventCreate(&e) != hipSuccess) return KDN_E_DEVICE;
+        }
+        hipEventRecord(ev[0], st);
+    }
     hipLaunchKernelGGL(k_dn_rays, grid, block, 0, st, L, W, H, cam.eye[0], cam.eye[1], cam.eye[2], rays);
     if (!extendRays(c, rays, (uint32_t)n, hits, st)) return KDN_E_DEVICE;
     hipLaunchKernelGGL(k_dn_history, grid, block, 0, st, (const double*)c.accum, (const double*)c.accum_sq, (const uint32_t*)c.accum_n,
                        (const kdn_hit*)hits, (const kdn_ray*)rays, (const uint32_t*)c.scene.leaf_material,
                        (const MaterialDev*)c.materials_dev, c.n_materials_dev, (uint32_t)n, px);
-    if (!denoiseDevice(c, f, px, drgb, st) ||
+    if (stats) hipEventRecord(ev[1], st);
+    const bool filtered = denoiseDevice(c, f, px, drgb, st);
+    if (stats) {
+        hipEventRecord(ev[2], st);
+        hipEventSynchronize(ev[2]);
+        float a = 0, b = 0;
+        hipEventElapsedTime(&a, ev[0], ev[1]);
+        hipEventElapsedTime(&b, ev[1], ev[2]);
+        stats->gbuffer_ms = a;
+        stats->filter_ms = b;
+        for (hipEvent_t e : ev) hipEventDestroy(e);
+    }
+    if (!filtered ||
         hipMemcpyAsync(rgb, drgb, n * 3 * sizeof(float), hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess) {
         if (c.error.empty()) c.error = "kdn_denoise_accumulation: device call failed";
