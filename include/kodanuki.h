@@ -315,7 +315,11 @@ typedef struct kdn_pass_desc {
                                bit 4: the learned guide: sample directions
                                       with it (one-sample MIS with the
                                       BSDF) once it has trained, and train
-                                      it on the pass's paths */
+                                      it on the pass's paths;
+                               bit 5: the plain guide: the first vertex
+                                      of each path samples with its
+                                      pixel's guide (kdn_highlight_guides
+                                      made for this image; not with bit 4) */
 } kdn_pass_desc;
 
 typedef struct kdn_pass_stats {
@@ -569,6 +573,43 @@ const char* kdn_serve_error(const kdn_server* server);
 /* Why a job failed ("" for one that did not). */
 const char* kdn_serve_job_error(const kdn_server* server, uint64_t id);
 void kdn_serve_close(kdn_server* server);
+
+/* ---- The plain (highlight) guide ---------------------------------------
+ * Replaces the reference's sampler/highlight_guide plugin: its frame hook
+ * guideSurveyPrepared (plugins/sampler/highlight_guide/plugin.cpp:13-28)
+ * that surveys the frame's highlights (surveyOf, survey.h:59) and makes
+ * every pixel's guide (pixelGuides, survey.h:64), and the mixture that
+ * samples the first vertex with it (mixtureDirection / mixtureDensity,
+ * mixture.h). A pixel's packed guide is KDN_HG_FLOATS floats: count,
+ * material, share, unused, then per lump axis (3), across (3), point (3),
+ * mean (2), factor (3) and weight. */
+#define KDN_HG_FLOATS 184
+
+typedef struct kdn_hguide_stats {
+    double ms;               /* survey and pixel guides, HIP events */
+    uint32_t survey_points;  /* every eighth pixel each way, from the fourth */
+    uint32_t highlights;     /* highlights the survey kept, over its points */
+    uint32_t guided_pixels;  /* pixels with at least one lump */
+    uint32_t pad;
+    double mean_lumps;       /* lumps per guided pixel */
+    double mean_share;       /* the draws' share the guide takes, per guided pixel */
+} kdn_hguide_stats;
+
+/* Survey the uploaded scene's highlights as the camera sees them at
+ * width x height and make every pixel's guide (kept on the device for
+ * kdn_render_pass flag bit 5; the scene's octree must be built). */
+int kdn_highlight_guides(kdn_ctx* ctx, int32_t width, int32_t height, kdn_hguide_stats* stats);
+/* Pixel (x, y)'s packed guide (KDN_HG_FLOATS floats). */
+int kdn_highlight_guide_at(kdn_ctx* ctx, int32_t x, int32_t y, float* packed);
+/* A packed guide (1 to 12 lumps of non-negative weight, not all zero) at a
+ * vertex: the lumps' density of count directions (3 floats each) as the
+ * mixture reads it (lumpDensity, guide.cpp:227-237: zero unless both
+ * directions leave the side the normal faces), and count directions
+ * drawn from the lumps by (pick, u, v) triples (drawnFromLump,
+ * guide.cpp:239-252). */
+int kdn_highlight_density(kdn_ctx* ctx, const float* packed, const float place[3], const float normal[3], const float out[3],
+                          const float* dirs, uint32_t count, float* density);
+int kdn_highlight_draw(kdn_ctx* ctx, const float* packed, const float place[3], const float* numbers, uint32_t count, float* drawn);
 
 #ifdef __cplusplus
 }

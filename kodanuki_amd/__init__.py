@@ -107,7 +107,7 @@ class PassStats(C.Structure):
                 ("guide_steps", C.c_uint32)]
 
 
-PASS_KEEP, PASS_TIME, PASS_COUNT, PASS_NO_FINISH, PASS_GUIDE = 1, 2, 4, 8, 16
+PASS_KEEP, PASS_TIME, PASS_COUNT, PASS_NO_FINISH, PASS_GUIDE, PASS_HIGHLIGHT = 1, 2, 4, 8, 16, 32
 
 
 class GuideInfo(C.Structure):
@@ -132,6 +132,16 @@ class DenoiseFrame(C.Structure):
 
 class DenoiseStats(C.Structure):
     _fields_ = [("gbuffer_ms", C.c_double), ("filter_ms", C.c_double)]
+
+
+# a pixel's packed plain guide: count, material, share, unused, then 12
+# lumps of axis (3), across (3), point (3), mean (2), factor (3), weight
+HG_FLOATS, HG_HEAD, HG_LUMP_FLOATS, HG_LUMPS = 184, 4, 15, 12
+
+
+class HighlightStats(C.Structure):
+    _fields_ = [("ms", C.c_double), ("survey_points", C.c_uint32), ("highlights", C.c_uint32),
+                ("guided_pixels", C.c_uint32), ("pad", C.c_uint32), ("mean_lumps", C.c_double), ("mean_share", C.c_double)]
 
 
 class GuideConfig(C.Structure):
@@ -195,6 +205,7 @@ EXPORTS = [
     "kdn_guide_gradients", "kdn_guide_train",
     "kdn_serve_open", "kdn_serve_submit", "kdn_serve_step", "kdn_serve_status", "kdn_serve_result",
     "kdn_serve_release", "kdn_serve_error", "kdn_serve_job_error", "kdn_serve_close",
+    "kdn_highlight_guides", "kdn_highlight_guide_at", "kdn_highlight_density", "kdn_highlight_draw",
 ]
 
 _lib = None
@@ -246,6 +257,10 @@ def _bind(lib):
         "kdn_denoise": (C.c_int, [P, C.POINTER(DenoiseFrame), P, P]),
         "kdn_denoise_device": (C.c_int, [P, C.POINTER(DenoiseFrame), P, P, P]),
         "kdn_denoise_accumulation": (C.c_int, [P, P, C.POINTER(DenoiseStats)]),
+        "kdn_highlight_guides": (C.c_int, [P, C.c_int32, C.c_int32, C.POINTER(HighlightStats)]),
+        "kdn_highlight_guide_at": (C.c_int, [P, C.c_int32, C.c_int32, P]),
+        "kdn_highlight_density": (C.c_int, [P, P, P, P, P, P, C.c_uint32, P]),
+        "kdn_highlight_draw": (C.c_int, [P, P, P, P, C.c_uint32, P]),
         "kdn_guide_reset": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "kdn_guide_describe": (C.c_int, [P, C.POINTER(GuideInfo)]),
         "kdn_guide_configure": (C.c_int, [P, C.POINTER(GuideConfig)]),
@@ -466,6 +481,39 @@ class Device:
         st = DenoiseStats()
         _check(library().kdn_denoise_accumulation(self._h, _ptr(rgb), C.byref(st) if stats else None), self._h)
         return (rgb, st) if stats else rgb
+
+    # ---- the plain (highlight) guide ----
+    def highlight_guides(self, width, height):
+        """Survey the scene's highlights and make every pixel's guide (for
+        render_pass(flags=PASS_HIGHLIGHT) at this size)."""
+        st = HighlightStats()
+        _check(library().kdn_highlight_guides(self._h, width, height, C.byref(st)), self._h)
+        return st
+
+    def highlight_guide_at(self, x, y):
+        """Pixel (x, y)'s packed guide (HG_FLOATS float32)."""
+        g = np.zeros(HG_FLOATS, np.float32)
+        _check(library().kdn_highlight_guide_at(self._h, x, y, _ptr(g)), self._h)
+        return g
+
+    def highlight_density(self, packed, place, normal, out, dirs):
+        """The lumps' density of directions [n, 3] at a vertex."""
+        g = np.ascontiguousarray(packed, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        v = [np.ascontiguousarray(a, np.float32) for a in (place, normal, out)]
+        r = np.zeros(len(d), np.float32)
+        _check(library().kdn_highlight_density(self._h, _ptr(g), _ptr(v[0]), _ptr(v[1]), _ptr(v[2]), _ptr(d), len(d), _ptr(r)),
+               self._h)
+        return r
+
+    def highlight_draw(self, packed, place, numbers):
+        """Directions [n, 3] drawn from the lumps by (pick, u, v) [n, 3]."""
+        g = np.ascontiguousarray(packed, np.float32)
+        q = np.ascontiguousarray(numbers, np.float32).reshape(-1, 3)
+        x = np.ascontiguousarray(place, np.float32)
+        r = np.zeros((len(q), 3), np.float32)
+        _check(library().kdn_highlight_draw(self._h, _ptr(g), _ptr(x), _ptr(q), len(q), _ptr(r)), self._h)
+        return r
 
     # ---- the learned guide ----
     def guide_reset(self, seed=1, lower=None, size=None):

@@ -203,7 +203,7 @@ uint64_t residentBytes(const Ctx& c) {
                      (uint64_t)s.n_leaves * 4;
     bytes += c.tree.nodes.bytes + c.tree.entries.bytes + c.tree.pool.bytes + c.tree.spool.bytes;
     for (const DevBuf& b : c.scratch) bytes += b.bytes;
-    bytes += c.paths_bytes + c.round_bytes;
+    bytes += c.paths_bytes + c.round_bytes + c.hguides.bytes;
     if (c.accum) bytes += (uint64_t)c.width * c.height * (6 * sizeof(double) + sizeof(uint32_t));
     return bytes;
 }
@@ -233,6 +233,8 @@ void releaseAll(Ctx& c) {
     c.accum_n = nullptr;
     freeIntegrator(c);
     freeGuide(c);
+    freeBuf(c.hguides);
+    c.hguide_width = c.hguide_height = 0;
     if (c.materials_dev) hipFree(c.materials_dev);
     if (c.lamps_dev) hipFree(c.lamps_dev);
     c.materials_dev = c.lamps_dev = nullptr;
@@ -432,7 +434,8 @@ int kdn_upload_scene(kdn_ctx* ctx, const kdn_scene* s) {
     if (d.leaf_material) hipFree(d.leaf_material);
     d = kdn::SceneDev{};
     c.tree.built = false;
-    kdn::freeGuide(c);  // the guide learns one scene
+    kdn::freeGuide(c);  // the guides belong to one scene
+    c.hguide_width = c.hguide_height = 0;
     std::vector<kdn_material> mats = s->scene.materials;
     const uint32_t fallback = (uint32_t)mats.size();
     kdn_material grey{};

@@ -75,6 +75,8 @@ struct Ctx {
     size_t round_bytes = 0;
     bool scene_has_glass = false;   // a leaf is painted with a glass
     void* guide = nullptr;          // the learned guide (GuideState), made on first use
+    DevBuf hguides;                 // the plain guide: a packed guide per pixel
+    int hguide_width = 0, hguide_height = 0;  // the image they were made for (0: none)
 };
 
 // Glass scattering is held off the device: renders with glass faulted the
@@ -102,6 +104,9 @@ void freeIntegrator(Ctx& c);
 bool guideReset(Ctx& c, uint32_t seed, const float lower[3], const float size[3]);
 bool guideReady(const Ctx& c);
 void freeGuide(Ctx& c);
+
+// hguide.hip: the plain guide of every pixel of a width x height image
+bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats);
 
 // scan.hip: exclusive prefix sum of n u32 values into out, returns the total
 bool scanU32(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* total, DevBuf& tmp, hipStream_t s, std::string& err);

@@ -267,6 +267,14 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
         KDN_HIP(hipMemsetAsync(work, 0, 32, st));
     }
 
+    if (flags & 32) {
+        // the plain guide: one job, of the image the guides were made for
+        if (guiding || jobs.size() != 1 || c.hguide_width != jobs[0].width || c.hguide_height != jobs[0].height) {
+            c.error = "kdn_render_pass: no plain guide for this image (kdn_highlight_guides first; not with the learned guide)";
+            return false;
+        }
+        P.hg = (const float*)c.hguides.ptr;
+    }
     if (guiding) {
         if (!guideReady(c) && !guideReset(c, 1, c.tree.lower, c.tree.size)) return false;
         if (!guideBeginRound(c, P, total)) return false;

@@ -4,6 +4,7 @@
 #pragma once
 
 #include "guide.h"
+#include "hguide.h"
 #include "integrator_dev.h"
 
 namespace kdn {
@@ -74,6 +75,9 @@ struct PassParams {
     const float* lobes;
     float* recs;
     uint32_t train_stride;
+    // the plain guide: every pixel's packed guide of Gaussian lumps
+    // (kHgFloats each, pixel-major; null: none), steering the first vertex
+    const float* hg;
 };
 
 KDN_D const JobDev& job_of(const PassParams& P, uint32_t i) { return P.jobs[P.slices[P.s.slice[i]].job]; }
@@ -259,6 +263,39 @@ KDN_D Scatter guided_scatter(const MaterialDev& m, const Lanes& w, V3 n, V3 out,
     return s;
 }
 
+// The plain guide's mixture at the first vertex (mixtureDirection,
+// mixture.cpp:57-116): with chance share a direction from the lumps, else
+// the material's; either way its density is the mixture of both.
+KDN_D Scatter highlight_scatter(const MaterialDev& m, const Lanes& w, V3 n, V3 out, const float* g, V3 place, float share, float uPart,
+                                float uPick, float uLobe, float u, float v) {
+    Scatter s;
+    s.found = false;
+    s.exact = false;
+    s.through = false;
+    s.collapse = false;
+    if (dot(n, out) <= 0) return s;
+    V3 in;
+    if (uPart < share) {
+        in = hg_draw(g, place, uPick, u, v);
+        if (!__builtin_isfinite(in.x + in.y + in.z)) return s;
+    } else {
+        const Scatter b = rough_scatter(m, w, n, out, uLobe, u, v);
+        if (!b.found) return s;
+        in = b.in;
+    }
+    const float arriving = dot(n, in);
+    if (arriving <= 0) return s;
+    const float density = share * hg_density(g, place, n, out, in) + (1 - share) * rough_density(m, n, out, in);
+    if (!(density > 0)) return s;
+    float f[kLanes];
+    rough_value(m, w, n, out, in, f);
+    s.found = true;
+    s.in = in;
+    s.density = density;
+    for (int l = 0; l < kLanes; ++l) s.weight[l] = f[l] * (arriving / density);
+    return s;
+}
+
 KDN_D float lift_at(V3 x) {
     return 8e-5f + 8e-6f * fmax_(fmax_(__builtin_fabsf(x.x), __builtin_fabsf(x.y)), __builtin_fabsf(x.z));
 }
@@ -344,6 +381,14 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
         GuideLobes lobes;
         if (guided) load_lobes(P.lobes, N, i, lobes);
         const float share = guided ? lobes.select : 0.0f;
+        // the pixel's plain guide steers the first vertex on its material
+        // (steersPlace, guide.cpp:162-164)
+        const float* hg = nullptr;
+        if (P.hg != nullptr && depth == 0 && !inside && s.pixel[i] != 0xFFFFFFFFu) {
+            const float* g = P.hg + (size_t)s.pixel[i] * kHgFloats;
+            if (g[0] > 0 && (uint32_t)g[1] == mi && m.kind != KDN_GLASS) hg = g;
+        }
+        const float hgShare = hg ? hg[2] : 0.0f;
         // aim at one lamp (aimedFrom, trace.cpp:896-947), MIS against meeting
         if (!exact && J.n_lamps > 0) {
             const float uPick = number(stream, sample, depth, Pick);
@@ -363,7 +408,8 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
                 spectrum_of(L.radiance, w, Le);
                 const float pdfL = L.pick * dist * dist / (cosL * L.area);
                 const float pdfS = guided ? share * mixture_pdf(lobes, in) + (1 - share) * rough_density(m, n, out, in)
-                                          : rough_density(m, n, out, in);
+                                   : hg     ? hgShare * hg_density(hg, x, n, out, in) + (1 - hgShare) * rough_density(m, n, out, in)
+                                            : rough_density(m, n, out, in);
                 const float wt = power_weight(pdfL, pdfS);
                 float c[kLanes];
                 float any = 0;
@@ -396,6 +442,8 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
             const Scatter sc = m.kind == KDN_GLASS ? glass_scatter(m, w, n, out, inside, uSide)
                                : guided ? guided_scatter(m, w, n, out, lobes, share, guide_number(stream, sample, depth, 0),
                                                          guide_number(stream, sample, depth, 1), uLobe, du, dv)
+                               : hg     ? highlight_scatter(m, w, n, out, hg, x, hgShare, guide_number(stream, sample, depth, 0),
+                                                            guide_number(stream, sample, depth, 1), uLobe, du, dv)
                                         : rough_scatter(m, w, n, out, uLobe, du, dv);
             if (sc.found) {
                 const float side = dot(sc.in, n);

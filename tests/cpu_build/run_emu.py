@@ -129,6 +129,22 @@ def serving(dev):
             assert same
 
 
+def highlight(dev):
+    """The plain guide's survey and pixel guides of the highlight room, and
+    a guided render beside a plain one."""
+    s = k.Scene(scenes.highlight_room())
+    lower, size = s.region(5)
+    dev.upload(s)
+    dev.build_octree(lower, size, 5)
+    st = dev.highlight_guides(40, 30)
+    say(f"[highlight] points {st.survey_points} highlights {st.highlights} guided pixels {st.guided_pixels} "
+        f"lumps {st.mean_lumps:.2f} share {st.mean_share:.3f}")
+    g = dev.highlight_guide_at(30, 10)
+    say(f"[highlight] pixel (30, 10): count {g[0]:.0f} material {g[1]:.0f} share {g[2]:.3f}")
+    a = dev.render_pass(40, 30, 4, seed=1, surfaces=5, flags=k.PASS_HIGHLIGHT)
+    say(f"[highlight] guided pass segments {a.segments}")
+
+
 def main():
     quick = "--quick" in sys.argv
     O.PATH = os.path.join(HERE, "liboracle_asan.so")
@@ -139,6 +155,7 @@ def main():
     integrator(dev, quick)
     production(dev, quick)
     serving(dev)
+    highlight(dev)
     dev.close()
     say("[emu] done")
 

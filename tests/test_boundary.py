@@ -75,7 +75,7 @@ def test_struct_layouts_match_the_header(tmp_path):
              "kdn_node_record": k.NodeRecord, "kdn_entry_record": k.EntryRecord, "kdn_pass_desc": k.PassDesc,
              "kdn_pass_stats": k.PassStats, "kdn_job_desc": k.JobDesc, "kdn_job_status": k.JobStatus,
              "kdn_serve_stats": k.ServeStats, "kdn_guide_info": k.GuideInfo, "kdn_denoise_frame": k.DenoiseFrame, "kdn_denoise_stats": k.DenoiseStats, "kdn_guide_config": k.GuideConfig, "kdn_guide_query": k.GuideQuery,
-             "kdn_guide_sample": k.GuideSample, "kdn_guide_lobe": k.GuideLobe}
+             "kdn_guide_sample": k.GuideSample, "kdn_guide_lobe": k.GuideLobe, "kdn_hguide_stats": k.HighlightStats}
     src = tmp_path / "sizes.c"
     src.write_text('#include <stdio.h>\n#include "kodanuki.h"\nint main(void) {\n' +
                    "".join(f'    printf("{n} %zu\\n", sizeof({n}));\n' for n in names) + "    return 0;\n}\n")
