@@ -141,8 +141,9 @@ MLP_FLOPS_PER_QUERY = 2 * 64 * 64 * 4  # four 64x64 layers, multiply-adds as two
 BF16_PEAK_TFLOPS = 2500.0
 
 
-def guided_leg(k, dev, label, text, depth, W, H, spp, passes, config=None):
-    """Plain and learned-guided renders of the same samples of one image.
+def guided_leg(k, dev, label, text, depth, W, H, spp, passes, config=None, guide="learned"):
+    """Plain and guided renders of the same samples of one image, guided by
+    the learned guide or by the plain (highlight) guide.
 
     Both estimators are unbiased (the guided one by one-sample MIS; its
     mean is checked against a reference in the tests), so each one's MSE is
@@ -153,9 +154,10 @@ def guided_leg(k, dev, label, text, depth, W, H, spp, passes, config=None):
     method's own noise enters the normalisation). Equal-MSE Msamples/s = samples / seconds x
     (relMSE of the plain render / relMSE of the method): the plain
     render's rate scaled by how many fewer samples the method needs for
-    the same error. The guided passes train the guide as they go (the
+    the same error. The learned guide's passes train it as they go (the
     first pass samples with the BSDF only); their time includes inference
-    and training."""
+    and training. The plain guide's time includes its survey and pixel
+    guides."""
     import numpy as np
     scene = k.Scene(text)
     cam, rs = scene.settings()
@@ -167,13 +169,16 @@ def guided_leg(k, dev, label, text, depth, W, H, spp, passes, config=None):
     rgb, _, n = dev.accumulation(W, H)
     denom = (rgb / n[..., None]) ** 2 + 1e-2
     out = {}
-    for method, flags in (("plain", 0), ("guided", k.PASS_GUIDE)):
-        if flags:
+    survey = None
+    for method, flags in (("plain", 0), ("guided", k.PASS_GUIDE if guide == "learned" else k.PASS_HIGHLIGHT)):
+        if flags == k.PASS_GUIDE:
             dev.guide_reset(1)
             if config:
                 dev.guide_configure(**config)
         stats = []
         start = time.perf_counter()
+        if flags == k.PASS_HIGHLIGHT:
+            survey = dev.highlight_guides(W, H)
         for p in range(passes):
             stats.append(dev.render_pass(W, H, spp, first_sample=p * spp, seed=rs.seed, surfaces=rs.surfaces, keep=p > 0,
                                          flags=flags | k.PASS_TIME))
@@ -183,13 +188,18 @@ def guided_leg(k, dev, label, text, depth, W, H, spp, passes, config=None):
         var = np.maximum(sq / n[..., None] - mean ** 2, 0) / n[..., None]
         out[method] = dict(seconds=dt, mean=mean, var=var, stats=stats)
     samples = W * H * spp * passes
-    res = {"workload": f"{label} {W}x{H}, {passes} passes of {spp} spp, octree depth {depth}, {rs.surfaces} surfaces"}
+    res = {"workload": f"{label} {W}x{H}, {passes} passes of {spp} spp, octree depth {depth}, {rs.surfaces} surfaces",
+           "guide": guide}
     rel_plain = float(np.mean(out["plain"]["var"] / denom))
     for method in ("plain", "guided"):
         rel = float(np.mean(out[method]["var"] / denom))
         rate = samples / out[method]["seconds"] / 1e6
         res[method] = {"msamples_s": rate, "relmse": rel, "equal_mse_msamples_s": rate * rel_plain / rel,
                        "seconds": out[method]["seconds"]}
+    if survey is not None:
+        res["guided"].update({"survey_ms": survey.ms, "survey_points": survey.survey_points, "highlights": survey.highlights,
+                              "guided_pixels": survey.guided_pixels, "mean_lumps": survey.mean_lumps,
+                              "mean_share": survey.mean_share})
     g = out["guided"]["stats"]
     infer_ms = sum(st.guide_infer_ms for st in g)
     queries = sum(st.guide_queries for st in g)
@@ -323,7 +333,9 @@ def main():
         guided = [guided_leg(k, dev, "production scene (512 primitives, glass as white matte)", text, args.depth, W, H, S,
                              args.guide_passes),
                   guided_leg(k, dev, "window room (sky through a ceiling window)", scenes.window_room(W, H), 6, W, H, S,
-                             args.guide_passes)]
+                             args.guide_passes),
+                  guided_leg(k, dev, "highlight room (a lamp mirrored onto the far wall by a steel plate)",
+                             scenes.highlight_room(W, H), 6, W, H, S, args.guide_passes, guide="plain")]
 
     # the denoiser on the 1920x1080 history of the timed steps (their FP64
     # sums): a G-buffer pass of primary rays, then the filter
