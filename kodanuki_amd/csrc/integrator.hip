@@ -310,12 +310,12 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
         stats->segments += count;
         ++stats->bounces;
         ++stats->trace_launches;
-        const uint32_t blocks = (count + 255) / 256;
+        const uint32_t blocks = (count + 255) / 256, traceBlocks = (count + kTraceBlock - 1) / kTraceBlock;
         timed(&stats->trace_ms, [&] {
             if (counting) {
-                hipLaunchKernelGGL(k_trace_queue_count, dim3(blocks), dim3(kTraceBlock), 0, st, P, in, work);
+                hipLaunchKernelGGL(k_trace_queue_count, dim3(traceBlocks), dim3(kTraceBlock), 0, st, P, in, work);
             } else {
-                hipLaunchKernelGGL(k_trace_queue, dim3(blocks), dim3(kTraceBlock), 0, st, P, in);
+                hipLaunchKernelGGL(k_trace_queue, dim3(traceBlocks), dim3(kTraceBlock), 0, st, P, in);
             }
         });
         if (guiding && bounce < (uint32_t)kGuideDepths) {
