@@ -203,7 +203,7 @@ uint64_t residentBytes(const Ctx& c) {
                      (uint64_t)s.n_leaves * 4;
     bytes += c.tree.nodes.bytes + c.tree.entries.bytes + c.tree.pool.bytes + c.tree.spool.bytes;
     for (const DevBuf& b : c.scratch) bytes += b.bytes;
-    bytes += c.paths_bytes + c.round_bytes + c.hguides.bytes;
+    bytes += c.paths_bytes + c.round_bytes + c.hguides.bytes + c.sort_tmp.bytes;
     if (c.accum) bytes += (uint64_t)c.width * c.height * (6 * sizeof(double) + sizeof(uint32_t));
     return bytes;
 }
@@ -234,6 +234,7 @@ void releaseAll(Ctx& c) {
     freeIntegrator(c);
     freeGuide(c);
     freeBuf(c.hguides);
+    freeBuf(c.sort_tmp);
     c.hguide_width = c.hguide_height = 0;
     if (c.materials_dev) hipFree(c.materials_dev);
     if (c.lamps_dev) hipFree(c.lamps_dev);

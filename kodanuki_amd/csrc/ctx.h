@@ -75,6 +75,7 @@ struct Ctx {
     size_t round_bytes = 0;
     bool scene_has_glass = false;   // a leaf is painted with a glass
     void* guide = nullptr;          // the learned guide (GuideState), made on first use
+    DevBuf sort_tmp;                // the ray queues' sort's temporary storage
     DevBuf hguides;                 // the plain guide: a packed guide per pixel
     int hguide_width = 0, hguide_height = 0;  // the image they were made for (0: none)
 };
@@ -107,6 +108,11 @@ void freeGuide(Ctx& c);
 
 // hguide.hip: the plain guide of every pixel of a width x height image
 bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats);
+
+// sort.hip: n (key, value) pairs sorted by the key's low endBit bits;
+// *sorted is vals or valsAlt, whichever holds the sorted values
+bool sortPairsU32(uint32_t* keys, uint32_t* keysAlt, uint32_t* vals, uint32_t* valsAlt, uint32_t n, int endBit, DevBuf& tmp,
+                  hipStream_t s, std::string& err, uint32_t** sorted);
 
 // scan.hip: exclusive prefix sum of n u32 values into out, returns the total
 bool scanU32(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* total, DevBuf& tmp, hipStream_t s, std::string& err);

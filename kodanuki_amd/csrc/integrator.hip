@@ -78,6 +78,14 @@ __global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_finish(PassParams 
     finish_one(P, q, inQueue, lr, tally, work);
 }
 
+// the sort keys of a queue's rays (the paths' next rays, or their shadow rays)
+__global__ void k_sort_keys(PassParams P, const uint32_t* queue, uint32_t count, uint32_t shadow) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    const uint32_t i = queue[q];
+    P.s.keys[q] = ray_key(job_of(P, i).tv, shadow ? P.s.shadow[i] : P.s.rays[i]);
+}
+
 __global__ void k_fold(PassParams P, uint32_t slice) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     fold_one(P, slice, t);
@@ -109,7 +117,7 @@ static bool allocPaths(Ctx& c, size_t n) {
     }
     // one allocation, 256-byte aligned pieces
     const size_t pieces[] = {n * 4, n * sizeof(kdn_ray), n * sizeof(kdn_hit), n * sizeof(kdn_ray), n * 16, n * 16, n * 16, n * 16,
-                             n * 16, n * 4, n * 4, n * 4, n * 4, n * 8, n * 4, n * 4, n * 4, 64};
+                             n * 16, n * 4, n * 4, n * 4, n * 4, n * 8, n * 4, n * 4, n * 4, 64, n * 4, n * 4, n * 4};
     size_t bytes = 0;
     for (size_t b : pieces) bytes += (b + 255) & ~size_t(255);
     char* base = nullptr;
@@ -141,6 +149,9 @@ static bool allocPaths(Ctx& c, size_t n) {
     s->stream = (uint64_t*)take();
     for (int q = 0; q < 3; ++q) s->queue[q] = (uint32_t*)take();
     s->counters = (uint32_t*)take();
+    s->keys = (uint32_t*)take();
+    s->keys_alt = (uint32_t*)take();
+    s->vals_alt = (uint32_t*)take();
     s->capacity = n;
     c.paths = s;
     c.paths_bytes = bytes;
@@ -275,6 +286,22 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
         }
         P.hg = (const float*)c.hguides.ptr;
     }
+    // a queue's rays ordered by direction octant and origin cell, so that a
+    // wave's lanes walk nearby cells alike (each path's result does not
+    // depend on where it stands in a queue)
+    auto sortQueue = [&](uint32_t qi, uint32_t n, uint32_t shadow) -> bool {
+        if (n < kSortMin) return true;
+        bool ok = true;
+        timed(&stats->other_ms, [&] {
+            uint32_t* queue = s->queue[qi];
+            hipLaunchKernelGGL(k_sort_keys, dim3((n + 255) / 256), dim3(256), 0, st, P, (const uint32_t*)queue, n, shadow);
+            uint32_t* sorted = nullptr;
+            ok = sortPairsU32(s->keys, s->keys_alt, queue, s->vals_alt, n, 30, c.sort_tmp, st, c.error, &sorted);
+            if (ok && sorted != queue) ok = hipMemcpyAsync(queue, sorted, (size_t)n * 4, hipMemcpyDeviceToDevice, st) == hipSuccess;
+        });
+        if (!ok && c.error.empty()) c.error = "sorting a ray queue failed";
+        return ok;
+    };
     if (guiding) {
         if (!guideReady(c) && !guideReset(c, 1, c.tree.lower, c.tree.size)) return false;
         if (!guideBeginRound(c, P, total)) return false;
@@ -293,6 +320,7 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
         // finishing launch shades without the network's lobes)
         if (finishing && (!guiding || bounce >= (uint32_t)kGuideDepths) && (uint64_t)count * kFinishShare <= total) {
             // the remaining paths to their ends in one launch
+            if (!sortQueue(in, count, 0)) return false;
             KDN_HIP(hipMemsetAsync(&s->counters[4], 0, 8, st));
             timed(&stats->finish_ms, [&] {
                 hipLaunchKernelGGL(k_finish, dim3((count + kTraceBlock - 1) / kTraceBlock), dim3(kTraceBlock), 0, st, P, in,
@@ -310,6 +338,7 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
         stats->segments += count;
         ++stats->bounces;
         ++stats->trace_launches;
+        if (bounce > 0 && !sortQueue(in, count, 0)) return false;  // camera rays come in 8x8 tiles already
         const uint32_t blocks = (count + 255) / 256, traceBlocks = (count + kTraceBlock - 1) / kTraceBlock;
         timed(&stats->trace_ms, [&] {
             if (counting) {
@@ -335,6 +364,7 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
         KDN_HIP(hipStreamSynchronize(st));
         stats->shadow_rays += sc;
         if (sc) {
+            if (!sortQueue(2, sc, 1)) return false;
             ++stats->shadow_launches;
             timed(&stats->shadow_ms, [&] {
                 hipLaunchKernelGGL(k_shadow, dim3((sc + kTraceBlock - 1) / kTraceBlock), dim3(kTraceBlock), 0, st, P,

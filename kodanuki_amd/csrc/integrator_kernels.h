@@ -26,6 +26,9 @@ struct PathState {
     uint64_t* stream;
     uint32_t* queue[3];
     uint32_t* counters;
+    uint32_t* keys;      // a queue's sort keys, and the sort's other buffers
+    uint32_t* keys_alt;
+    uint32_t* vals_alt;
     size_t capacity;
 };
 
@@ -298,6 +301,28 @@ KDN_D Scatter highlight_scatter(const MaterialDev& m, const Lanes& w, V3 n, V3 o
 
 KDN_D float lift_at(V3 x) {
     return 8e-5f + 8e-6f * fmax_(fmax_(__builtin_fabsf(x.x), __builtin_fabsf(x.y)), __builtin_fabsf(x.z));
+}
+
+// The sort key of a ray: its direction's octant over the Morton code of its
+// origin's cell in a 512^3 grid over the region (9 bits an axis), so that
+// sorted rays leave nearby cells the same way.
+KDN_D uint32_t spread_bits(uint32_t v) {
+    v &= 0x1FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+KDN_D uint32_t ray_key(const TreeView& tv, const kdn_ray& r) {
+    uint32_t cell[3];
+    for (int k = 0; k < 3; ++k) {
+        const float f = (r.origin[k] - tv.lower[k]) / tv.size[k] * 512.0f;
+        cell[k] = f > 0 ? (f < 511.0f ? (uint32_t)f : 511u) : 0u;
+    }
+    const uint32_t octant = (r.direction[0] < 0 ? 1u : 0u) | (r.direction[1] < 0 ? 2u : 0u) | (r.direction[2] < 0 ? 4u : 0u);
+    return (octant << 27) | spread_bits(cell[0]) | (spread_bits(cell[1]) << 1) | (spread_bits(cell[2]) << 2);
 }
 
 // shade path i at its hit: emission met, a shadow test toward one lamp

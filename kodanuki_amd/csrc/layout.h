@@ -39,6 +39,9 @@ constexpr int kTraceWaves = 5;
 // long as its slowest ray; the first bounces' many coherent rays go faster
 // in a launch per kernel).
 constexpr uint32_t kFinishShare = 8;
+// Queues of at least this many rays are sorted by direction octant and
+// origin cell before their march launch (smaller ones are not worth a sort).
+constexpr uint32_t kSortMin = 1u << 16;
 // Octree levels a march keeps the ancestry of (depth caps 0..14).
 constexpr int kPathLevels = 16;
 
