@@ -68,14 +68,13 @@ __global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_shadow(PassParams 
     shadow_one(P, q, lr, work);
 }
 
-// The paths still in the input queue, each to its end in one lane (the
-// wavefront's last bounces carry few paths, and each launch of them costs
-// as long as its slowest ray).
+// The paths still in the input queue to their ends, in as many lanes as
+// the device holds at once (the wavefront's last bounces carry few paths,
+// and each launch of them costs as long as its slowest ray): each lane
+// takes the queue's entries in turn (tally[2] counts them out).
 __global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_finish(PassParams P, uint32_t inQueue, uint32_t* tally, double* work) {
     KDN_LANE_REGS;
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= P.s.counters[inQueue]) return;
-    finish_one(P, q, inQueue, lr, tally, work);
+    finish_lane(P, inQueue, P.s.counters[inQueue], lr, tally, work);
 }
 
 // the sort keys of a queue's rays (the paths' next rays, or their shadow rays)
@@ -225,6 +224,14 @@ JobDev jobOf(const Ctx& scene, int width, int height, uint32_t seed, uint32_t su
     return J;
 }
 
+// trace-kernel blocks the device runs at once: kTraceWaves waves on each
+// of a compute unit's four SIMDs
+static uint32_t residentTraceBlocks(const Ctx& c) {
+    int units = 0;
+    if (hipDeviceGetAttribute(&units, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess || units <= 0) units = 256;
+    return (uint32_t)units * 4 * kTraceWaves * 64 / kTraceBlock;
+}
+
 bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDev>& slices, uint32_t flags, kdn_pass_stats* stats) {
     if (!ensureColour(c)) return false;
     hipStream_t st = c.stream;
@@ -321,10 +328,11 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
         if (finishing && (!guiding || bounce >= (uint32_t)kGuideDepths) && (uint64_t)count * kFinishShare <= total) {
             // the remaining paths to their ends in one launch
             if (!sortQueue(in, count, 0)) return false;
-            KDN_HIP(hipMemsetAsync(&s->counters[4], 0, 8, st));
+            KDN_HIP(hipMemsetAsync(&s->counters[4], 0, 12, st));
+            const uint32_t finishBlocks = std::min<uint32_t>((count + kTraceBlock - 1) / kTraceBlock, residentTraceBlocks(c));
             timed(&stats->finish_ms, [&] {
-                hipLaunchKernelGGL(k_finish, dim3((count + kTraceBlock - 1) / kTraceBlock), dim3(kTraceBlock), 0, st, P, in,
-                                   &s->counters[4], counting ? work + 2 : nullptr);
+                hipLaunchKernelGGL(k_finish, dim3(finishBlocks), dim3(kTraceBlock), 0, st, P, in, &s->counters[4],
+                                   counting ? work + 2 : nullptr);
             });
             uint32_t tally[2] = {0, 0};
             KDN_HIP(hipMemcpyAsync(tally, &s->counters[4], 8, hipMemcpyDeviceToHost, st));

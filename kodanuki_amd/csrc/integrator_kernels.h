@@ -571,16 +571,20 @@ KDN_D void shadow_one(const PassParams& P, uint32_t q, const LaneRegs& lr, doubl
     shadow_path(P, P.s.queue[2][q], lr, work);
 }
 
-// finish: entry q of the input queue, its path to the end in one lane:
-// extend, shade and the shadow test in turn until the path stops (shading
-// stops every path at its last surface). The same per-path arithmetic as
-// the wavefront's launches, so the estimates do not change; counts
-// segments and shadow tests in tally[0] and tally[1].
-KDN_D void finish_one(PassParams& P, uint32_t q, uint32_t inQueue, const LaneRegs& lr, uint32_t* tally, double* work) {
+// finish: the lane takes entries of the input queue in turn (tally[2]
+// counts them out; count of them in all) and runs one segment of its path
+// an iteration: extend, shade and the shadow test, taking the next entry
+// when its path stops (shading stops every path at its last surface), so
+// that a wave's lanes stay busy until the queue drains rather than wait
+// for the wave's longest path. The same per-path arithmetic as the
+// wavefront's launches, so the estimates do not change; counts segments
+// and shadow tests in tally[0] and tally[1].
+KDN_D void finish_lane(PassParams& P, uint32_t inQueue, uint32_t count, const LaneRegs& lr, uint32_t* tally, double* work) {
     PathState& s = P.s;
-    const uint32_t i = s.queue[inQueue][q];
     uint32_t segments = 0, shadows = 0;
-    for (;;) {
+    uint32_t q = atomicAdd(&tally[2], 1u);
+    while (q < count) {
+        const uint32_t i = s.queue[inQueue][q];
         trace_path(P, i, lr, work);
         ++segments;
         bool shadow;
@@ -589,7 +593,7 @@ KDN_D void finish_one(PassParams& P, uint32_t q, uint32_t inQueue, const LaneReg
             shadow_path(P, i, lr, work ? work + 1 : nullptr);
             ++shadows;
         }
-        if (!alive) break;
+        if (!alive) q = atomicAdd(&tally[2], 1u);
     }
     atomicAdd(&tally[0], segments);
     atomicAdd(&tally[1], shadows);

@@ -71,6 +71,13 @@ typedef EmuEvent* hipEvent_t;
 inline const char* hipGetErrorString(hipError_t e) { return e == hipSuccess ? "no error" : "emulated HIP error"; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline hipError_t hipSetDevice(int) { return hipSuccess; }
+// a small device: the launches that size their grid by it run few blocks,
+// so their lanes take many queue entries each
+enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 0 };
+inline hipError_t hipDeviceGetAttribute(int* value, hipDeviceAttribute_t, int) {
+    *value = 2;
+    return hipSuccess;
+}
 inline hipError_t hipGetDeviceCount(int* n) {
     *n = 1;
     return hipSuccess;
