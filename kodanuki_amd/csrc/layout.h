@@ -35,10 +35,12 @@ constexpr int kTraceBlock = 64;
 // of it (the LDS of a lane, 128 B, allows five).
 constexpr int kTraceWaves = 5;
 // A pass finishes its paths in one launch once no more than one in
-// kFinishShare of a chunk's paths are left (each wavefront launch costs as
-// long as its slowest ray; the first bounces' many coherent rays go faster
-// in a launch per kernel).
-constexpr uint32_t kFinishShare = 8;
+// kFinishShare of a chunk's paths are left: with 1, every path runs in the
+// finishing launch from its camera ray on (its lanes pull paths and run a
+// segment an iteration, so a wave waits only for its slowest segment; 1.2x
+// the samples per second of a wavefront launch per bounce at 1080p). A
+// learned-guide round keeps a launch per bounce over the guided depths.
+constexpr uint32_t kFinishShare = 1;
 // Queues of at least this many rays are sorted by direction octant and
 // origin cell before their march launch (smaller ones are not worth a sort).
 constexpr uint32_t kSortMin = 1u << 16;
