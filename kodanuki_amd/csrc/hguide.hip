@@ -17,9 +17,11 @@
 // lobe, and gives the lump the spread a flat glossy mirror of that width
 // would: (2 width + the lamp's angular radius) d / (D + d) radians across,
 // d from the point to the lamp, D from the viewer. A pixel's guide takes the highlights of its
-// four nearest survey points on its own material, weighs each by
-// nearness, the lamp's glare, the lump's size and the material's density
-// toward it (weighedLump, survey.cpp:427-430), keeps the 12 heaviest and
+// four nearest survey points on its own material, each walked again from
+// the pixel's own stand (one mirror, one step) and one lump per highlight,
+// weighs each by nearness, the lamp's glare, the lump's size and the
+// material's density toward it (weighedLump, survey.cpp:427-430), keeps
+// the 12 heaviest and
 // drops those under a hundredth of the heaviest, and shares its draws by
 // the light the lumps bring against the direct light, within [0.02, 0.5]
 // (guideShare, survey.cpp:818-828). The guide's lumps are the reference's
@@ -41,9 +43,11 @@ namespace {
 constexpr int kHgBlock = 256;
 constexpr int kSurveyDraws = 128;
 constexpr int kSpots = 16;           // highlights a survey point keeps
-constexpr int kSpotFloats = 12;      // kept, lamp, point 3, spread, width, distance, stand material, importance, leaf bits, material
+constexpr int kSpotFloats = 15;      // kept, lamp, point 3, spread, width, distance, stand material, importance, leaf bits, material,
+                                     // the surface's normal there (3, facing the stand)
 constexpr float kShinyWidth = 0.125f;
 constexpr float kSameSpot = 0.05f;
+constexpr float kMoved = 0.1f;       // the farthest a highlight walked again from a pixel may move
 constexpr double kGuideEfficiency = 0.3;
 constexpr float kLeastShare = 0.02f, kMostShare = 0.5f;
 
@@ -289,6 +293,9 @@ __global__ void k_hg_candidates(HgScene S, const kdn_ray* aims, const kdn_hit* h
     c[7] = D;
     c[9] = lamp_glare(L) * sigma * sigma;
     c[10] = __builtin_bit_cast(float, h.leaf);
+    c[12] = nv.x;
+    c[13] = nv.y;
+    c[14] = nv.z;
 }
 
 // each survey point's highlights: its candidates less the ones of a
@@ -356,6 +363,7 @@ __global__ void k_hg_pixels(HgScene S, const float* stands, const float* spots, 
     float heavy[kHgLumps];
     float lump[kHgLumps][kHgLumpFloats];
     float radiance[kHgLumps];
+    uint32_t lampOf[kHgLumps];
     int count = 0;
     for (int q = 0; q < 4; ++q) {
         if (!(blend[q] > 0)) continue;
@@ -367,23 +375,73 @@ __global__ void k_hg_pixels(HgScene S, const float* stands, const float* spots, 
         for (int k = 0; k < kept; ++k) {
             const float* h = spot + 1 + k * kSpotFloats;
             if ((uint32_t)h[8] != mi) continue;
-            const V3 point = v3(h[2], h[3], h[4]);
+            const uint32_t lamp = (uint32_t)h[1];
+            const LampDev& L = S.lamps[lamp];
+            const V3 centre = lamp_centre(L);
+            // walked again from this stand: the point of the highlight's
+            // tangent plane that mirrors the lamp's centre toward it, kept
+            // when it moves less than kMoved (the plane may end sooner)
+            V3 point = v3(h[2], h[3], h[4]);
+            {
+                const V3 mirrorN = v3(h[12], h[13], h[14]);
+                const float above = dot(centre - point, mirrorN);
+                const V3 image = centre - mirrorN * (2 * above);
+                const float t = dot(point - x, mirrorN) / dot(image - x, mirrorN);
+                if (above > 0 && t > 0 && t < 1) {
+                    const V3 moved = x + (image - x) * t;
+                    if (length(moved - point) < kMoved) point = moved;
+                }
+            }
             const V3 towards = point - x;
-            const float dist = length(towards);
-            if (!(dist > 1e-4f)) continue;
+            const float dist = length(towards), dl = length(centre - point);
+            if (!(dist > 1e-4f && dl > 1e-4f)) continue;
             const V3 axis = towards * (1.0f / dist);
-            const float spread = h[5] * (h[7] * h[7]) / (dist * dist);  // scaledSpread (guide.cpp:284-287)
-            if (!(spread > 1e-12f && spread < 1)) continue;
-            const float f0 = __builtin_sqrtf(spread);
-            const LampDev& L = S.lamps[(uint32_t)h[1]];
+            // the spread a flat glossy mirror of the start's width gives
+            // seen from this stand (scaledSpread, guide.cpp:284-287, scales
+            // the survey's by distance)
+            const float rho = 0.5f * __builtin_sqrtf(L.area) / dl;
+            const float sigma = fmax_((2 * h[6] + rho) * dl / (dist + dl), 1e-3f);
+            const float spread = sigma * sigma;
+            if (!(spread < 1)) continue;
+            const float f0 = sigma;
             const float w = lamp_glare(L) * f0 * f0 * rough_density(m, n, out, axis) * blend[q];
             if (!(w > 0)) continue;
+            // the same highlight seen through another survey point adds
+            // its weight to the lump already kept
+            int same = -1;
+            for (int k = 0; k < count && same < 0; ++k) {
+                const V3 d = v3(lump[k][6], lump[k][7], lump[k][8]) - point;
+                if (lampOf[k] == lamp && dot(d, d) < kSameSpot * kSameSpot) same = k;
+            }
+            if (same >= 0) {
+                heavy[same] += w;
+                lump[same][14] += w;
+                while (same > 0 && heavy[same - 1] < heavy[same]) {
+                    const float hw = heavy[same];
+                    heavy[same] = heavy[same - 1];
+                    heavy[same - 1] = hw;
+                    const float r = radiance[same];
+                    radiance[same] = radiance[same - 1];
+                    radiance[same - 1] = r;
+                    const uint32_t lo = lampOf[same];
+                    lampOf[same] = lampOf[same - 1];
+                    lampOf[same - 1] = lo;
+                    for (int f = 0; f < kHgLumpFloats; ++f) {
+                        const float v = lump[same][f];
+                        lump[same][f] = lump[same - 1][f];
+                        lump[same - 1][f] = v;
+                    }
+                    --same;
+                }
+                continue;
+            }
             // keep the kHgLumps heaviest, heaviest first
             int place = count < kHgLumps ? count : kHgLumps - 1;
             if (count == kHgLumps && !(w > heavy[place])) continue;
             while (place > 0 && heavy[place - 1] < w) {
                 heavy[place] = heavy[place - 1];
                 radiance[place] = radiance[place - 1];
+                lampOf[place] = lampOf[place - 1];
                 for (int f = 0; f < kHgLumpFloats; ++f) lump[place][f] = lump[place - 1][f];
                 --place;
             }
@@ -392,6 +450,7 @@ __global__ void k_hg_pixels(HgScene S, const float* stands, const float* spots, 
                                               0, 0, f0, 0, f0, w};
             for (int f = 0; f < kHgLumpFloats; ++f) lump[place][f] = rec[f];
             heavy[place] = w;
+            lampOf[place] = lamp;
             const MaterialDev& sm = S.materials[(uint32_t)h[11]];
             radiance[place] = lamp_glare(L) * (0.2126f * sm.albedo[0] + 0.7152f * sm.albedo[1] + 0.0722f * sm.albedo[2]);
             if (count < kHgLumps) ++count;

@@ -106,8 +106,9 @@ def test_survey_guides_the_wall_toward_the_plate(device, room):
             lumps = g[H.HEAD:H.HEAD + n * H.LUMP_FLOATS].reshape(n, H.LUMP_FLOATS)
             assert np.all(lumps[:, 14] > 0)
             assert np.allclose(np.linalg.norm(lumps[:, 0:3], axis=1), 1, atol=1e-5)
-            # every lump's point lies on the plate (the steel box on the floor)
-            assert np.all(np.abs(lumps[:, 6:8]) <= 0.61) and np.all(np.abs(lumps[:, 8] - 0.02) < 0.01)
+            # every lump's point lies on the plate's plane, within 0.1 of the
+            # plate (a highlight walked again from the pixel moves at most 0.1)
+            assert np.all(np.abs(lumps[:, 6:8]) <= 0.71) and np.all(np.abs(lumps[:, 8] - 0.02) < 0.01)
     assert guided > 0
 
 
