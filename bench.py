@@ -57,14 +57,16 @@ def cpu_baseline(scene, lower, size, width, height, seed, surfaces, budget_s=12.
                       f"(FP64 sphere trace over every object program), {threads} threads, {dt:.1f} s"}
 
 
-def serving_stream(k, scenes, dev, jobs, rate, seed, max_paths, budget, sizes=None, spps=None, pool_texts=None):
+def serving_stream(k, scenes, dev, jobs, rate, seed, max_paths, budget, sizes=None, spps=None, pool_texts=None, guided=False):
     """The serving stream: `jobs` render jobs with Poisson arrivals at
     `rate` jobs/s, each a scene of the eight-scene pool (32 to 4096
     primitives), a size of {256^2, 512^2, 1024^2, 1920x1080} and
     {1, 4, 16, 64} spp, served by kdn_serve_* (continuous batching into
     shared rounds under a memory budget). Latency is from a job's arrival
     to the end of the round that folds its last sample; scene builds are
-    part of the first job of each scene."""
+    part of the first job of each scene. With guided, every job samples
+    its first vertices with the plain guide (its survey at admission is
+    part of its latency)."""
     import random
     rng = random.Random(seed)
     pool = []
@@ -96,7 +98,7 @@ def serving_stream(k, scenes, dev, jobs, rate, seed, max_paths, budget, sizes=No
         while pending and pending[0][0] <= now:
             arrival, si, w, h, spp = pending.pop(0)
             sc, depth, rs = pool[si]
-            j = server.submit(sc, w, h, spp, seed=rs.seed, surfaces=rs.surfaces, depth=depth)
+            j = server.submit(sc, w, h, spp, seed=rs.seed, surfaces=rs.surfaces, depth=depth, guided=guided)
             open_jobs[j] = (arrival, si, w, h, spp)
         if not open_jobs:
             time.sleep(max(0.0, pending[0][0] - (time.perf_counter() - start)))
@@ -130,7 +132,7 @@ def serving_stream(k, scenes, dev, jobs, rate, seed, max_paths, budget, sizes=No
                         f"{', '.join(f'{w}x{h}' for w, h in sizes)}; spp from {', '.join(map(str, spps))}; scenes from "
                         f"{len(pool)} seeded scenes" + (" of 32 to 4096 primitives" if pool_texts is None else "") +
                         f" (glass painted as white matte); rounds of at most {max_paths} paths, "
-                        f"budget {budget / 2 ** 30:g} GiB",
+                        f"budget {budget / 2 ** 30:g} GiB" + ("; every job with the plain guide" if guided else ""),
             "p50_ms": pct(0.50), "p99_ms": pct(0.99), "mean_ms": sum(lat) / len(lat),
             "aggregate_msamples_s": total_samples / wall_s / 1e6, "samples": total_samples, "wall_s": wall_s,
             "rounds": rounds, "mean_jobs_per_round": jobs_in_rounds / max(rounds, 1),
@@ -362,10 +364,14 @@ def main():
     print(f"[denoise] filter {fl:.2f} ms, G-buffer {gb:.2f} ms ({achieved:.0f} GB/s of unique bytes)", file=sys.stderr,
           flush=True)
 
-    serving = None
+    serving = serving_guided = None
     if not args.no_serve:
         serving = serving_stream(k, scenes, dev, args.serve_jobs, args.serve_rate, args.serve_seed, args.serve_paths,
                                  args.serve_budget_gib << 30)
+        if not args.no_guide:
+            # the same stream served with the plain guide on every job
+            serving_guided = serving_stream(k, scenes, dev, args.serve_jobs, args.serve_rate, args.serve_seed, args.serve_paths,
+                                            args.serve_budget_gib << 30, guided=True)
 
     cpu = None
     if not args.no_cpu:
@@ -392,6 +398,7 @@ def main():
         "guided": guided,
         "denoise": denoise,
         "serving": serving,
+        "serving_guided": serving_guided,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -526,6 +526,10 @@ typedef struct kdn_job_desc {
     uint32_t surfaces;       /* the most surfaces a path stands on */
     uint32_t min_depth;      /* roulette starts at this depth */
     int32_t depth;           /* the octree's depth cap */
+    uint32_t flags;          /* bit 0: the plain guide (a survey of the job's
+                                image at admission; its guides are the job's
+                                memory, 736 B a pixel) */
+    uint32_t pad;
 } kdn_job_desc;
 
 #define KDN_JOB_WAITING 0

@@ -165,7 +165,11 @@ class GuideLobe(C.Structure):
 
 class JobDesc(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_uint32), ("seed", C.c_uint32),
-                ("surfaces", C.c_uint32), ("min_depth", C.c_uint32), ("depth", C.c_int32)]
+                ("surfaces", C.c_uint32), ("min_depth", C.c_uint32), ("depth", C.c_int32), ("flags", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+JOB_PLAIN_GUIDE = 1
 
 
 class JobStatus(C.Structure):
@@ -608,8 +612,10 @@ class Server:
     def __exit__(self, *a):
         self.close()
 
-    def submit(self, scene, width, height, spp, seed=1, surfaces=7, min_depth=4, depth=8):
-        d = JobDesc(scene.handle, width, height, spp, seed, surfaces, min_depth, depth)
+    def submit(self, scene, width, height, spp, seed=1, surfaces=7, min_depth=4, depth=8, guided=False):
+        """A job; guided: its first vertices sampled with the plain guide
+        (a survey of its image when it is admitted)."""
+        d = JobDesc(scene.handle, width, height, spp, seed, surfaces, min_depth, depth, JOB_PLAIN_GUIDE if guided else 0, 0)
         job = C.c_uint64()
         self._check(library().kdn_serve_submit(self._h, C.byref(d), C.byref(job)))
         self._scenes[job.value] = scene

@@ -107,7 +107,7 @@ bool guideReady(const Ctx& c);
 void freeGuide(Ctx& c);
 
 // hguide.hip: the plain guide of every pixel of a width x height image
-bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats);
+bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats, DevBuf& out, DevBuf& work);
 
 // sort.hip: n (key, value) pairs sorted by the key's low endBit bits;
 // *sorted is vals or valsAlt, whichever holds the sorted values

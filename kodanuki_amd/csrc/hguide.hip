@@ -490,8 +490,9 @@ __global__ void k_hg_pixels(HgScene S, const float* stands, const float* spots, 
 }  // namespace
 
 // Survey and pixel guides for an image of width x height of the context's
-// scene and camera, into c.hguides.
-bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats) {
+// scene and camera, into out (kHgFloats a pixel), with work as the
+// survey's scratch.
+bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats, DevBuf& out, DevBuf& work) {
     hipStream_t st = c.stream;
     const kdn_camera& cam = c.scene.camera;
     double fwd[3], rt[3], upv[3], len = 0;
@@ -524,8 +525,8 @@ bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats) {
     const size_t nRays = std::max<size_t>(nDraws, nPixels);
     const size_t bytes = nRays * (sizeof(kdn_ray) + sizeof(kdn_hit)) + (size_t)nDraws * sizeof(kdn_ray) + (size_t)std::max<size_t>(nPoints, nPixels) * kStandFloats * 4 +
                          (size_t)nDraws * kSpotFloats * 4 + (size_t)nPoints * (1 + kSpots * kSpotFloats) * 4 + 8 * 256;
-    if (!ensure(c.scratch[5], bytes, c.error, false)) return false;
-    char* at = (char*)c.scratch[5].ptr;
+    if (!ensure(work, bytes, c.error, false)) return false;
+    char* at = (char*)work.ptr;
     auto take = [&](size_t b) {
         char* p = at;
         at += (b + 255) & ~size_t(255);
@@ -537,7 +538,7 @@ bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats) {
     float* stands = (float*)take((size_t)std::max<size_t>(nPoints, nPixels) * kStandFloats * 4);
     float* cand = (float*)take((size_t)nDraws * kSpotFloats * 4);
     float* spots = (float*)take((size_t)nPoints * (1 + kSpots * kSpotFloats) * 4);
-    if (!ensure(c.hguides, (size_t)nPixels * kHgFloats * 4, c.error, false)) return false;
+    if (!ensure(out, (size_t)nPixels * kHgFloats * 4, c.error, false)) return false;
     auto grid = [](uint32_t n) { return dim3((n + kHgBlock - 1) / kHgBlock); };
     hipEvent_t ev[2];
     KDN_HIP(hipEventCreate(&ev[0]));
@@ -562,7 +563,7 @@ bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats) {
     if (!extendRays(c, rays, nPixels, hits, st)) return false;
     hipLaunchKernelGGL(k_hg_stands, grid(nPixels), dim3(kHgBlock), 0, st, S, (const kdn_ray*)rays, (const kdn_hit*)hits, nPixels, stands);
     hipLaunchKernelGGL(k_hg_pixels, grid(nPixels), dim3(kHgBlock), 0, st, S, (const float*)stands, (const float*)spots, width, height,
-                       (nPoints > 0 && c.n_lamps > 0) ? cols : 0, rows, (float*)c.hguides.ptr);
+                       (nPoints > 0 && c.n_lamps > 0) ? cols : 0, rows, (float*)out.ptr);
     KDN_HIP(hipEventRecord(ev[1], st));
     KDN_HIP(hipEventSynchronize(ev[1]));
     KDN_HIP(hipGetLastError());
@@ -570,12 +571,10 @@ bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats) {
     KDN_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
     KDN_HIP(hipEventDestroy(ev[0]));
     KDN_HIP(hipEventDestroy(ev[1]));
-    c.hguide_width = width;
-    c.hguide_height = height;
     if (stats) {
         // guided pixels and highlights, counted on the host
         std::vector<float> heads((size_t)nPixels * kHgFloats);
-        KDN_HIP(hipMemcpy(heads.data(), c.hguides.ptr, heads.size() * 4, hipMemcpyDeviceToHost));
+        KDN_HIP(hipMemcpy(heads.data(), out.ptr, heads.size() * 4, hipMemcpyDeviceToHost));
         uint32_t guided = 0;
         double lumps = 0, share = 0;
         for (uint32_t p = 0; p < nPixels; ++p) {
@@ -618,7 +617,11 @@ extern "C" int kdn_highlight_guides(kdn_ctx* ctx, int32_t width, int32_t height,
         return KDN_E_STATE;
     }
     hipSetDevice(c.device);
-    return kdn::hguideBuild(c, width, height, stats) ? KDN_OK : KDN_E_DEVICE;
+    c.hguide_width = c.hguide_height = 0;
+    if (!kdn::hguideBuild(c, width, height, stats, c.hguides, c.scratch[5])) return KDN_E_DEVICE;
+    c.hguide_width = width;
+    c.hguide_height = height;
+    return KDN_OK;
 }
 
 extern "C" int kdn_highlight_guide_at(kdn_ctx* ctx, int32_t x, int32_t y, float* packed) {

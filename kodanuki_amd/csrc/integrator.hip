@@ -285,13 +285,13 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
         KDN_HIP(hipMemsetAsync(work, 0, 32, st));
     }
 
-    if (flags & 32) {
-        // the plain guide: one job, of the image the guides were made for
-        if (guiding || jobs.size() != 1 || c.hguide_width != jobs[0].width || c.hguide_height != jobs[0].height) {
-            c.error = "kdn_render_pass: no plain guide for this image (kdn_highlight_guides first; not with the learned guide)";
-            return false;
+    if (guiding) {
+        for (const JobDev& J : jobs) {
+            if (J.hg) {
+                c.error = "a round guides its paths by the learned guide or by the plain guide, not both";
+                return false;
+            }
         }
-        P.hg = (const float*)c.hguides.ptr;
     }
     // a queue's rays ordered by direction octant and origin cell, so that a
     // wave's lanes walk nearby cells alike (each path's result does not
@@ -435,7 +435,15 @@ bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
         KDN_HIP(hipMemsetAsync(c.accum_sq, 0, px * 3 * sizeof(double), st));
         KDN_HIP(hipMemsetAsync(c.accum_n, 0, px * sizeof(uint32_t), st));
     }
-    const std::vector<JobDev> jobs{jobOf(c, d.width, d.height, d.seed, d.surfaces, d.min_depth)};
+    std::vector<JobDev> jobs{jobOf(c, d.width, d.height, d.seed, d.surfaces, d.min_depth)};
+    if (d.flags & 32) {
+        // the plain guide, of the image the guides were made for
+        if ((d.flags & 16) || c.hguide_width != d.width || c.hguide_height != d.height) {
+            c.error = "kdn_render_pass: no plain guide for this image (kdn_highlight_guides first; not with the learned guide)";
+            return false;
+        }
+        jobs[0].hg = (const float*)c.hguides.ptr;
+    }
     const uint32_t plane = jobs[0].plane;
     const size_t maxPaths = c.max_paths ? c.max_paths : (size_t)1 << 23;
     const uint32_t chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(d.spp, maxPaths / plane));

@@ -49,6 +49,9 @@ struct JobDev {
     const LampDev* lamps;
     const MaterialDev* materials;
     const uint32_t* leaf_material;
+    // the plain guide: every pixel's packed guide of Gaussian lumps
+    // (kHgFloats each, pixel-major; null: none), steering the first vertex
+    const float* hg;
 };
 
 // A slice of a round: samples [first_sample, first_sample + spp) of every
@@ -78,9 +81,6 @@ struct PassParams {
     const float* lobes;
     float* recs;
     uint32_t train_stride;
-    // the plain guide: every pixel's packed guide of Gaussian lumps
-    // (kHgFloats each, pixel-major; null: none), steering the first vertex
-    const float* hg;
 };
 
 KDN_D const JobDev& job_of(const PassParams& P, uint32_t i) { return P.jobs[P.slices[P.s.slice[i]].job]; }
@@ -409,8 +409,8 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
         // the pixel's plain guide steers the first vertex on its material
         // (steersPlace, guide.cpp:162-164)
         const float* hg = nullptr;
-        if (P.hg != nullptr && depth == 0 && !inside && s.pixel[i] != 0xFFFFFFFFu) {
-            const float* g = P.hg + (size_t)s.pixel[i] * kHgFloats;
+        if (J.hg != nullptr && depth == 0 && !inside && s.pixel[i] != 0xFFFFFFFFu) {
+            const float* g = J.hg + (size_t)s.pixel[i] * kHgFloats;
             if (g[0] > 0 && (uint32_t)g[1] == mi && m.kind != KDN_GLASS) hg = g;
         }
         const float hgShare = hg ? hg[2] : 0.0f;

@@ -55,6 +55,7 @@ struct Job {
     double* accum = nullptr;
     double* accumSq = nullptr;
     uint32_t* accumN = nullptr;
+    DevBuf hg;             // the plain guide's pixel guides (flags bit 0)
     uint64_t bytes = 0;
 };
 
@@ -160,7 +161,8 @@ int tooBig(kdn_server& s, uint64_t need) {
 // may pass the budget while it runs).
 int admit(kdn_server& s, kdn::Job& j) {
     const uint64_t px = (uint64_t)j.desc.width * j.desc.height;
-    const uint64_t bytes = px * (6 * sizeof(double) + sizeof(uint32_t));
+    const bool guided = (j.desc.flags & 1) != 0;
+    const uint64_t bytes = px * (6 * sizeof(double) + sizeof(uint32_t) + (guided ? kdn::kHgFloats * sizeof(float) : 0));
     bool built = false;
     if (!s.scenes.count(j.key)) {
         auto known = s.sceneBytes.find(j.key);
@@ -201,6 +203,19 @@ int admit(kdn_server& s, kdn::Job& j) {
         res.users -= 1;
         return failed(&s, "serving: accumulator clear failed", KDN_E_DEVICE);
     }
+    if (guided) {
+        // the survey of the job's image on its scene's context, in the
+        // server's own scratch (kept for the next survey, and counted)
+        kdn::Ctx& sc = res.ctx->c;
+        const bool ok = kdn::hguideBuild(sc, j.desc.width, j.desc.height, nullptr, j.hg, c.scratch[5]);
+        hipSetDevice(c.device);
+        if (!ok) {
+            const std::string message = "serving: the plain guide's survey failed: " + sc.error;
+            releaseJob(s, j);
+            res.users -= 1;
+            return failed(&s, message, KDN_E_DEVICE);
+        }
+    }
     const uint32_t plane = kdn::planeOf(j.desc.width, j.desc.height);
     // fixed by the job and the server alone, never by the other jobs
     const uint64_t slicePaths = std::min<uint64_t>(kdn::kSlicePaths, s.maxPaths);
@@ -216,6 +231,8 @@ void releaseJob(kdn_server& s, kdn::Job& j) {
     if (j.accumN) hipFree(j.accumN);
     j.accum = j.accumSq = nullptr;
     j.accumN = nullptr;
+    if (j.hg.ptr) hipFree(j.hg.ptr);
+    j.hg = kdn::DevBuf{};
     j.bytes = 0;
 }
 
@@ -260,9 +277,9 @@ int kdn_serve_open(kdn_ctx* ctx, uint64_t budget_bytes, uint32_t max_paths, kdn_
 int kdn_serve_submit(kdn_server* s, const kdn_job_desc* job, uint64_t* id) {
     if (!s || !job || !job->scene || !id) return failed(s, "kdn_serve_submit: null argument", KDN_E_ARGUMENT);
     if (job->width <= 0 || job->height <= 0 || job->width > 16384 || job->height > 16384 || job->spp == 0 || job->surfaces == 0 ||
-        job->depth < 1 || job->depth > 14) {
-        return failed(s, "kdn_serve_submit: image size out of 1..16384, no samples, no surfaces or depth cap out of 1..14",
-                      KDN_E_ARGUMENT);
+        job->depth < 1 || job->depth > 14 || (job->flags & ~1u) != 0) {
+        return failed(s, "kdn_serve_submit: image size out of 1..16384, no samples, no surfaces, depth cap out of 1..14 or "
+                         "unknown flags", KDN_E_ARGUMENT);
     }
     if ((uint64_t)kdn::planeOf(job->width, job->height) > s->maxPaths) {
         return failed(s, "kdn_serve_submit: one sample of the image needs more paths than a round holds", KDN_E_ARGUMENT);
@@ -325,6 +342,7 @@ int kdn_serve_step(kdn_server* s, kdn_serve_stats* stats) {
                     const kdn::Resident& res = s->scenes[j.key];
                     jobIndex[r] = (int)jobs.size();
                     jobs.push_back(kdn::jobOf(res.ctx->c, j.desc.width, j.desc.height, j.desc.seed, j.desc.surfaces, j.desc.min_depth));
+                    jobs.back().hg = (const float*)j.hg.ptr;
                     owners.push_back(j.id);
                 }
                 slices.push_back(kdn::SliceDev{(uint32_t)jobIndex[r], nextOf[r], spp, (uint32_t)paths, j.accum, j.accumSq, j.accumN});

@@ -162,3 +162,60 @@ def test_glass_job_fails_on_the_device(device):
         server.step()
         assert server.status(j).state == k.JOB_FAILED
         assert "glass" in server.job_error(j)
+
+
+# guided jobs: (width, height, spp, seed) of the highlight room with the plain guide
+GUIDED = [(64, 48, 8, 3), (96, 72, 12, 4)]
+
+
+@pytest.fixture(scope="module")
+def room():
+    return k.Scene(scenes.highlight_room())
+
+
+def test_guided_one_slice_job_equals_the_guided_pixel_pass(device, room):
+    """A served job with the plain guide: its survey at admission gives
+    the guides kdn_highlight_guides gives, and its one slice the pixel
+    pass of pass flag bit 5, to the bit."""
+    w, h, spp, seed = GUIDED[0]
+    with k.Server(device, max_paths=1 << 23) as server:
+        j = server.submit(room, w, h, spp, seed=seed, surfaces=6, depth=6, guided=True)
+        server.run()
+        assert server.status(j).state == k.JOB_DONE, server.job_error(j)
+        got = server.result(j)
+    lower, size = room.region(6)
+    device.upload(room)
+    device.build_octree(lower, size, 6)
+    st = device.highlight_guides(w, h)
+    assert st.guided_pixels > 0
+    device.render_pass(w, h, spp, seed=seed, surfaces=6, flags=k.PASS_HIGHLIGHT)
+    want = device.accumulation(w, h)
+    assert _same(got, want)
+    device.render_pass(w, h, spp, seed=seed, surfaces=6)
+    assert not _same(got, device.accumulation(w, h))  # the guide changed the samples
+
+
+def test_guided_jobs_in_a_mix_match_jobs_alone(device, room, scene_set):
+    jobs = [("guided", *GUIDED[0]), ("plain", *JOBS[0][1:5]), ("guided", *GUIDED[1])]
+
+    def submit(server, job):
+        kind, w, h, spp, seed = job
+        if kind == "guided":
+            return server.submit(room, w, h, spp, seed=seed, surfaces=6, depth=6, guided=True)
+        return server.submit(scene_set["plain0"], w, h, spp, seed=seed, surfaces=6, depth=6)
+
+    alone = []
+    with k.Server(device, max_paths=1 << 16) as server:
+        for job in jobs:
+            j = submit(server, job)
+            server.run([j])
+            assert server.status(j).state == k.JOB_DONE, server.job_error(j)
+            alone.append(server.result(j))
+    with k.Server(device, max_paths=1 << 16) as server:
+        ids = [submit(server, jobs[0]), submit(server, jobs[1])]
+        server.step()
+        ids.append(submit(server, jobs[2]))
+        server.run()
+        for i, j in enumerate(ids):
+            assert server.status(j).state == k.JOB_DONE, server.job_error(j)
+            assert _same(server.result(j), alone[i]), i
