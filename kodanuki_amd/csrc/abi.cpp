@@ -236,6 +236,7 @@ void releaseAll(Ctx& c) {
     freeBuf(c.hguides);
     freeBuf(c.sort_tmp);
     c.hguide_width = c.hguide_height = 0;
+    c.hguide_deep = 0;
     if (c.materials_dev) hipFree(c.materials_dev);
     if (c.lamps_dev) hipFree(c.lamps_dev);
     c.materials_dev = c.lamps_dev = nullptr;
@@ -437,6 +438,7 @@ int kdn_upload_scene(kdn_ctx* ctx, const kdn_scene* s) {
     c.tree.built = false;
     kdn::freeGuide(c);  // the guides belong to one scene
     c.hguide_width = c.hguide_height = 0;
+    c.hguide_deep = 0;
     std::vector<kdn_material> mats = s->scene.materials;
     const uint32_t fallback = (uint32_t)mats.size();
     kdn_material grey{};

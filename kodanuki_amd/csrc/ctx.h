@@ -78,6 +78,7 @@ struct Ctx {
     DevBuf sort_tmp;                // the ray queues' sort's temporary storage
     DevBuf hguides;                 // the plain guide: a packed guide per pixel
     int hguide_width = 0, hguide_height = 0;  // the image they were made for (0: none)
+    size_t hguide_deep = 0;         // byte offset of the deep guides' filing in hguides (0: none)
 };
 
 // Glass scattering is held off the device: renders with glass faulted the
@@ -107,7 +108,7 @@ bool guideReady(const Ctx& c);
 void freeGuide(Ctx& c);
 
 // hguide.hip: the plain guide of every pixel of a width x height image
-bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats, DevBuf& out, DevBuf& work);
+bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats, DevBuf& out, DevBuf& work, size_t* deepOffset);
 
 // sort.hip: n (key, value) pairs sorted by the key's low endBit bits;
 // *sorted is vals or valsAlt, whichever holds the sorted values

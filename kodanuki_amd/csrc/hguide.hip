@@ -42,14 +42,10 @@ namespace {
 
 constexpr int kHgBlock = 256;
 constexpr int kSurveyDraws = 128;
-constexpr int kSpots = 16;           // highlights a survey point keeps
-constexpr int kSpotFloats = 15;      // kept, lamp, point 3, spread, width, distance, stand material, importance, leaf bits, material,
-                                     // the surface's normal there (3, facing the stand)
-constexpr float kShinyWidth = 0.125f;
-constexpr float kSameSpot = 0.05f;
-constexpr float kMoved = 0.1f;       // the farthest a highlight walked again from a pixel may move
-constexpr double kGuideEfficiency = 0.3;
-constexpr float kLeastShare = 0.02f, kMostShare = 0.5f;
+constexpr int kSpots = kHgSpots;
+constexpr int kSpotFloats = kHgSpotFloats;
+constexpr float kShinyWidth = kHgShinyWidth;
+constexpr float kSameSpot = kHgSameSpot;
 
 struct HgLens {
     double corner[3], across[3], down[3];
@@ -75,26 +71,6 @@ KDN_D const MaterialDev& mat_of(const HgScene& S, uint32_t leaf, uint32_t* mi) {
     *mi = leaf != KDN_NO_LEAF ? S.leafMaterial[leaf] : S.nMaterials - 1;
     if (*mi >= S.nMaterials) *mi = S.nMaterials - 1;
     return S.materials[*mi];
-}
-
-KDN_D float lamp_glare(const LampDev& L) {
-    // luminance of the power over the area (lampGlare, guide.cpp:262-273):
-    // radiance is power / area
-    return 0.2126f * L.radiance[0] + 0.7152f * L.radiance[1] + 0.0722f * L.radiance[2];
-}
-
-KDN_D V3 lamp_centre(const LampDev& L) {
-    return v3(L.at[0] + 0.5f * (L.along[0] + L.across[0]), L.at[1] + 0.5f * (L.along[1] + L.across[1]),
-              L.at[2] + 0.5f * (L.along[2] + L.across[2]));
-}
-
-KDN_D Lanes grey_lanes() {
-    Lanes w;
-    for (int l = 0; l < kLanes; ++l) {
-        w.nm[l] = 550;
-        w.density[l] = 1;
-    }
-    return w;
 }
 
 // A stand: where a ray from the eye through (x, y) first meets a rough
@@ -166,7 +142,7 @@ __global__ void k_hg_draws(HgScene S, const float* stands, uint32_t nPoints, kdn
     if (s[0] == 1) {
         const MaterialDev& m = S.materials[(uint32_t)s[10]];
         const V3 x = v3(s[1], s[2], s[3]), n = v3(s[4], s[5], s[6]), out = v3(s[7], s[8], s[9]);
-        const Scatter sc = rough_scatter(m, grey_lanes(), n, out, survey_number(point, draw, 0), survey_number(point, draw, 1),
+        const Scatter sc = rough_scatter(m, hg_grey_lanes(), n, out, survey_number(point, draw, 0), survey_number(point, draw, 1),
                                          survey_number(point, draw, 2));
         if (sc.found) {
             const V3 o = x + n * lift_at(x);
@@ -215,7 +191,7 @@ __global__ void k_hg_starts(HgScene S, const float* stands, const kdn_ray* rays,
             float best = 0;
             for (uint32_t j = 0; j < S.nLamps; ++j) {
                 const LampDev& L = S.lamps[j];
-                const V3 centre = lamp_centre(L);
+                const V3 centre = hg_lamp_centre(L);
                 const float above = dot(centre - x, nv);
                 if (!(above > 0)) continue;
                 const V3 image = centre - nv * (2 * above);
@@ -228,7 +204,7 @@ __global__ void k_hg_starts(HgScene S, const float* stands, const kdn_ray* rays,
                 if (!(-dot(v3(L.facing[0], L.facing[1], L.facing[2]), toward) > 0)) continue;
                 const float rho = 0.5f * __builtin_sqrtf(L.area) / dl;
                 const float sigma = fmax_((2 * m.width + rho) * dl / (D + dl), 1e-3f);
-                const float importance = lamp_glare(L) * sigma * sigma;
+                const float importance = hg_lamp_glare(L) * sigma * sigma;
                 if (importance > best) {
                     best = importance;
                     const V3 o = stand + sn * lift_at(stand);
@@ -276,7 +252,7 @@ __global__ void k_hg_candidates(HgScene S, const kdn_ray* aims, const kdn_hit* h
     V3 nv = v3(h.normal[0], h.normal[1], h.normal[2]);
     if (dot(nv, d) > 0) nv = -nv;
     const V3 mirror = mirrored(-d, nv);
-    const V3 toward = lamp_centre(L) - x;
+    const V3 toward = hg_lamp_centre(L) - x;
     const float dl = length(toward), D = length(x - o);
     if (!(dl > 1e-4f)) return;
     const V3 dir = toward * (1.0f / dl);
@@ -291,7 +267,7 @@ __global__ void k_hg_candidates(HgScene S, const kdn_ray* aims, const kdn_hit* h
     c[4] = x.z;
     c[5] = sigma * sigma;
     c[7] = D;
-    c[9] = lamp_glare(L) * sigma * sigma;
+    c[9] = hg_lamp_glare(L) * sigma * sigma;
     c[10] = __builtin_bit_cast(float, h.leaf);
     c[12] = nv.x;
     c[13] = nv.y;
@@ -326,14 +302,6 @@ __global__ void k_hg_spots(const float* cand, uint32_t nPoints, float* spots) {
         if (kept < kSpots) ++kept;
     }
     out[0] = (float)kept;
-}
-
-KDN_D float responded(const MaterialDev& m, V3 n, V3 out, V3 in) {
-    const float facing = dot(n, in);
-    if (facing <= 0) return 0.0f;
-    float f[kLanes];
-    rough_value(m, grey_lanes(), n, out, in, f);
-    return 0.25f * (f[0] + f[1] + f[2] + f[3]) * facing;
 }
 
 // every pixel's guide (pixelGuideAt, survey.cpp:916-966, as above)
@@ -375,36 +343,12 @@ __global__ void k_hg_pixels(HgScene S, const float* stands, const float* spots, 
         for (int k = 0; k < kept; ++k) {
             const float* h = spot + 1 + k * kSpotFloats;
             if ((uint32_t)h[8] != mi) continue;
-            const uint32_t lamp = (uint32_t)h[1];
-            const LampDev& L = S.lamps[lamp];
-            const V3 centre = lamp_centre(L);
-            // walked again from this stand: the point of the highlight's
-            // tangent plane that mirrors the lamp's centre toward it, kept
-            // when it moves less than kMoved (the plane may end sooner)
-            V3 point = v3(h[2], h[3], h[4]);
-            {
-                const V3 mirrorN = v3(h[12], h[13], h[14]);
-                const float above = dot(centre - point, mirrorN);
-                const V3 image = centre - mirrorN * (2 * above);
-                const float t = dot(point - x, mirrorN) / dot(image - x, mirrorN);
-                if (above > 0 && t > 0 && t < 1) {
-                    const V3 moved = x + (image - x) * t;
-                    if (length(moved - point) < kMoved) point = moved;
-                }
-            }
-            const V3 towards = point - x;
-            const float dist = length(towards), dl = length(centre - point);
-            if (!(dist > 1e-4f && dl > 1e-4f)) continue;
-            const V3 axis = towards * (1.0f / dist);
-            // the spread a flat glossy mirror of the start's width gives
-            // seen from this stand (scaledSpread, guide.cpp:284-287, scales
-            // the survey's by distance)
-            const float rho = 0.5f * __builtin_sqrtf(L.area) / dl;
-            const float sigma = fmax_((2 * h[6] + rho) * dl / (dist + dl), 1e-3f);
-            const float spread = sigma * sigma;
-            if (!(spread < 1)) continue;
-            const float f0 = sigma;
-            const float w = lamp_glare(L) * f0 * f0 * rough_density(m, n, out, axis) * blend[q];
+            HgSeen seen;
+            if (!hg_seen(h, x, n, out, m, S.lamps, S.materials, &seen)) continue;
+            const uint32_t lamp = seen.lamp;
+            const V3 point = seen.point, axis = seen.axis;
+            const float f0 = seen.sigma;
+            const float w = seen.weight * blend[q];
             if (!(w > 0)) continue;
             // the same highlight seen through another survey point adds
             // its weight to the lump already kept
@@ -451,8 +395,7 @@ __global__ void k_hg_pixels(HgScene S, const float* stands, const float* spots, 
             for (int f = 0; f < kHgLumpFloats; ++f) lump[place][f] = rec[f];
             heavy[place] = w;
             lampOf[place] = lamp;
-            const MaterialDev& sm = S.materials[(uint32_t)h[11]];
-            radiance[place] = lamp_glare(L) * (0.2126f * sm.albedo[0] + 0.7152f * sm.albedo[1] + 0.0722f * sm.albedo[2]);
+            radiance[place] = seen.light;
             if (count < kHgLumps) ++count;
         }
     }
@@ -464,35 +407,64 @@ __global__ void k_hg_pixels(HgScene S, const float* stands, const float* spots, 
     double spotsLight = 0;
     for (int k = 0; k < live; ++k) {
         const V3 axis = v3(lump[k][0], lump[k][1], lump[k][2]);
-        spotsLight += (double)responded(m, n, out, axis) * 2 * kPiD * lump[k][11] * lump[k][13] * radiance[k];
+        spotsLight += (double)hg_responded(m, n, out, axis) * 2 * kPiD * lump[k][11] * lump[k][13] * radiance[k];
     }
-    double rest = 0;
-    for (uint32_t j = 0; j < S.nLamps; ++j) {
-        const LampDev& L = S.lamps[j];
-        const V3 toward = lamp_centre(L) - x;
-        const float dl = length(toward);
-        const V3 dir = toward * (1.0f / dl);
-        if (!(-dot(v3(L.facing[0], L.facing[1], L.facing[2]), dir) > 0)) continue;
-        const float size = fmax_(0.5f * __builtin_sqrtf(L.area) / dl, 1e-3f);
-        rest += (double)responded(m, n, out, dir) * kPiD * size * size * lamp_glare(L);
-    }
-    const double spotted = spotsLight / __builtin_sqrt(kGuideEfficiency);
-    const double all = spotted + rest;
-    double share = all > 1e-30 ? spotted / all : 0.0;
-    share = share < kLeastShare ? kLeastShare : (share > kMostShare ? kMostShare : share);
+    const float share = hg_share(spotsLight, m, x, n, out, S.lamps, S.nLamps);
     g[0] = (float)live;
     g[1] = (float)mi;
-    g[2] = (float)share;
+    g[2] = share;
     for (int k = 0; k < live; ++k)
         for (int f = 0; f < kHgLumpFloats; ++f) g[kHgHead + k * kHgLumpFloats + f] = lump[k][f];
 }
 
 }  // namespace
 
+// The survey points that found highlights filed by cell (HgDeepHeader and
+// what it points at), built on the host from the survey's stands and
+// highlights: at most kHgDeepFiled points a cell, those with the most
+// highlights first.
+struct DeepFiling {
+    std::vector<uint32_t> table, list;
+    std::vector<float> place;
+};
+
+static DeepFiling fileDeep(const Ctx& c, const std::vector<float>& stands, const std::vector<float>& spots, uint32_t nPoints) {
+    DeepFiling f;
+    const uint32_t G = kHgDeepGrid;
+    std::vector<std::vector<std::pair<uint32_t, uint32_t>>> cells((size_t)G * G * G);  // (highlights, point)
+    f.place.resize((size_t)nPoints * 3);
+    for (uint32_t p = 0; p < nPoints; ++p) {
+        const float* s = &stands[(size_t)p * kStandFloats];
+        for (int k = 0; k < 3; ++k) f.place[3 * p + k] = s[1 + k];
+        const uint32_t kept = (uint32_t)spots[(size_t)p * kHgPointFloats];
+        if (s[0] != 1 || kept == 0) continue;
+        uint32_t at[3];
+        bool inside = true;
+        for (int k = 0; k < 3; ++k) {
+            const float q = (s[1 + k] - c.tree.lower[k]) * (G / c.tree.size[k]);
+            inside = inside && q >= 0 && q < (float)G;
+            at[k] = inside ? std::min<uint32_t>((uint32_t)q, G - 1) : 0;
+        }
+        if (inside) cells[((size_t)at[2] * G + at[1]) * G + at[0]].push_back({kept, p});
+    }
+    f.table.assign((size_t)G * G * G * 2, 0);
+    for (size_t cell = 0; cell < cells.size(); ++cell) {
+        auto& v = cells[cell];
+        std::stable_sort(v.begin(), v.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+        const uint32_t n = (uint32_t)std::min<size_t>(v.size(), kHgDeepFiled);
+        f.table[2 * cell] = (uint32_t)f.list.size();
+        f.table[2 * cell + 1] = n;
+        for (uint32_t i = 0; i < n; ++i) f.list.push_back(v[i].second);
+    }
+    return f;
+}
+
 // Survey and pixel guides for an image of width x height of the context's
-// scene and camera, into out (kHgFloats a pixel), with work as the
-// survey's scratch.
-bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats, DevBuf& out, DevBuf& work) {
+// scene and camera, into out (kHgFloats a pixel, then the deep guides'
+// filing at byte *deepOffset, 0 for none), with work as the survey's
+// scratch.
+bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats, DevBuf& out, DevBuf& work, size_t* deepOffset) {
+    *deepOffset = 0;
     hipStream_t st = c.stream;
     const kdn_camera& cam = c.scene.camera;
     double fwd[3], rt[3], upv[3], len = 0;
@@ -538,7 +510,15 @@ bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats, DevBuf&
     float* stands = (float*)take((size_t)std::max<size_t>(nPoints, nPixels) * kStandFloats * 4);
     float* cand = (float*)take((size_t)nDraws * kSpotFloats * 4);
     float* spots = (float*)take((size_t)nPoints * (1 + kSpots * kSpotFloats) * 4);
-    if (!ensure(out, (size_t)nPixels * kHgFloats * 4, c.error, false)) return false;
+    const bool surveyed = nPoints > 0 && c.n_lamps > 0;
+    const size_t pixelBytes = ((size_t)nPixels * kHgFloats * 4 + 255) & ~size_t(255);
+    const size_t G3 = (size_t)kHgDeepGrid * kHgDeepGrid * kHgDeepGrid;
+    // the filing's words after its header: table, list, places, highlights
+    constexpr size_t headWords = 16;
+    static_assert(sizeof(HgDeepHeader) <= headWords * 4, "the filing's header fits its 64 bytes");
+    const size_t tableWords = G3 * 2, listWords = nPoints, placeWords = (size_t)nPoints * 3, spotWords = (size_t)nPoints * kHgPointFloats;
+    const size_t deepBytes = surveyed ? (headWords + tableWords + listWords + placeWords + spotWords) * 4 : 0;
+    if (!ensure(out, pixelBytes + deepBytes, c.error, false)) return false;
     auto grid = [](uint32_t n) { return dim3((n + kHgBlock - 1) / kHgBlock); };
     hipEvent_t ev[2];
     KDN_HIP(hipEventCreate(&ev[0]));
@@ -557,6 +537,31 @@ bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats, DevBuf&
         hipLaunchKernelGGL(k_hg_candidates, grid(nDraws), dim3(kHgBlock), 0, st, S, (const kdn_ray*)aims, (const kdn_hit*)hits,
                            nDraws, cand);
         hipLaunchKernelGGL(k_hg_spots, grid(nPoints), dim3(kHgBlock), 0, st, (const float*)cand, nPoints, spots);
+        // the deep guides' filing, from the survey's stands (before the
+        // pixels' stands take their place) and highlights
+        std::vector<float> hostStands((size_t)nPoints * kStandFloats), hostSpots((size_t)nPoints * kHgPointFloats);
+        KDN_HIP(hipMemcpyAsync(hostStands.data(), stands, hostStands.size() * 4, hipMemcpyDeviceToHost, st));
+        KDN_HIP(hipMemcpyAsync(hostSpots.data(), spots, hostSpots.size() * 4, hipMemcpyDeviceToHost, st));
+        KDN_HIP(hipStreamSynchronize(st));
+        const DeepFiling filing = fileDeep(c, hostStands, hostSpots, nPoints);
+        HgDeepHeader head{};
+        head.grid = kHgDeepGrid;
+        for (int k = 0; k < 3; ++k) {
+            head.lower[k] = c.tree.lower[k];
+            head.inv[k] = kHgDeepGrid / c.tree.size[k];
+        }
+        head.points = nPoints;
+        head.table = (uint32_t)headWords;
+        head.list = (uint32_t)(headWords + tableWords);
+        head.place = (uint32_t)(headWords + tableWords + listWords);
+        head.spots = (uint32_t)(headWords + tableWords + listWords + placeWords);
+        uint32_t* words = (uint32_t*)((char*)out.ptr + pixelBytes);
+        KDN_HIP(hipMemcpy(words, &head, sizeof head, hipMemcpyHostToDevice));
+        KDN_HIP(hipMemcpy(words + head.table, filing.table.data(), filing.table.size() * 4, hipMemcpyHostToDevice));
+        if (!filing.list.empty()) KDN_HIP(hipMemcpy(words + head.list, filing.list.data(), filing.list.size() * 4, hipMemcpyHostToDevice));
+        KDN_HIP(hipMemcpy(words + head.place, filing.place.data(), filing.place.size() * 4, hipMemcpyHostToDevice));
+        KDN_HIP(hipMemcpyAsync(words + head.spots, spots, spotWords * 4, hipMemcpyDeviceToDevice, st));
+        *deepOffset = pixelBytes;
     }
     // the pixels' stands, then their guides
     hipLaunchKernelGGL(k_hg_rays, grid(nPixels), dim3(kHgBlock), 0, st, L, width, height, 1, 0, width, height, rays);
@@ -585,7 +590,7 @@ bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats, DevBuf&
             }
         }
         std::vector<float> sp((size_t)nPoints * (1 + kSpots * kSpotFloats));
-        if (nPoints > 0 && c.n_lamps > 0) {
+        if (surveyed) {
             KDN_HIP(hipMemcpy(sp.data(), spots, sp.size() * 4, hipMemcpyDeviceToHost));
             for (uint32_t q = 0; q < nPoints; ++q) found += (uint32_t)sp[(size_t)q * (1 + kSpots * kSpotFloats)];
         }
@@ -618,7 +623,7 @@ extern "C" int kdn_highlight_guides(kdn_ctx* ctx, int32_t width, int32_t height,
     }
     hipSetDevice(c.device);
     c.hguide_width = c.hguide_height = 0;
-    if (!kdn::hguideBuild(c, width, height, stats, c.hguides, c.scratch[5])) return KDN_E_DEVICE;
+    if (!kdn::hguideBuild(c, width, height, stats, c.hguides, c.scratch[5], &c.hguide_deep)) return KDN_E_DEVICE;
     c.hguide_width = width;
     c.hguide_height = height;
     return KDN_OK;

@@ -443,6 +443,7 @@ bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
             return false;
         }
         jobs[0].hg = (const float*)c.hguides.ptr;
+        jobs[0].hg_deep = c.hguide_deep ? (const HgDeepHeader*)((const char*)c.hguides.ptr + c.hguide_deep) : nullptr;
     }
     const uint32_t plane = jobs[0].plane;
     const size_t maxPaths = c.max_paths ? c.max_paths : (size_t)1 << 23;

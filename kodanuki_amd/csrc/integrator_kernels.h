@@ -50,8 +50,10 @@ struct JobDev {
     const MaterialDev* materials;
     const uint32_t* leaf_material;
     // the plain guide: every pixel's packed guide of Gaussian lumps
-    // (kHgFloats each, pixel-major; null: none), steering the first vertex
+    // (kHgFloats each, pixel-major; null: none), steering the first vertex,
+    // and the survey's highlights filed by cell, steering deeper vertices
     const float* hg;
+    const HgDeepHeader* hg_deep;
 };
 
 // A slice of a round: samples [first_sample, first_sample + spp) of every
@@ -299,6 +301,37 @@ KDN_D Scatter highlight_scatter(const MaterialDev& m, const Lanes& w, V3 n, V3 o
     return s;
 }
 
+// The same mixture at a deeper vertex, with its deep guide's lumps.
+KDN_D Scatter deep_scatter(const MaterialDev& m, const Lanes& w, V3 n, V3 out, const HgDeepView& dv, V3 x, uint32_t mi,
+                           const LampDev* lamps, const MaterialDev* materials, float uPart, float uPick, float uLobe, float u, float v) {
+    Scatter s;
+    s.found = false;
+    s.exact = false;
+    s.through = false;
+    s.collapse = false;
+    if (dot(n, out) <= 0) return s;
+    V3 in;
+    if (uPart < dv.share) {
+        in = hg_deep_draw(dv, x, n, out, mi, m, lamps, materials, uPick, u, v);
+        if (!__builtin_isfinite(in.x + in.y + in.z) || dot(in, in) == 0) return s;
+    } else {
+        const Scatter b = rough_scatter(m, w, n, out, uLobe, u, v);
+        if (!b.found) return s;
+        in = b.in;
+    }
+    const float arriving = dot(n, in);
+    if (arriving <= 0) return s;
+    const float density = dv.share * hg_deep_density(dv, x, n, out, mi, m, lamps, materials, in) + (1 - dv.share) * rough_density(m, n, out, in);
+    if (!(density > 0)) return s;
+    float f[kLanes];
+    rough_value(m, w, n, out, in, f);
+    s.found = true;
+    s.in = in;
+    s.density = density;
+    for (int l = 0; l < kLanes; ++l) s.weight[l] = f[l] * (arriving / density);
+    return s;
+}
+
 KDN_D float lift_at(V3 x) {
     return 8e-5f + 8e-6f * fmax_(fmax_(__builtin_fabsf(x.x), __builtin_fabsf(x.y)), __builtin_fabsf(x.z));
 }
@@ -414,6 +447,12 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
             if (g[0] > 0 && (uint32_t)g[1] == mi && m.kind != KDN_GLASS) hg = g;
         }
         const float hgShare = hg ? hg[2] : 0.0f;
+        // deeper vertices of a broad surface: the survey's highlights filed
+        // in their cell (the deep guide)
+        HgDeepView deepView;
+        const bool deep = !hg && !guided && J.hg_deep != nullptr && depth >= 1 && !inside && m.kind != KDN_GLASS &&
+                          m.width > kHgShinyWidth &&
+                          hg_deep_view(J.hg_deep, x, n, out, mi, m, J.lamps, J.n_lamps, J.materials, &deepView);
         // aim at one lamp (aimedFrom, trace.cpp:896-947), MIS against meeting
         if (!exact && J.n_lamps > 0) {
             const float uPick = number(stream, sample, depth, Pick);
@@ -434,6 +473,8 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
                 const float pdfL = L.pick * dist * dist / (cosL * L.area);
                 const float pdfS = guided ? share * mixture_pdf(lobes, in) + (1 - share) * rough_density(m, n, out, in)
                                    : hg     ? hgShare * hg_density(hg, x, n, out, in) + (1 - hgShare) * rough_density(m, n, out, in)
+                                   : deep   ? deepView.share * hg_deep_density(deepView, x, n, out, mi, m, J.lamps, J.materials, in) +
+                                                (1 - deepView.share) * rough_density(m, n, out, in)
                                             : rough_density(m, n, out, in);
                 const float wt = power_weight(pdfL, pdfS);
                 float c[kLanes];
@@ -469,6 +510,8 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
                                                          guide_number(stream, sample, depth, 1), uLobe, du, dv)
                                : hg     ? highlight_scatter(m, w, n, out, hg, x, hgShare, guide_number(stream, sample, depth, 0),
                                                             guide_number(stream, sample, depth, 1), uLobe, du, dv)
+                               : deep   ? deep_scatter(m, w, n, out, deepView, x, mi, J.lamps, J.materials, guide_number(stream, sample, depth, 0),
+                                                       guide_number(stream, sample, depth, 1), uLobe, du, dv)
                                         : rough_scatter(m, w, n, out, uLobe, du, dv);
             if (sc.found) {
                 const float side = dot(sc.in, n);

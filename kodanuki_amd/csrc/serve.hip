@@ -55,7 +55,8 @@ struct Job {
     double* accum = nullptr;
     double* accumSq = nullptr;
     uint32_t* accumN = nullptr;
-    DevBuf hg;             // the plain guide's pixel guides (flags bit 0)
+    DevBuf hg;             // the plain guide's pixel guides and deep filing (flags bit 0)
+    size_t hgDeep = 0;     // the filing's byte offset in hg (0: none)
     uint64_t bytes = 0;
 };
 
@@ -162,7 +163,10 @@ int tooBig(kdn_server& s, uint64_t need) {
 int admit(kdn_server& s, kdn::Job& j) {
     const uint64_t px = (uint64_t)j.desc.width * j.desc.height;
     const bool guided = (j.desc.flags & 1) != 0;
-    const uint64_t bytes = px * (6 * sizeof(double) + sizeof(uint32_t) + (guided ? kdn::kHgFloats * sizeof(float) : 0));
+    // (a guided job's pixel guides, and its survey points' filing: about
+    // 1 KB a point, one point to 64 pixels, and the 32^3 cell table)
+    const uint64_t bytes = px * (6 * sizeof(double) + sizeof(uint32_t) + (guided ? kdn::kHgFloats * sizeof(float) + 16 : 0)) +
+                           (guided ? (1u << 18) : 0);
     bool built = false;
     if (!s.scenes.count(j.key)) {
         auto known = s.sceneBytes.find(j.key);
@@ -207,7 +211,7 @@ int admit(kdn_server& s, kdn::Job& j) {
         // the survey of the job's image on its scene's context, in the
         // server's own scratch (kept for the next survey, and counted)
         kdn::Ctx& sc = res.ctx->c;
-        const bool ok = kdn::hguideBuild(sc, j.desc.width, j.desc.height, nullptr, j.hg, c.scratch[5]);
+        const bool ok = kdn::hguideBuild(sc, j.desc.width, j.desc.height, nullptr, j.hg, c.scratch[5], &j.hgDeep);
         hipSetDevice(c.device);
         if (!ok) {
             const std::string message = "serving: the plain guide's survey failed: " + sc.error;
@@ -233,6 +237,7 @@ void releaseJob(kdn_server& s, kdn::Job& j) {
     j.accumN = nullptr;
     if (j.hg.ptr) hipFree(j.hg.ptr);
     j.hg = kdn::DevBuf{};
+    j.hgDeep = 0;
     j.bytes = 0;
 }
 
@@ -343,6 +348,7 @@ int kdn_serve_step(kdn_server* s, kdn_serve_stats* stats) {
                     jobIndex[r] = (int)jobs.size();
                     jobs.push_back(kdn::jobOf(res.ctx->c, j.desc.width, j.desc.height, j.desc.seed, j.desc.surfaces, j.desc.min_depth));
                     jobs.back().hg = (const float*)j.hg.ptr;
+                    jobs.back().hg_deep = j.hgDeep ? (const kdn::HgDeepHeader*)((const char*)j.hg.ptr + j.hgDeep) : nullptr;
                     owners.push_back(j.id);
                 }
                 slices.push_back(kdn::SliceDev{(uint32_t)jobIndex[r], nextOf[r], spp, (uint32_t)paths, j.accum, j.accumSq, j.accumN});
