@@ -121,8 +121,12 @@ typedef struct kdn_material {
     float strength;
 } kdn_material;
 
+/* Light kinds (source/sdf/scene.h:155-181; the moon is not loaded). */
 #define KDN_LIGHT_POINT 0
 #define KDN_LIGHT_QUAD 1
+#define KDN_LIGHT_DIRECTIONAL 2
+#define KDN_LIGHT_SPHERE 3
+#define KDN_LIGHT_SUN 4
 
 /* A lamp (source/sdf/scene.h:481-528, quad and point kinds). */
 typedef struct kdn_light {
@@ -131,6 +135,9 @@ typedef struct kdn_light {
     float along[3];
     float across[3];
     float power[3];
+    float direction[3];   /* toward a sun or a directional light */
+    float radius;         /* a sphere's */
+    float angle;          /* a sun's full angle across, radians */
 } kdn_light;
 
 /* A leaf as placed (source/sdf/scene.h:237-288): its primitive, the

@@ -50,7 +50,10 @@ class Material(C.Structure):
 
 class Light(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("at", C.c_float * 3), ("along", C.c_float * 3), ("across", C.c_float * 3),
-                ("power", C.c_float * 3)]
+                ("power", C.c_float * 3), ("direction", C.c_float * 3), ("radius", C.c_float), ("angle", C.c_float)]
+
+
+LIGHT_POINT, LIGHT_QUAD, LIGHT_DIRECTIONAL, LIGHT_SPHERE, LIGHT_SUN = 0, 1, 2, 3, 4
 
 
 class Leaf(C.Structure):

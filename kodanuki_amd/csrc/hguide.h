@@ -213,12 +213,23 @@ KDN_D float hg_share(double spotsLight, const MaterialDev& m, V3 x, V3 n, V3 out
     double rest = 0;
     for (uint32_t j = 0; j < nLamps; ++j) {
         const LampDev& L = lamps[j];
-        const V3 toward = hg_lamp_centre(L) - x;
-        const float dl = length(toward);
-        const V3 dir = toward * (1.0f / dl);
-        if (!(-dot(v3(L.facing[0], L.facing[1], L.facing[2]), dir) > 0)) continue;
-        const float size = fmax_(0.5f * __builtin_sqrtf(L.area) / dl, 1e-3f);
-        rest += (double)hg_responded(m, n, out, dir) * kPiD * size * size * hg_lamp_glare(L);
+        const float power = 0.2126f * L.power[0] + 0.7152f * L.power[1] + 0.0722f * L.power[2];
+        if (L.kind == KDN_LIGHT_QUAD) {
+            const V3 toward = hg_lamp_centre(L) - x;
+            const float dl = length(toward);
+            const V3 dir = toward * (1.0f / dl);
+            if (!(-dot(v3(L.facing[0], L.facing[1], L.facing[2]), dir) > 0)) continue;
+            const float size = fmax_(0.5f * __builtin_sqrtf(L.area) / dl, 1e-3f);
+            rest += (double)hg_responded(m, n, out, dir) * kPiD * size * size * hg_lamp_glare(L);
+        } else if (L.kind == KDN_LIGHT_POINT || L.kind == KDN_LIGHT_SPHERE) {
+            // a point's or a sphere's irradiance: power / distance^2
+            const V3 toward = v3(L.at[0], L.at[1], L.at[2]) - x;
+            const float dl2 = fmax_(dot(toward, toward), 1e-12f);
+            rest += (double)hg_responded(m, n, out, toward * (1.0f / __builtin_sqrtf(dl2))) * power / dl2;
+        } else {
+            // a directional light's or a sun's irradiance: its power
+            rest += (double)hg_responded(m, n, out, v3(L.direction[0], L.direction[1], L.direction[2])) * power;
+        }
     }
     const double spotted = spotsLight / __builtin_sqrt(kHgEfficiency);
     const double all = spotted + rest;

@@ -191,6 +191,7 @@ __global__ void k_hg_starts(HgScene S, const float* stands, const kdn_ray* rays,
             float best = 0;
             for (uint32_t j = 0; j < S.nLamps; ++j) {
                 const LampDev& L = S.lamps[j];
+                if (L.kind != KDN_LIGHT_QUAD) continue;  // the survey mirrors quad lamps
                 const V3 centre = hg_lamp_centre(L);
                 const float above = dot(centre - x, nv);
                 if (!(above > 0)) continue;

@@ -381,6 +381,41 @@ KDN_D bool meet_quad(const LampDev& L, V3 o, V3 d, float tMax, LampHit* h) {
     return true;
 }
 
+// Where a ray meets a sphere lamp before tMax (meetSphere,
+// light.cpp:436-452): its nearer crossing, or the farther from inside.
+KDN_D bool meet_sphere(const LampDev& L, V3 o, V3 d, float tMax, float* t) {
+    const V3 off = o - v3(L.at[0], L.at[1], L.at[2]);
+    const float along = dot(off, d);
+    const float under = along * along - (dot(off, off) - L.radius * L.radius);
+    if (!(under > 0)) return false;
+    const float root = __builtin_sqrtf(under);
+    const float first = -along - root;
+    const float reach = first <= 1e-9f ? -along + root : first;
+    if (!(reach > 1e-9f) || !(reach < tMax)) return false;
+    *t = reach;
+    return true;
+}
+
+// the cosine of the cone a sphere of radius r fills seen from distance
+// (coneCosine, light.cpp:303-306), and a cone's uniform density
+// (coneDensity, :308-311)
+KDN_D float cone_cosine(float r, float distance) {
+    const float sideways = r / distance;
+    return __builtin_sqrtf(fmax_(1 - sideways * sideways, 0.0f));
+}
+KDN_D float cone_density(float cosine) {
+    const float cone = 2 * kPi * (1 - cosine);
+    return cone > 1e-12f ? 1 / cone : 0.0f;
+}
+
+// uniformCone (math/random.cpp:41-46)
+KDN_D V3 uniform_cone(V3 axis, float cosine, float u, float v) {
+    const float height = 1 - u * (1 - cosine);
+    const float ring = __builtin_sqrtf(fmax_(1 - height * height, 0.0f));
+    const float angle = 2 * kPi * v;
+    return to_world(basis_around(axis), v3(ring * f_cos(angle), ring * f_sin(angle), height));
+}
+
 KDN_D void sky_radiance(const float* zenith, const float* horizon, const float* ground, V3 d, const Lanes& w, float out[kLanes]) {
     const float height = d.z;
     const float* far = height >= 0 ? zenith : ground;

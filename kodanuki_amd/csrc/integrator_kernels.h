@@ -388,15 +388,29 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
     const float pdfBefore = s.pdf[i];
     const bool training = P.recs != nullptr && i % P.train_stride == 0;
 
-    // lamps met along the ray (metAlong, trace.cpp:352-374), MIS against aiming
+    // lamps met along the ray (metAlong, trace.cpp:352-374), MIS against
+    // aiming: quads and spheres before the hit, suns where the ray escapes
     if (!inside) {
         for (uint32_t j = 0; j < J.n_lamps; ++j) {
             const LampDev& L = J.lamps[j];
-            LampHit lh;
-            if (!meet_quad(L, o, d, tHit, &lh)) continue;
+            float pdfL;
+            if (L.kind == KDN_LIGHT_QUAD) {
+                LampHit lh;
+                if (!meet_quad(L, o, d, tHit, &lh)) continue;
+                pdfL = L.pick * lh.t * lh.t / (lh.cosine * L.area);
+            } else if (L.kind == KDN_LIGHT_SPHERE) {
+                float t;
+                if (!meet_sphere(L, o, d, tHit, &t)) continue;
+                const float away = length(v3(L.at[0], L.at[1], L.at[2]) - o);
+                pdfL = away > L.radius ? L.pick * cone_density(cone_cosine(L.radius, away)) : 0.0f;
+            } else if (L.kind == KDN_LIGHT_SUN) {
+                if (hit.kind != KDN_HIT_MISSED || dot(d, v3(L.direction[0], L.direction[1], L.direction[2])) < L.cosine) continue;
+                pdfL = L.pick * cone_density(L.cosine);
+            } else {
+                continue;  // points and directional lights are met by aiming only
+            }
             float Le[kLanes];
             spectrum_of(L.radiance, w, Le);
-            const float pdfL = L.pick * lh.t * lh.t / (lh.cosine * L.area);
             const float wt = exactBefore ? 1.0f : power_weight(pdfBefore, pdfL);
             for (int l = 0; l < kLanes; ++l) rad[l] += thr[l] * Le[l] * wt;
         }
@@ -453,34 +467,84 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
         const bool deep = !hg && !guided && J.hg_deep != nullptr && depth >= 1 && !inside && m.kind != KDN_GLASS &&
                           m.width > kHgShinyWidth &&
                           hg_deep_view(J.hg_deep, x, n, out, mi, m, J.lamps, J.n_lamps, J.materials, &deepView);
-        // aim at one lamp (aimedFrom, trace.cpp:896-947), MIS against meeting
+        // aim at one lamp (aimedFrom, trace.cpp:896-947; aimLamp,
+        // light.cpp:359-370), MIS against meeting
         if (!exact && J.n_lamps > 0) {
             const float uPick = number(stream, sample, depth, Pick);
             uint32_t j = 0;
             while (j + 1 < J.n_lamps && uPick >= J.lamps[j].pick_cdf) ++j;
             const LampDev& L = J.lamps[j];
             const float u = number(stream, sample, depth, LightU), v = number(stream, sample, depth, LightV);
-            const V3 y = v3(L.at[0], L.at[1], L.at[2]) + v3(L.along[0], L.along[1], L.along[2]) * u + v3(L.across[0], L.across[1], L.across[2]) * v;
-            const V3 toward = y - x;
-            const float dist = length(toward);
-            const V3 in = toward * (1.0f / dist);
-            const float cosL = -dot(v3(L.facing[0], L.facing[1], L.facing[2]), in);
+            const V3 at = v3(L.at[0], L.at[1], L.at[2]);
+            // the direction, the distance to the lamp along it, what arrives
+            // over the chance of the direction (radiance / density, or the
+            // singular lights' irradiance), and the density (0: singular)
+            V3 in = v3(0, 0, 1);
+            float dist = 0, density = 0, arriving[3] = {0, 0, 0};
+            bool aimed = false;
+            if (L.kind == KDN_LIGHT_QUAD) {
+                const V3 y = at + v3(L.along[0], L.along[1], L.along[2]) * u + v3(L.across[0], L.across[1], L.across[2]) * v;
+                const V3 toward = y - x;
+                dist = length(toward);
+                in = toward * (1.0f / dist);
+                const float cosL = -dot(v3(L.facing[0], L.facing[1], L.facing[2]), in);
+                if (cosL > 0) {
+                    density = dist * dist / (cosL * L.area);
+                    for (int k = 0; k < 3; ++k) arriving[k] = L.radiance[k] / density;
+                    aimed = true;
+                }
+            } else if (L.kind == KDN_LIGHT_POINT) {
+                const V3 toward = at - x;
+                dist = length(toward);
+                if (dist > 1e-9f) {
+                    in = toward * (1.0f / dist);
+                    for (int k = 0; k < 3; ++k) arriving[k] = L.power[k] / (dist * dist);
+                    aimed = true;
+                }
+            } else if (L.kind == KDN_LIGHT_DIRECTIONAL) {
+                in = v3(L.direction[0], L.direction[1], L.direction[2]);
+                dist = kInf;
+                for (int k = 0; k < 3; ++k) arriving[k] = L.power[k];
+                aimed = true;
+            } else if (L.kind == KDN_LIGHT_SPHERE) {
+                const V3 toward = at - x;
+                const float away = length(toward);
+                if (away > 1e-9f && away > L.radius) {
+                    const V3 centre = toward * (1.0f / away);
+                    const float cosine = cone_cosine(L.radius, away);
+                    in = uniform_cone(centre, cosine, u, v);
+                    const float height = dot(in, centre);
+                    const float ring = __builtin_sqrtf(fmax_(1 - height * height, 0.0f));
+                    dist = fmax_(away * height - __builtin_sqrtf(fmax_(L.radius * L.radius - away * away * ring * ring, 0.0f)), 0.0f);
+                    density = cone_density(cosine);
+                    for (int k = 0; k < 3; ++k) arriving[k] = L.power[k] * (2 * (1 - cosine) / (L.radius * L.radius));
+                    aimed = density > 0;
+                }
+            } else if (L.kind == KDN_LIGHT_SUN) {
+                in = uniform_cone(v3(L.direction[0], L.direction[1], L.direction[2]), L.cosine, u, v);
+                dist = kInf;
+                density = cone_density(L.cosine);
+                for (int k = 0; k < 3; ++k) arriving[k] = L.radiance[k] / density;
+                aimed = density > 0;
+            }
             const float cosS = dot(n, in);
-            if (cosL > 0 && cosS > 0 && dist > 2 * lift) {
+            if (aimed && cosS > 0 && dist > 2 * lift) {
                 float f[kLanes], Le[kLanes];
                 rough_value(m, w, n, out, in, f);
-                spectrum_of(L.radiance, w, Le);
-                const float pdfL = L.pick * dist * dist / (cosL * L.area);
-                const float pdfS = guided ? share * mixture_pdf(lobes, in) + (1 - share) * rough_density(m, n, out, in)
-                                   : hg     ? hgShare * hg_density(hg, x, n, out, in) + (1 - hgShare) * rough_density(m, n, out, in)
-                                   : deep   ? deepView.share * hg_deep_density(deepView, x, n, out, mi, m, J.lamps, J.materials, in) +
-                                                (1 - deepView.share) * rough_density(m, n, out, in)
-                                            : rough_density(m, n, out, in);
-                const float wt = power_weight(pdfL, pdfS);
+                spectrum_of(arriving, w, Le);
+                float wt = 1.0f;  // singular lights: aiming alone finds them
+                if (density > 0) {
+                    const float pdfS = guided ? share * mixture_pdf(lobes, in) + (1 - share) * rough_density(m, n, out, in)
+                                       : hg   ? hgShare * hg_density(hg, x, n, out, in) + (1 - hgShare) * rough_density(m, n, out, in)
+                                       : deep ? deepView.share * hg_deep_density(deepView, x, n, out, mi, m, J.lamps, J.materials, in) +
+                                                    (1 - deepView.share) * rough_density(m, n, out, in)
+                                              : rough_density(m, n, out, in);
+                    wt = power_weight(L.pick * density, pdfS);
+                }
                 float c[kLanes];
                 float any = 0;
                 for (int l = 0; l < kLanes; ++l) {
-                    c[l] = thr[l] * f[l] * cosS * Le[l] / pdfL * wt;
+                    c[l] = thr[l] * f[l] * cosS * Le[l] / L.pick * wt;
                     any += c[l];
                 }
                 if (any > 0) {
@@ -492,7 +556,7 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
                     sr.direction[0] = in.x;
                     sr.direction[1] = in.y;
                     sr.direction[2] = in.z;
-                    sr.far_ = dist - lift;
+                    sr.far_ = dist < kInf ? dist - lift : kInf;
                     sr.inside = 0;
                     sr.within = KDN_NO_MATERIAL;
                     s.shadow[i] = sr;

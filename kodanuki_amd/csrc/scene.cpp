@@ -686,14 +686,29 @@ private:
     void light() {
         next();
         const Token& kind = next();
+        // the reference's defaults (readLight, syntax/evaluator.cpp:2114-2140)
         kdn_light l{};
+        l.direction[2] = 1;
+        l.along[0] = 1;
+        l.across[1] = 1;
+        l.angle = 0.0093f;
+        l.power[0] = l.power[1] = l.power[2] = 1;
         if (kind.text == "quad") l.kind = KDN_LIGHT_QUAD;
         else if (kind.text == "point") l.kind = KDN_LIGHT_POINT;
-        else fail(kind, "this loader takes quad and point lights only");
+        else if (kind.text == "directional") l.kind = KDN_LIGHT_DIRECTIONAL;
+        else if (kind.text == "sphere") l.kind = KDN_LIGHT_SPHERE;
+        else if (kind.text == "sun") l.kind = KDN_LIGHT_SUN;
+        else fail(kind, "this loader takes point, directional, sphere, quad and sun lights (no moon)");
         fields([&](const std::string& n, const Value& v, const Token& at) {
             if (n == "reaches" || n == "caustics") return;
+            if (n == "radius" || n == "angle") {
+                const std::vector<double> c = vec(v, 1, at);
+                (n == "radius" ? l.radius : l.angle) = float(c[0]);
+                return;
+            }
             std::vector<double> c = vec(v, 3, at);
-            float* dst = n == "at" ? l.at : n == "along" ? l.along : n == "across" ? l.across : n == "power" ? l.power : nullptr;
+            float* dst = n == "at" ? l.at : n == "along" ? l.along : n == "across" ? l.across : n == "power" ? l.power
+                       : n == "direction" ? l.direction : nullptr;
             if (!dst) fail(at, "light has no field " + n);
             for (int k = 0; k < 3; ++k) dst[k] = float(c[k]);
         });

@@ -167,23 +167,55 @@ std::vector<LampDev> lampTable(const kdn_light* lights, size_t count) {
     double totalPower = 0;
     for (size_t i = 0; i < count; ++i) {
         const kdn_light& l = lights[i];
-        if (l.kind != KDN_LIGHT_QUAD) continue;
         LampDev x{};
         x.kind = l.kind;
         const double f[3] = {(double)l.along[1] * l.across[2] - (double)l.along[2] * l.across[1],
                              (double)l.along[2] * l.across[0] - (double)l.along[0] * l.across[2],
                              (double)l.along[0] * l.across[1] - (double)l.along[1] * l.across[0]};
-        const double area = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
-        if (!(area > 1e-12)) continue;
+        const double quad = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+        const double dlen = std::sqrt((double)l.direction[0] * l.direction[0] + (double)l.direction[1] * l.direction[1] +
+                                      (double)l.direction[2] * l.direction[2]);
+        const double half = 0.5 * (double)l.angle, r = (double)l.radius;
+        // radianceOf and areaOf (light.cpp:32-61); a light that spreads its
+        // power on nothing is left out (as startsPaths and the aims do)
+        double spread = 1, area = 0;
+        switch (l.kind) {
+            case KDN_LIGHT_QUAD:
+                if (!(quad > 1e-12)) continue;
+                spread = quad;
+                area = quad;
+                break;
+            case KDN_LIGHT_SPHERE:
+                if (!(r > 0)) continue;
+                spread = kPi * r * r;
+                area = 4 * kPi * r * r;
+                break;
+            case KDN_LIGHT_SUN:
+                if (!(std::cos(half) < 1) || !(dlen > 0)) continue;
+                spread = kPi * std::sin(half) * std::sin(half);
+                break;
+            case KDN_LIGHT_DIRECTIONAL:
+                if (!(dlen > 0)) continue;
+                break;
+            case KDN_LIGHT_POINT:
+                break;
+            default:
+                continue;
+        }
         for (int k = 0; k < 3; ++k) {
             x.at[k] = l.at[k];
             x.along[k] = l.along[k];
             x.across[k] = l.across[k];
-            x.facing[k] = (float)(f[k] / area);
-            x.radiance[k] = (float)(l.power[k] / area);  // radianceOf (light.cpp:32-36)
+            x.facing[k] = quad > 1e-12 ? (float)(f[k] / quad) : 0.0f;
+            x.radiance[k] = (float)(l.power[k] / spread);
+            x.direction[k] = dlen > 0 ? (float)(l.direction[k] / dlen) : 0.0f;
+            x.power[k] = l.power[k];
         }
         x.area = (float)area;
-        const double emitted = std::max((l.power[0] + l.power[1] + l.power[2]) / 3.0, 0.0) * kPi;  // emittedOf (light.cpp:45-48)
+        x.radius = (float)r;
+        x.cosine = (float)std::cos(half);
+        // emittedOf (light.cpp:45-48): pi for quads, 4 pi otherwise, times the mean power
+        const double emitted = std::max((l.power[0] + l.power[1] + l.power[2]) / 3.0, 0.0) * (l.kind == KDN_LIGHT_QUAD ? kPi : 4 * kPi);
         x.pick = (float)emitted;
         totalPower += emitted;
         lamps.push_back(x);

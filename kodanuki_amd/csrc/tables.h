@@ -36,15 +36,20 @@ struct MaterialDev {
     float glow[3];         // glow * strength
 };
 
-// A quad lamp as the integrator reads it (light.cpp:32-110).
+// A lamp as the integrator reads it (lampOf, light.cpp:77-94): a point,
+// a directional light, a sphere, a quad or a sun.
 struct LampDev {
     float at[3], along[3], across[3];
     float facing[3];
-    float radiance[3];
+    float radiance[3];     // radianceOf (light.cpp:32-43): power over the area a kind spreads it on
     float area;
     float pick;            // the chance NEE picks it
     float pick_cdf;        // cumulative pick chance up to and with it
     uint32_t kind;
+    float direction[3];    // unit, toward a sun or a directional light
+    float radius;          // a sphere's
+    float cosine;          // a sun's half angle's cosine
+    float power[3];
 };
 
 void buildColourTables(ColourTables& t);
