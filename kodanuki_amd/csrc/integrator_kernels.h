@@ -332,6 +332,24 @@ KDN_D Scatter deep_scatter(const MaterialDev& m, const Lanes& w, V3 n, V3 out, c
     return s;
 }
 
+// A shadow test of path i toward a lamp, carrying c, when c brings light.
+KDN_D bool offer_shadow(PathState& s, size_t N, uint32_t i, V3 from, V3 in, float far, const float c[kLanes]) {
+    if (!(c[0] + c[1] + c[2] + c[3] > 0)) return false;
+    kdn_ray sr;
+    sr.origin[0] = from.x;
+    sr.origin[1] = from.y;
+    sr.origin[2] = from.z;
+    sr.direction[0] = in.x;
+    sr.direction[1] = in.y;
+    sr.direction[2] = in.z;
+    sr.far_ = far;
+    sr.inside = 0;
+    sr.within = KDN_NO_MATERIAL;
+    s.shadow[i] = sr;
+    for (int l = 0; l < kLanes; ++l) s.srad[l * N + i] = c[l];
+    return true;
+}
+
 KDN_D float lift_at(V3 x) {
     return 8e-5f + 8e-6f * fmax_(fmax_(__builtin_fabsf(x.x), __builtin_fabsf(x.y)), __builtin_fabsf(x.z));
 }
@@ -476,92 +494,87 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
             const LampDev& L = J.lamps[j];
             const float u = number(stream, sample, depth, LightU), v = number(stream, sample, depth, LightV);
             const V3 at = v3(L.at[0], L.at[1], L.at[2]);
-            // the direction, the distance to the lamp along it, what arrives
-            // over the chance of the direction (radiance / density, or the
-            // singular lights' irradiance), and the density (0: singular)
-            V3 in = v3(0, 0, 1);
-            float dist = 0, density = 0, arriving[3] = {0, 0, 0};
-            bool aimed = false;
             if (L.kind == KDN_LIGHT_QUAD) {
                 const V3 y = at + v3(L.along[0], L.along[1], L.along[2]) * u + v3(L.across[0], L.across[1], L.across[2]) * v;
                 const V3 toward = y - x;
-                dist = length(toward);
-                in = toward * (1.0f / dist);
+                const float dist = length(toward);
+                const V3 in = toward * (1.0f / dist);
                 const float cosL = -dot(v3(L.facing[0], L.facing[1], L.facing[2]), in);
-                if (cosL > 0) {
-                    density = dist * dist / (cosL * L.area);
-                    for (int k = 0; k < 3; ++k) arriving[k] = L.radiance[k] / density;
-                    aimed = true;
-                }
-            } else if (L.kind == KDN_LIGHT_POINT) {
-                const V3 toward = at - x;
-                dist = length(toward);
-                if (dist > 1e-9f) {
-                    in = toward * (1.0f / dist);
-                    for (int k = 0; k < 3; ++k) arriving[k] = L.power[k] / (dist * dist);
-                    aimed = true;
-                }
-            } else if (L.kind == KDN_LIGHT_DIRECTIONAL) {
-                in = v3(L.direction[0], L.direction[1], L.direction[2]);
-                dist = kInf;
-                for (int k = 0; k < 3; ++k) arriving[k] = L.power[k];
-                aimed = true;
-            } else if (L.kind == KDN_LIGHT_SPHERE) {
-                const V3 toward = at - x;
-                const float away = length(toward);
-                if (away > 1e-9f && away > L.radius) {
-                    const V3 centre = toward * (1.0f / away);
-                    const float cosine = cone_cosine(L.radius, away);
-                    in = uniform_cone(centre, cosine, u, v);
-                    const float height = dot(in, centre);
-                    const float ring = __builtin_sqrtf(fmax_(1 - height * height, 0.0f));
-                    dist = fmax_(away * height - __builtin_sqrtf(fmax_(L.radius * L.radius - away * away * ring * ring, 0.0f)), 0.0f);
-                    density = cone_density(cosine);
-                    for (int k = 0; k < 3; ++k) arriving[k] = L.power[k] * (2 * (1 - cosine) / (L.radius * L.radius));
-                    aimed = density > 0;
-                }
-            } else if (L.kind == KDN_LIGHT_SUN) {
-                in = uniform_cone(v3(L.direction[0], L.direction[1], L.direction[2]), L.cosine, u, v);
-                dist = kInf;
-                density = cone_density(L.cosine);
-                for (int k = 0; k < 3; ++k) arriving[k] = L.radiance[k] / density;
-                aimed = density > 0;
-            }
-            const float cosS = dot(n, in);
-            if (aimed && cosS > 0 && dist > 2 * lift) {
-                float f[kLanes], Le[kLanes];
-                rough_value(m, w, n, out, in, f);
-                spectrum_of(arriving, w, Le);
-                float wt = 1.0f;  // singular lights: aiming alone finds them
-                if (density > 0) {
+                const float cosS = dot(n, in);
+                if (cosL > 0 && cosS > 0 && dist > 2 * lift) {
+                    float f[kLanes], Le[kLanes];
+                    rough_value(m, w, n, out, in, f);
+                    spectrum_of(L.radiance, w, Le);
+                    const float pdfL = L.pick * dist * dist / (cosL * L.area);
                     const float pdfS = guided ? share * mixture_pdf(lobes, in) + (1 - share) * rough_density(m, n, out, in)
                                        : hg   ? hgShare * hg_density(hg, x, n, out, in) + (1 - hgShare) * rough_density(m, n, out, in)
                                        : deep ? deepView.share * hg_deep_density(deepView, x, n, out, mi, m, J.lamps, J.materials, in) +
                                                     (1 - deepView.share) * rough_density(m, n, out, in)
                                               : rough_density(m, n, out, in);
-                    wt = power_weight(L.pick * density, pdfS);
+                    const float wt = power_weight(pdfL, pdfS);
+                    float c[kLanes];
+                    for (int l = 0; l < kLanes; ++l) c[l] = thr[l] * f[l] * cosS * Le[l] / pdfL * wt;
+                    *shadow = offer_shadow(s, N, i, x + n * lift, in, dist - lift, c);
                 }
-                float c[kLanes];
-                float any = 0;
-                for (int l = 0; l < kLanes; ++l) {
-                    c[l] = thr[l] * f[l] * cosS * Le[l] / L.pick * wt;
-                    any += c[l];
+            } else {
+                // the direction, the distance to the lamp along it, what
+                // arrives over the chance of the direction (radiance over
+                // density, or the singular lights' irradiance), and the
+                // density (0: singular)
+                V3 in = v3(0, 0, 1);
+                float dist = 0, density = 0, arriving[3] = {0, 0, 0};
+                bool aimed = false;
+                if (L.kind == KDN_LIGHT_POINT) {
+                    const V3 toward = at - x;
+                    dist = length(toward);
+                    if (dist > 1e-9f) {
+                        in = toward * (1.0f / dist);
+                        for (int k = 0; k < 3; ++k) arriving[k] = L.power[k] / (dist * dist);
+                        aimed = true;
+                    }
+                } else if (L.kind == KDN_LIGHT_DIRECTIONAL) {
+                    in = v3(L.direction[0], L.direction[1], L.direction[2]);
+                    dist = kInf;
+                    for (int k = 0; k < 3; ++k) arriving[k] = L.power[k];
+                    aimed = true;
+                } else if (L.kind == KDN_LIGHT_SPHERE) {
+                    const V3 toward = at - x;
+                    const float away = length(toward);
+                    if (away > 1e-9f && away > L.radius) {
+                        const V3 centre = toward * (1.0f / away);
+                        const float cosine = cone_cosine(L.radius, away);
+                        in = uniform_cone(centre, cosine, u, v);
+                        const float height = dot(in, centre);
+                        const float ring = __builtin_sqrtf(fmax_(1 - height * height, 0.0f));
+                        dist = fmax_(away * height - __builtin_sqrtf(fmax_(L.radius * L.radius - away * away * ring * ring, 0.0f)), 0.0f);
+                        density = cone_density(cosine);
+                        for (int k = 0; k < 3; ++k) arriving[k] = L.power[k] * (2 * (1 - cosine) / (L.radius * L.radius));
+                        aimed = density > 0;
+                    }
+                } else if (L.kind == KDN_LIGHT_SUN) {
+                    in = uniform_cone(v3(L.direction[0], L.direction[1], L.direction[2]), L.cosine, u, v);
+                    dist = kInf;
+                    density = cone_density(L.cosine);
+                    for (int k = 0; k < 3; ++k) arriving[k] = L.radiance[k] / density;
+                    aimed = density > 0;
                 }
-                if (any > 0) {
-                    kdn_ray sr;
-                    const V3 so = x + n * lift;
-                    sr.origin[0] = so.x;
-                    sr.origin[1] = so.y;
-                    sr.origin[2] = so.z;
-                    sr.direction[0] = in.x;
-                    sr.direction[1] = in.y;
-                    sr.direction[2] = in.z;
-                    sr.far_ = dist < kInf ? dist - lift : kInf;
-                    sr.inside = 0;
-                    sr.within = KDN_NO_MATERIAL;
-                    s.shadow[i] = sr;
-                    for (int l = 0; l < kLanes; ++l) s.srad[l * N + i] = c[l];
-                    *shadow = true;
+                const float cosS = dot(n, in);
+                if (aimed && cosS > 0 && dist > 2 * lift) {
+                    float f[kLanes], Le[kLanes];
+                    rough_value(m, w, n, out, in, f);
+                    spectrum_of(arriving, w, Le);
+                    float wt = 1.0f;  // singular lights: aiming alone finds them
+                    if (density > 0) {
+                        const float pdfS = guided ? share * mixture_pdf(lobes, in) + (1 - share) * rough_density(m, n, out, in)
+                                           : hg   ? hgShare * hg_density(hg, x, n, out, in) + (1 - hgShare) * rough_density(m, n, out, in)
+                                           : deep ? deepView.share * hg_deep_density(deepView, x, n, out, mi, m, J.lamps, J.materials, in) +
+                                                        (1 - deepView.share) * rough_density(m, n, out, in)
+                                                  : rough_density(m, n, out, in);
+                        wt = power_weight(L.pick * density, pdfS);
+                    }
+                    float c[kLanes];
+                    for (int l = 0; l < kLanes; ++l) c[l] = thr[l] * f[l] * cosS * Le[l] / L.pick * wt;
+                    *shadow = offer_shadow(s, N, i, x + n * lift, in, dist < kInf ? dist - lift : kInf, c);
                 }
             }
         }
