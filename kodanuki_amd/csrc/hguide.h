@@ -128,15 +128,43 @@ constexpr float kHgMoved = 0.1f;         // the farthest a highlight walked agai
 constexpr double kHgEfficiency = 0.3;    // guideEfficiency (survey.cpp:818-828)
 constexpr float kHgLeastShare = 0.02f, kHgMostShare = 0.5f;
 
+// lampGlare (guide.cpp:269-281): the power's luminance over the area a
+// kind spreads it on (a quad's, pi r^2 for a sphere, pi times the half
+// angle squared for a sun), times 1e4 for a point or directional light
 KDN_D float hg_lamp_glare(const LampDev& L) {
-    // luminance of the power over the area (lampGlare, guide.cpp:262-273):
-    // radiance is power / area
-    return 0.2126f * L.radiance[0] + 0.7152f * L.radiance[1] + 0.0722f * L.radiance[2];
+    const float bright = 0.2126f * L.power[0] + 0.7152f * L.power[1] + 0.0722f * L.power[2];
+    switch (L.kind) {
+        case KDN_LIGHT_SPHERE: return bright / (kPi * fmax_(L.radius * L.radius, 1e-12f));
+        case KDN_LIGHT_QUAD: return bright / fmax_(L.area, 1e-12f);
+        case KDN_LIGHT_SUN: {
+            const float half = acosf(fmin_(fmax_(L.cosine, -1.0f), 1.0f));
+            return bright / (kPi * fmax_(half * half, 1e-12f));
+        }
+        default: return bright * 1e4f;
+    }
+}
+
+// The lamps a highlight mirrors: quads (from their centre, on their lit
+// side), spheres and points (from anywhere); a far light has no point a
+// mirror could image.
+KDN_D bool hg_mirrored(const LampDev& L) {
+    return L.kind == KDN_LIGHT_QUAD || L.kind == KDN_LIGHT_SPHERE || L.kind == KDN_LIGHT_POINT;
 }
 
 KDN_D V3 hg_lamp_centre(const LampDev& L) {
+    if (L.kind != KDN_LIGHT_QUAD) return v3(L.at[0], L.at[1], L.at[2]);
     return v3(L.at[0] + 0.5f * (L.along[0] + L.across[0]), L.at[1] + 0.5f * (L.along[1] + L.across[1]),
               L.at[2] + 0.5f * (L.along[2] + L.across[2]));
+}
+
+// whether the lamp sends light along dir (unit, from a point toward it)
+KDN_D bool hg_lamp_faces(const LampDev& L, V3 dir) {
+    return L.kind != KDN_LIGHT_QUAD || -dot(v3(L.facing[0], L.facing[1], L.facing[2]), dir) > 0;
+}
+
+// the lamp's angular radius seen from distance dl
+KDN_D float hg_lamp_reach(const LampDev& L, float dl) {
+    return L.kind == KDN_LIGHT_QUAD ? 0.5f * __builtin_sqrtf(L.area) / dl : L.kind == KDN_LIGHT_SPHERE ? L.radius / dl : 0.0f;
 }
 
 KDN_D Lanes hg_grey_lanes() {
@@ -191,7 +219,7 @@ KDN_D bool hg_seen(const float* h, V3 x, V3 n, V3 out, const MaterialDev& m, con
     const float dist = length(towards), dl = length(centre - point);
     if (!(dist > 1e-4f && dl > 1e-4f)) return false;
     const V3 axis = towards * (1.0f / dist);
-    const float rho = 0.5f * __builtin_sqrtf(L.area) / dl;
+    const float rho = hg_lamp_reach(L, dl);
     const float sigma = fmax_((2 * h[6] + rho) * dl / (dist + dl), 1e-3f);
     if (!(sigma < 1)) return false;
     const float weight = hg_lamp_glare(L) * sigma * sigma * rough_density(m, n, out, axis);

@@ -191,7 +191,7 @@ __global__ void k_hg_starts(HgScene S, const float* stands, const kdn_ray* rays,
             float best = 0;
             for (uint32_t j = 0; j < S.nLamps; ++j) {
                 const LampDev& L = S.lamps[j];
-                if (L.kind != KDN_LIGHT_QUAD) continue;  // the survey mirrors quad lamps
+                if (!hg_mirrored(L)) continue;
                 const V3 centre = hg_lamp_centre(L);
                 const float above = dot(centre - x, nv);
                 if (!(above > 0)) continue;
@@ -202,8 +202,8 @@ __global__ void k_hg_starts(HgScene S, const float* stands, const kdn_ray* rays,
                 const V3 toward = centre - point;
                 const float dl = length(toward), D = length(point - stand);
                 if (!(dl > 1e-4f && D > 1e-4f)) continue;
-                if (!(-dot(v3(L.facing[0], L.facing[1], L.facing[2]), toward) > 0)) continue;
-                const float rho = 0.5f * __builtin_sqrtf(L.area) / dl;
+                if (!hg_lamp_faces(L, toward * (1.0f / dl))) continue;
+                const float rho = hg_lamp_reach(L, dl);
                 const float sigma = fmax_((2 * m.width + rho) * dl / (D + dl), 1e-3f);
                 const float importance = hg_lamp_glare(L) * sigma * sigma;
                 if (importance > best) {
@@ -257,8 +257,8 @@ __global__ void k_hg_candidates(HgScene S, const kdn_ray* aims, const kdn_hit* h
     const float dl = length(toward), D = length(x - o);
     if (!(dl > 1e-4f)) return;
     const V3 dir = toward * (1.0f / dl);
-    if (!(-dot(v3(L.facing[0], L.facing[1], L.facing[2]), dir) > 0)) return;
-    const float rho = 0.5f * __builtin_sqrtf(L.area) / dl;
+    if (!hg_lamp_faces(L, dir)) return;
+    const float rho = hg_lamp_reach(L, dl);
     const float reach = rho + 2 * m.width;
     if (!(dot(mirror, dir) > f_cos(fmin_(reach, 1.5f)))) return;
     const float sigma = fmax_((2 * m.width + rho) * dl / (D + dl), 1e-3f);

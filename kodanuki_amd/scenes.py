@@ -187,7 +187,14 @@ paint(red) at([0.3, -0.2, 0.35]) sphere(0.35);
 """
 
 
-def highlight_room(width=64, height=48, samples=16, surfaces=6, roughness=0.15):
+_HIGHLIGHT_LAMPS = {
+    "quad": "light quad { at = [-1.35, -0.15, 1]; along = [0, 0.3, 0]; across = [0.3, 0, 0]; power = [40, 40, 38]; }",
+    # the same power from a ball where the quad's centre is
+    "sphere": "light sphere { at = [-1.2, 0, 1]; radius = 0.12; power = [40, 40, 38]; }",
+}
+
+
+def highlight_room(width=64, height=48, samples=16, surfaces=6, roughness=0.15, lamp="quad"):
     """A closed grey room lit by one small lamp that faces the floor, where
     a polished steel plate mirrors it onto the far wall: the wall above the
     lamp's height sees no lamp directly, and its light comes off the plate
@@ -195,7 +202,7 @@ def highlight_room(width=64, height=48, samples=16, surfaces=6, roughness=0.15):
     return f"""import "standard.sdf";
 render {{ width = {width}; height = {height}; seed = 1; samples = {samples}; surfaces = {surfaces}; far = 4; }}
 camera {{ eye = [-1.7, -1.7, 1.4]; target = [2, 0.3, 1.7]; up = [0, 0, 1]; fov = 1.1; }}
-light quad {{ at = [-1.35, -0.15, 1]; along = [0, 0.3, 0]; across = [0.3, 0, 0]; power = [40, 40, 38]; }}
+{_HIGHLIGHT_LAMPS[lamp]}
 let wall = matte([0.75, 0.75, 0.72]);
 let steel = metal([0.95, 0.93, 0.9], roughness = {roughness:.6g});
 paint(wall) cut {{ at([0, 0, 1.5]) box([4.4, 4.4, 3.4]); at([0, 0, 1.5]) box([4, 4, 3]); }}

@@ -143,3 +143,24 @@ def test_guided_error_is_lower_at_equal_samples(device, room):
     gain = errs["plain"] / errs["guided"]
     note(f"[hguide] relative MSE at 16 spp: plain {errs['plain']:.4g}, guided {errs['guided']:.4g}, gain {gain:.2f}x")
     assert gain > 1.2
+
+
+def test_sphere_lamp_is_surveyed_and_the_mean_kept(device):
+    """The survey mirrors sphere lamps as it mirrors quads; a guided render
+    of the room lit by a ball keeps its mean within 4 standard errors."""
+    s = k.Scene(scenes.highlight_room(W, H_, lamp="sphere"))
+    lower, size = s.region(6)
+    device.upload(s)
+    device.build_octree(lower, size, 6)
+    st = device.highlight_guides(W, H_)
+    note(f"[hguide] sphere lamp: {st.highlights} highlights, {st.guided_pixels} guided pixels")
+    assert st.highlights > 0 and st.guided_pixels > 0
+    seeds, spp = (3, 4) if EMULATION else (12, 64)
+    arms = {}
+    for name, flags in (("plain", 0), ("guided", k.PASS_HIGHLIGHT)):
+        means = np.array([_frame(device, spp, 200 + i, flags).mean(axis=(0, 1)) for i in range(seeds)])
+        arms[name] = (means.mean(axis=0), means.std(axis=0, ddof=1) / np.sqrt(seeds))
+    (a, ea), (b, eb) = arms["plain"], arms["guided"]
+    gap = np.abs(a - b) / np.maximum(np.sqrt(ea * ea + eb * eb), 1e-12)
+    note(f"[hguide] sphere lamp: plain {a}, guided {b}, worst gap {gap.max():.2f} standard errors")
+    assert gap.max() < 4
