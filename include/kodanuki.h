@@ -121,12 +121,14 @@ typedef struct kdn_material {
     float strength;
 } kdn_material;
 
-/* Light kinds (source/sdf/scene.h:155-181; the moon is not loaded). */
+/* Light kinds (source/sdf/scene.h:155-181). A moon is aimed at and met as
+ * a sun is (light.cpp treats the two alike). */
 #define KDN_LIGHT_POINT 0
 #define KDN_LIGHT_QUAD 1
 #define KDN_LIGHT_DIRECTIONAL 2
 #define KDN_LIGHT_SPHERE 3
 #define KDN_LIGHT_SUN 4
+#define KDN_LIGHT_MOON 5
 
 /* A lamp (source/sdf/scene.h:481-528, quad and point kinds). */
 typedef struct kdn_light {

@@ -53,7 +53,7 @@ class Light(C.Structure):
                 ("power", C.c_float * 3), ("direction", C.c_float * 3), ("radius", C.c_float), ("angle", C.c_float)]
 
 
-LIGHT_POINT, LIGHT_QUAD, LIGHT_DIRECTIONAL, LIGHT_SPHERE, LIGHT_SUN = 0, 1, 2, 3, 4
+LIGHT_POINT, LIGHT_QUAD, LIGHT_DIRECTIONAL, LIGHT_SPHERE, LIGHT_SUN, LIGHT_MOON = 0, 1, 2, 3, 4, 5
 
 
 class Leaf(C.Structure):

@@ -698,7 +698,8 @@ private:
         else if (kind.text == "directional") l.kind = KDN_LIGHT_DIRECTIONAL;
         else if (kind.text == "sphere") l.kind = KDN_LIGHT_SPHERE;
         else if (kind.text == "sun") l.kind = KDN_LIGHT_SUN;
-        else fail(kind, "this loader takes point, directional, sphere, quad and sun lights (no moon)");
+        else if (kind.text == "moon") l.kind = KDN_LIGHT_MOON;
+        else fail(kind, "unknown light kind (point, directional, sphere, quad, sun, moon)");
         fields([&](const std::string& n, const Value& v, const Token& at) {
             if (n == "reaches" || n == "caustics") return;
             if (n == "radius" || n == "angle") {

@@ -168,7 +168,7 @@ std::vector<LampDev> lampTable(const kdn_light* lights, size_t count) {
     for (size_t i = 0; i < count; ++i) {
         const kdn_light& l = lights[i];
         LampDev x{};
-        x.kind = l.kind;
+        x.kind = l.kind == KDN_LIGHT_MOON ? KDN_LIGHT_SUN : l.kind;  // a moon is aimed at and met as a sun
         const double f[3] = {(double)l.along[1] * l.across[2] - (double)l.along[2] * l.across[1],
                              (double)l.along[2] * l.across[0] - (double)l.along[0] * l.across[2],
                              (double)l.along[0] * l.across[1] - (double)l.along[1] * l.across[0]};
@@ -179,7 +179,7 @@ std::vector<LampDev> lampTable(const kdn_light* lights, size_t count) {
         // radianceOf and areaOf (light.cpp:32-61); a light that spreads its
         // power on nothing is left out (as startsPaths and the aims do)
         double spread = 1, area = 0;
-        switch (l.kind) {
+        switch (x.kind) {
             case KDN_LIGHT_QUAD:
                 if (!(quad > 1e-12)) continue;
                 spread = quad;
@@ -215,7 +215,7 @@ std::vector<LampDev> lampTable(const kdn_light* lights, size_t count) {
         x.radius = (float)r;
         x.cosine = (float)std::cos(half);
         // emittedOf (light.cpp:45-48): pi for quads, 4 pi otherwise, times the mean power
-        const double emitted = std::max((l.power[0] + l.power[1] + l.power[2]) / 3.0, 0.0) * (l.kind == KDN_LIGHT_QUAD ? kPi : 4 * kPi);
+        const double emitted = std::max((l.power[0] + l.power[1] + l.power[2]) / 3.0, 0.0) * (x.kind == KDN_LIGHT_QUAD ? kPi : 4 * kPi);
         x.pick = (float)emitted;
         totalPower += emitted;
         lamps.push_back(x);

@@ -105,3 +105,10 @@ def test_sun(device):
     P = 2.0
     got, se = _render(device, f"light sun {{ direction = [0, 0, 1]; angle = 0.4; power = [{P}, {P}, {P}]; }}")
     _check("sun", got, se, np.full((H, W), ALBEDO / np.pi * P))
+
+
+def test_moon_is_a_sun(device):
+    """A moon is aimed at and met as a sun (light.cpp treats the two alike)."""
+    P = 2.0
+    got, se = _render(device, f"light moon {{ direction = [0, 0, 1]; angle = 0.4; power = [{P}, {P}, {P}]; }}")
+    _check("moon", got, se, np.full((H, W), ALBEDO / np.pi * P))
