@@ -692,28 +692,53 @@ KDN_D void shadow_one(const PassParams& P, uint32_t q, const LaneRegs& lr, doubl
 }
 
 // finish: the lane takes entries of the input queue in turn (tally[2]
-// counts them out; count of them in all) and runs one segment of its path
-// an iteration: extend, shade and the shadow test, taking the next entry
-// when its path stops (shading stops every path at its last surface), so
-// that a wave's lanes stay busy until the queue drains rather than wait
-// for the wave's longest path. The same per-path arithmetic as the
+// counts them out; count of them in all) and marches one ray of its path
+// an iteration: the path's next segment (then shading, which stops every
+// path at its last surface) or, by turns, the shadow test shading asked
+// for; it takes the next entry when its path stops. A wave's lanes thus
+// run one march between them an iteration, whichever rays they hold, and
+// stay busy until the queue drains rather than wait for the wave's longest
+// path. The same per-path arithmetic, in the same order, as the
 // wavefront's launches, so the estimates do not change; counts segments
 // and shadow tests in tally[0] and tally[1].
 KDN_D void finish_lane(PassParams& P, uint32_t inQueue, uint32_t count, const LaneRegs& lr, uint32_t* tally, double* work) {
     PathState& s = P.s;
+    const size_t N = s.capacity;
     uint32_t segments = 0, shadows = 0;
     uint32_t q = atomicAdd(&tally[2], 1u);
+    bool shadowTurn = false, aliveAfter = false;
     while (q < count) {
         const uint32_t i = s.queue[inQueue][q];
-        trace_path(P, i, lr, work);
-        ++segments;
-        bool shadow;
-        const bool alive = shade_path(P, i, &shadow);
-        if (shadow) {
-            shadow_path(P, i, lr, work ? work + 1 : nullptr);
-            ++shadows;
+        const kdn_ray r = shadowTurn ? s.shadow[i] : s.rays[i];
+        TreeView t = job_of(P, i).tv;
+        double w = 0;
+        t.work = work ? &w : nullptr;
+        kdn_hit h;
+        int iterations = 0;
+        march_any(t, r, lr, h, iterations, shadowTurn);
+        KDN_TRACE_ITERATIONS(iterations);
+        if (work) atomicAdd(shadowTurn ? work + 1 : work, w);
+        bool next = false;
+        if (shadowTurn) {
+            if (h.kind == KDN_HIT_MISSED || !(h.distance < r.far_)) {
+                for (int l = 0; l < kLanes; ++l) s.rad[l * N + i] += s.srad[l * N + i];
+            }
+            shadowTurn = false;
+            next = !aliveAfter;
+        } else {
+            s.hits[i] = h;
+            ++segments;
+            bool shadow;
+            const bool alive = shade_path(P, i, &shadow);
+            if (shadow) {
+                ++shadows;
+                shadowTurn = true;
+                aliveAfter = alive;
+            } else {
+                next = !alive;
+            }
         }
-        if (!alive) q = atomicAdd(&tally[2], 1u);
+        if (next) q = atomicAdd(&tally[2], 1u);
     }
     atomicAdd(&tally[0], segments);
     atomicAdd(&tally[1], shadows);

@@ -280,8 +280,10 @@ KDN_D float slope_of(float part) {
 // before its far end: the march stops where it first comes within its
 // tolerance of a surface (or enters a filled cell) and reports that point,
 // with no tie-break, landing or source.
-template <bool AnyHit>
-KDN_D void march_ray(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr, kdn_hit& hit, int& iterations) {
+// (AnyHit is a run-time flag here: march_ray fixes it per call site, and a
+// lane that marches its path's rays and its shadow rays by turns passes it
+// per ray, so that a wave's lanes run one march between them.)
+KDN_D void march_any(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr, kdn_hit& hit, int& iterations, const bool AnyHit) {
     Dl* const buf = lr.ssa;
     const float o[3] = {ray.origin[0], ray.origin[1], ray.origin[2]};
     const float d[3] = {ray.direction[0], ray.direction[1], ray.direction[2]};
@@ -462,6 +464,11 @@ KDN_D void march_ray(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr,
         lvl = shared < lvl ? shared : lvl;
         ci[0] = next[0]; ci[1] = next[1]; ci[2] = next[2];
     }
+}
+
+template <bool AnyHit>
+KDN_D void march_ray(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr, kdn_hit& hit, int& iterations) {
+    march_any(tv, ray, lr, hit, iterations, AnyHit);
 }
 
 // The extend step of one ray: its hit (missed when it leaves the region or
