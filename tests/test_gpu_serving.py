@@ -73,7 +73,7 @@ def test_jobs_in_a_mix_match_jobs_alone(device, scene_set, alone, max_paths):
     with k.Server(device, max_paths=max_paths) as server:
         ids = [_submit(server, scene_set, job) for job in JOBS[:2]]
         st = server.step()
-        note(f"[serve] round 1: {st.jobs_in_round} jobs {st.slices_in_round} slices {st.paths} paths")
+        note(f"[serve] first serving round: {st.jobs_in_round} jobs {st.slices_in_round} slices {st.paths} paths")
         # two more arrive while the first two run
         ids += [_submit(server, scene_set, job) for job in JOBS[2:]]
         rounds = 1 + server.run()
