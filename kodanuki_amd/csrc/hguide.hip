@@ -10,10 +10,9 @@
 // that meets a sharp surface (metal of facet width at most 0.125) starts a
 // highlight. Where the reference walks a specular chain of up to several
 // mirrors from each start and lamp to the point that mirrors the lamp
-// exactly (chain.cpp), this survey walks a chain of one mirror in two
-// steps (a tangent plane's point where the stand's line to the lamp's
-// mirror image crosses it: from the start's plane, then from the plane
-// where the first aim met the surface), checks with a ray that the surface is
+// exactly (chain.cpp), this survey walks a chain of one mirror in one step
+// (the start's tangent plane: the point where the stand's line to the
+// lamp's mirror image crosses it), checks with a ray that the surface is
 // there and mirrors the lamp within its outline widened by the facets'
 // lobe, and gives the lump the spread a flat glossy mirror of that width
 // would: (2 width + the lamp's angular radius) d / (D + d) radians across,
@@ -164,7 +163,7 @@ __global__ void k_hg_draws(HgScene S, const float* stands, uint32_t nPoints, kdn
 // mirrors the lamp's centre toward the stand (the stand joined to the
 // centre's mirror image across the plane); the most important lamp's is
 // kept, and a ray from the stand toward it checks that the surface is
-// there (a chain of one mirror, walked a step: exact for planes).
+// there (a chain of one mirror, walked in one step: exact for planes).
 __global__ void k_hg_starts(HgScene S, const float* stands, const kdn_ray* rays, const kdn_hit* hits, uint32_t n, kdn_ray* aims,
                             float* cand) {
     const uint32_t at = blockIdx.x * blockDim.x + threadIdx.x;
@@ -229,45 +228,6 @@ __global__ void k_hg_starts(HgScene S, const float* stands, const kdn_ray* rays,
             }
         }
     }
-    aims[at] = a;
-}
-
-// A second step of the walk, for curved surfaces: a start whose aim met
-// the same sharp surface takes the tangent plane where it met it and aims
-// again at that plane's point that mirrors the lamp's centre toward the
-// stand (on a plane the point does not move); an aim that missed keeps its
-// ray and fails as before.
-__global__ void k_hg_restep(HgScene S, const float* stands, kdn_ray* aims, const kdn_hit* hits, uint32_t n, const float* cand) {
-    const uint32_t at = blockIdx.x * blockDim.x + threadIdx.x;
-    if (at >= n) return;
-    const float* c = cand + (size_t)at * kSpotFloats;
-    if (c[0] == 0) return;
-    const kdn_hit h = hits[at];
-    if (h.kind != KDN_HIT_SURFACE) return;
-    uint32_t mi;
-    mat_of(S, h.leaf, &mi);
-    if (mi != (uint32_t)c[11]) return;
-    const float* s = stands + (size_t)(at / kSurveyDraws) * kStandFloats;
-    const V3 stand = v3(s[1], s[2], s[3]);
-    const V3 x = v3(h.point[0], h.point[1], h.point[2]);
-    V3 nv = v3(h.normal[0], h.normal[1], h.normal[2]);
-    if (dot(nv, x - stand) > 0) nv = -nv;
-    const V3 centre = hg_lamp_centre(S.lamps[(uint32_t)c[1]]);
-    const float above = dot(centre - x, nv);
-    if (!(above > 0)) return;
-    const V3 image = centre - nv * (2 * above);
-    const float t = dot(x - stand, nv) / dot(image - stand, nv);
-    if (!(t > 0 && t < 1)) return;
-    const V3 point = stand + (image - stand) * t;
-    kdn_ray a = aims[at];
-    const V3 o = v3(a.origin[0], a.origin[1], a.origin[2]);
-    const V3 d = point - o;
-    const float reach = length(d);
-    if (!(reach > 1e-6f)) return;
-    a.direction[0] = d.x / reach;
-    a.direction[1] = d.y / reach;
-    a.direction[2] = d.z / reach;
-    a.far_ = reach * 1.05f + 0.01f;
     aims[at] = a;
 }
 
@@ -574,9 +534,6 @@ bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats, DevBuf&
         if (!extendRays(c, rays, nDraws, hits, st)) return false;
         hipLaunchKernelGGL(k_hg_starts, grid(nDraws), dim3(kHgBlock), 0, st, S, (const float*)stands, (const kdn_ray*)rays,
                            (const kdn_hit*)hits, nDraws, aims, cand);
-        if (!extendRays(c, aims, nDraws, hits, st)) return false;
-        hipLaunchKernelGGL(k_hg_restep, grid(nDraws), dim3(kHgBlock), 0, st, S, (const float*)stands, aims, (const kdn_hit*)hits, nDraws,
-                           (const float*)cand);
         if (!extendRays(c, aims, nDraws, hits, st)) return false;
         hipLaunchKernelGGL(k_hg_candidates, grid(nDraws), dim3(kHgBlock), 0, st, S, (const kdn_ray*)aims, (const kdn_hit*)hits,
                            nDraws, cand);
