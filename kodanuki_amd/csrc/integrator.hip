@@ -72,9 +72,12 @@ __global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_shadow(PassParams 
 // the device holds at once (the wavefront's last bounces carry few paths,
 // and each launch of them costs as long as its slowest ray): each lane
 // takes the queue's entries in turn (tally[2] counts them out).
+// (One: the round has one job, read from the kernel's arguments, and no
+// operations are counted.)
+template <bool One>
 __global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_finish(PassParams P, uint32_t inQueue, uint32_t* tally, double* work) {
     KDN_LANE_REGS;
-    finish_lane(P, inQueue, P.s.counters[inQueue], lr, tally, work);
+    finish_lane<One>(P, inQueue, P.s.counters[inQueue], lr, tally, work);
 }
 
 // the sort keys of a queue's rays (the paths' next rays, or their shadow rays)
@@ -257,6 +260,7 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
     P.paths = total;
     P.colour = (const ColourTables*)c.colour;
     P.s = *c.paths;
+    P.job0 = jobs[0];
     PathState* s = c.paths;
 
     const bool timing = (flags & 2) != 0, counting = (flags & 4) != 0, finishing = (flags & 8) == 0, guiding = (flags & 16) != 0;
@@ -333,8 +337,12 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
             KDN_HIP(hipMemsetAsync(&s->counters[4], 0, 12, st));
             const uint32_t finishBlocks = std::min<uint32_t>((count + kTraceBlock - 1) / kTraceBlock, residentTraceBlocks(c));
             timed(&stats->finish_ms, [&] {
-                hipLaunchKernelGGL(k_finish, dim3(finishBlocks), dim3(kTraceBlock), 0, st, P, in, &s->counters[4],
-                                   counting ? work + 2 : nullptr);
+                if (jobs.size() == 1 && !counting) {
+                    hipLaunchKernelGGL(k_finish<true>, dim3(finishBlocks), dim3(kTraceBlock), 0, st, P, in, &s->counters[4], nullptr);
+                } else {
+                    hipLaunchKernelGGL(k_finish<false>, dim3(finishBlocks), dim3(kTraceBlock), 0, st, P, in, &s->counters[4],
+                                       counting ? work + 2 : nullptr);
+                }
             });
             uint32_t tally[2] = {0, 0};
             KDN_HIP(hipMemcpyAsync(tally, &s->counters[4], 8, hipMemcpyDeviceToHost, st));

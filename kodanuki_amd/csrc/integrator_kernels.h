@@ -83,9 +83,22 @@ struct PassParams {
     const float* lobes;
     float* recs;
     uint32_t train_stride;
+    // the round's one job, where it has one (the kernels specialised for
+    // that case read it from their arguments, the same for every lane)
+    JobDev job0;
 };
 
 KDN_D const JobDev& job_of(const PassParams& P, uint32_t i) { return P.jobs[P.slices[P.s.slice[i]].job]; }
+
+// Path i's job: with One, the round's one job (P.job0).
+template <bool One>
+KDN_D const JobDev& job_in(const PassParams& P, uint32_t i) {
+    if constexpr (One) {
+        return P.job0;
+    } else {
+        return job_of(P, i);
+    }
+}
 
 KDN_D float lane(const float* a, uint32_t l, size_t i, size_t n) { return a[l * n + i]; }
 
@@ -379,10 +392,11 @@ KDN_D uint32_t ray_key(const TreeView& tv, const kdn_ray& r) {
 // shade path i at its hit: emission met, a shadow test toward one lamp
 // (its ray in s.shadow[i] and what it carries in s.srad; *shadow set) and
 // the scattered ray that continues the path (in s.rays[i]; returns true).
+template <bool One = false>
 KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
     PathState& s = P.s;
     const size_t N = s.capacity;
-    const JobDev& J = job_of(P, i);
+    const JobDev& J = job_in<One>(P, i);
     *shadow = false;
     const kdn_hit hit = s.hits[i];
     const kdn_ray ray = s.rays[i];
@@ -701,6 +715,8 @@ KDN_D void shadow_one(const PassParams& P, uint32_t q, const LaneRegs& lr, doubl
 // path. The same per-path arithmetic, in the same order, as the
 // wavefront's launches, so the estimates do not change; counts segments
 // and shadow tests in tally[0] and tally[1].
+// (With One: the round has one job and counts no operations.)
+template <bool One>
 KDN_D void finish_lane(PassParams& P, uint32_t inQueue, uint32_t count, const LaneRegs& lr, uint32_t* tally, double* work) {
     PathState& s = P.s;
     const size_t N = s.capacity;
@@ -710,14 +726,18 @@ KDN_D void finish_lane(PassParams& P, uint32_t inQueue, uint32_t count, const La
     while (q < count) {
         const uint32_t i = s.queue[inQueue][q];
         const kdn_ray r = shadowTurn ? s.shadow[i] : s.rays[i];
-        TreeView t = job_of(P, i).tv;
-        double w = 0;
-        t.work = work ? &w : nullptr;
         kdn_hit h;
         int iterations = 0;
-        march_any(t, r, lr, h, iterations, shadowTurn);
+        if constexpr (One) {
+            march_any(P.job0.tv, r, lr, h, iterations, shadowTurn);
+        } else {
+            TreeView t = job_of(P, i).tv;
+            double w = 0;
+            t.work = work ? &w : nullptr;
+            march_any(t, r, lr, h, iterations, shadowTurn);
+            if (work) atomicAdd(shadowTurn ? work + 1 : work, w);
+        }
         KDN_TRACE_ITERATIONS(iterations);
-        if (work) atomicAdd(shadowTurn ? work + 1 : work, w);
         bool next = false;
         if (shadowTurn) {
             if (h.kind == KDN_HIT_MISSED || !(h.distance < r.far_)) {
@@ -729,7 +749,7 @@ KDN_D void finish_lane(PassParams& P, uint32_t inQueue, uint32_t count, const La
             s.hits[i] = h;
             ++segments;
             bool shadow;
-            const bool alive = shade_path(P, i, &shadow);
+            const bool alive = shade_path<One>(P, i, &shadow);
             if (shadow) {
                 ++shadows;
                 shadowTurn = true;
