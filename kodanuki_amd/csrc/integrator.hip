@@ -40,11 +40,15 @@ __global__ void k_generate(PassParams P) {
     Dl buf[kMaxRegs];                                              \
     const LaneRegs lr{buf, SlotRegs{slots + threadIdx.x, kTraceBlock}, PathRegs{paths + threadIdx.x, kTraceBlock}}
 
+// (One, here and below: the round has one job, read from the kernel's
+// arguments, the same for every lane.)
+template <bool One>
 __global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_trace_queue(PassParams P, uint32_t inQueue) {
     KDN_LANE_REGS;
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= P.s.counters[inQueue]) return;
-    trace_path(P, P.s.queue[inQueue][q], lr, nullptr);
+    const uint32_t i = P.s.queue[inQueue][q];
+    trace_item(job_in<One>(P, i).tv, i, P.s.rays, P.s.hits, lr, nullptr);
 }
 
 // The same march counting the FP32 operations of every evaluation (the
@@ -56,16 +60,18 @@ __global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_trace_queue_count(
     trace_path(P, P.s.queue[inQueue][q], lr, work);
 }
 
+template <bool One>
 __global__ void __launch_bounds__(256) k_shade(PassParams P, uint32_t inQueue, uint32_t outQueue) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < P.s.counters[inQueue]) shade_one(P, q, inQueue, outQueue);
+    if (q < P.s.counters[inQueue]) shade_one<One>(P, q, inQueue, outQueue);
 }
 
+template <bool One>
 __global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_shadow(PassParams P, double* work) {
     KDN_LANE_REGS;
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= P.s.counters[2]) return;
-    shadow_one(P, q, lr, work);
+    shadow_path<One>(P, P.s.queue[2][q], lr, work);
 }
 
 // The paths still in the input queue to their ends, in as many lanes as
@@ -362,7 +368,11 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
             if (counting) {
                 hipLaunchKernelGGL(k_trace_queue_count, dim3(traceBlocks), dim3(kTraceBlock), 0, st, P, in, work);
             } else {
-                hipLaunchKernelGGL(k_trace_queue, dim3(traceBlocks), dim3(kTraceBlock), 0, st, P, in);
+                if (jobs.size() == 1) {
+                    hipLaunchKernelGGL(k_trace_queue<true>, dim3(traceBlocks), dim3(kTraceBlock), 0, st, P, in);
+                } else {
+                    hipLaunchKernelGGL(k_trace_queue<false>, dim3(traceBlocks), dim3(kTraceBlock), 0, st, P, in);
+                }
             }
         });
         if (guiding && bounce < (uint32_t)kGuideDepths) {
@@ -376,7 +386,13 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
         }
         KDN_HIP(hipMemsetAsync(&s->counters[out], 0, 4, st));
         KDN_HIP(hipMemsetAsync(&s->counters[2], 0, 4, st));
-        timed(&stats->shade_ms, [&] { hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, st, P, in, out); });
+        timed(&stats->shade_ms, [&] {
+            if (jobs.size() == 1) {
+                hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(256), 0, st, P, in, out);
+            } else {
+                hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(256), 0, st, P, in, out);
+            }
+        });
         uint32_t sc = 0;
         KDN_HIP(hipMemcpyAsync(&sc, &s->counters[2], 4, hipMemcpyDeviceToHost, st));
         KDN_HIP(hipStreamSynchronize(st));
@@ -385,8 +401,12 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
             if (!sortQueue(2, sc, 1)) return false;
             ++stats->shadow_launches;
             timed(&stats->shadow_ms, [&] {
-                hipLaunchKernelGGL(k_shadow, dim3((sc + kTraceBlock - 1) / kTraceBlock), dim3(kTraceBlock), 0, st, P,
-                                   counting ? work + 1 : nullptr);
+                const dim3 grid((sc + kTraceBlock - 1) / kTraceBlock);
+                if (jobs.size() == 1) {
+                    hipLaunchKernelGGL(k_shadow<true>, grid, dim3(kTraceBlock), 0, st, P, counting ? work + 1 : nullptr);
+                } else {
+                    hipLaunchKernelGGL(k_shadow<false>, grid, dim3(kTraceBlock), 0, st, P, counting ? work + 1 : nullptr);
+                }
             });
         }
         std::swap(in, out);

@@ -668,11 +668,12 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
 
 // shade: queue entry q of the input queue; shadow tests and live paths
 // are appended to their queues
+template <bool One = false>
 KDN_D void shade_one(PassParams& P, uint32_t q, uint32_t inQueue, uint32_t outQueue) {
     PathState& s = P.s;
     const uint32_t i = s.queue[inQueue][q];
     bool shadow;
-    const bool alive = shade_path(P, i, &shadow);
+    const bool alive = shade_path<One>(P, i, &shadow);
     if (shadow) {
         const uint32_t at = atomicAdd(&s.counters[2], 1u);
         s.queue[2][at] = i;
@@ -685,11 +686,12 @@ KDN_D void shade_one(PassParams& P, uint32_t q, uint32_t inQueue, uint32_t outQu
 
 // the shadow test of path i: what it carries reaches the path when nothing
 // lies on its ray (with work set, the march's operations are counted)
+template <bool One = false>
 KDN_D void shadow_path(const PassParams& P, uint32_t i, const LaneRegs& lr, double* work) {
     const PathState& s = P.s;
     const kdn_ray r = s.shadow[i];
     kdn_hit h;
-    TreeView t = job_of(P, i).tv;
+    TreeView t = job_in<One>(P, i).tv;
     double w = 0;
     t.work = work ? &w : nullptr;
     occlusion_ray(t, r, lr, h);
