@@ -271,7 +271,15 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
 
     const bool timing = (flags & 2) != 0, counting = (flags & 4) != 0, finishing = (flags & 8) == 0, guiding = (flags & 16) != 0;
     // kernel timing: an event pair per launch, read after each bounce's sync
-    hipEvent_t ev[2];
+    // (destroyed on every way out of the round)
+    struct EventPair {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~EventPair() {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } events;
+    hipEvent_t* ev = events.e;
     if (timing) {
         KDN_HIP(hipEventCreate(&ev[0]));
         KDN_HIP(hipEventCreate(&ev[1]));
@@ -424,15 +432,13 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
     KDN_HIP(hipStreamSynchronize(st));
     KDN_HIP(hipGetLastError());
     if (counting) {
-        double counted[3] = {0, 0, 0};
-        KDN_HIP(hipMemcpy(counted, work, 24, hipMemcpyDeviceToHost));
+        // trace, shadow, and the finishing launch's path marches and shadow
+        // marches (it counts them at work + 2 and work + 3)
+        double counted[4] = {0, 0, 0, 0};
+        KDN_HIP(hipMemcpy(counted, work, 32, hipMemcpyDeviceToHost));
         stats->trace_flops += counted[0];
         stats->shadow_flops += counted[1];
-        stats->finish_flops += counted[2];
-    }
-    if (timing) {
-        KDN_HIP(hipEventDestroy(ev[0]));
-        KDN_HIP(hipEventDestroy(ev[1]));
+        stats->finish_flops += counted[2] + counted[3];
     }
     return true;
 }

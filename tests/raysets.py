@@ -63,3 +63,42 @@ def random_rays(arrays, lower, size, count, seed, inside_share=0.0):
     rays = k.make_rays(np.array(origins, np.float32), np.array(dirs, np.float32))
     rays["inside"] = np.array(inside, np.uint32)
     return rays
+
+
+def leaf_rays(arrays, scene, count, seed):
+    """Random leaf-to-leaf rays (target/tests/walk.cpp:174-193): from a point
+    within 1.5 of one primitive's placement toward a point within 0.3 of
+    another's; a ray that starts inside a solid is marked inside."""
+    rng = np.random.default_rng(seed)
+    centres = np.array([l.origin[:] for l in scene.leaves()], np.float64)
+    a = centres[rng.integers(0, len(centres), count)] + rng.uniform(-1.5, 1.5, (count, 3))
+    b = centres[rng.integers(0, len(centres), count)] + rng.uniform(-0.3, 0.3, (count, 3))
+    d = b - a
+    n = np.linalg.norm(d, axis=1)
+    keep = n > 1e-3
+    a, d, n = a[keep], d[keep], n[keep]
+    rays = k.make_rays(a.astype(np.float32), (d / n[:, None]).astype(np.float32))
+    rays["inside"] = np.array([1 if arrays.csg_distance(p) < 0 else 0 for p in rays["origin"].astype(np.float64)], np.uint32)
+    return rays
+
+
+def pixel_rays(scene, count, seed, width, height):
+    """Camera rays through the centres of `count` random pixels of a
+    width x height image (the reference's pixelRay set, walk.cpp:227-231)."""
+    cam, _ = scene.settings()
+    rng = np.random.default_rng(seed)
+    eye = np.array(cam.eye[:], np.float64)
+    target = np.array(cam.target[:], np.float64)
+    up = np.array(cam.up[:], np.float64)
+    forward = (target - eye) / np.linalg.norm(target - eye)
+    right = np.cross(forward, up)
+    right /= np.linalg.norm(right)
+    upv = np.cross(right, forward)
+    reach = np.tan(cam.fov / 2)
+    x = rng.integers(0, width, count) + 0.5
+    y = rng.integers(0, height, count) + 0.5
+    u = 2 * x / width - 1
+    v = 1 - 2 * y / height
+    d = forward[None] + right[None] * (u * reach * width / height)[:, None] + upv[None] * (v * reach)[:, None]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return k.make_rays(np.repeat(eye[None].astype(np.float32), count, 0), d.astype(np.float32))
