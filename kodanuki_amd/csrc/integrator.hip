@@ -37,8 +37,7 @@ __global__ void k_generate(PassParams P) {
 #define KDN_LANE_REGS                                              \
     __shared__ float slots[kSlots * kTraceBlock];                  \
     __shared__ uint32_t paths[kPathLevels * kTraceBlock];          \
-    Dl buf[kMaxRegs];                                              \
-    const LaneRegs lr{buf, SlotRegs{slots + threadIdx.x, kTraceBlock}, PathRegs{paths + threadIdx.x, kTraceBlock}}
+    const LaneRegs lr{SlotRegs{slots + threadIdx.x, kTraceBlock}, PathRegs{paths + threadIdx.x, kTraceBlock}}
 
 // (One, here and below: the round has one job, read from the kernel's
 // arguments, the same for every lane.)
@@ -79,11 +78,12 @@ __global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_shadow(PassParams 
 // and each launch of them costs as long as its slowest ray): each lane
 // takes the queue's entries in turn (tally[2] counts them out).
 // (One: the round has one job, read from the kernel's arguments, and no
-// operations are counted.)
-template <bool One>
+// operations are counted. Plain: no guide of either kind and no training
+// records in the round.)
+template <bool One, bool Plain>
 __global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_finish(PassParams P, uint32_t inQueue, uint32_t* tally, double* work) {
     KDN_LANE_REGS;
-    finish_lane<One>(P, inQueue, P.s.counters[inQueue], lr, tally, work);
+    finish_lane<One, Plain>(P, inQueue, P.s.counters[inQueue], lr, tally, work);
 }
 
 // the sort keys of a queue's rays (the paths' next rays, or their shadow rays)
@@ -311,6 +311,10 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
             }
         }
     }
+    // a round with no guide of either kind and no training records runs the
+    // finishing launch with the guided sampling compiled out
+    bool plain = !guiding;
+    for (const JobDev& J : jobs) plain = plain && J.hg == nullptr && J.hg_deep == nullptr;
     // a queue's rays ordered by direction octant and origin cell, so that a
     // wave's lanes walk nearby cells alike (each path's result does not
     // depend on where it stands in a queue)
@@ -351,11 +355,17 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
             KDN_HIP(hipMemsetAsync(&s->counters[4], 0, 12, st));
             const uint32_t finishBlocks = std::min<uint32_t>((count + kTraceBlock - 1) / kTraceBlock, residentTraceBlocks(c));
             timed(&stats->finish_ms, [&] {
+                const dim3 grid(finishBlocks), block(kTraceBlock);
                 if (jobs.size() == 1 && !counting) {
-                    hipLaunchKernelGGL(k_finish<true>, dim3(finishBlocks), dim3(kTraceBlock), 0, st, P, in, &s->counters[4], nullptr);
+                    if (plain) {
+                        hipLaunchKernelGGL((k_finish<true, true>), grid, block, 0, st, P, in, &s->counters[4], nullptr);
+                    } else {
+                        hipLaunchKernelGGL((k_finish<true, false>), grid, block, 0, st, P, in, &s->counters[4], nullptr);
+                    }
+                } else if (plain) {
+                    hipLaunchKernelGGL((k_finish<false, true>), grid, block, 0, st, P, in, &s->counters[4], counting ? work + 2 : nullptr);
                 } else {
-                    hipLaunchKernelGGL(k_finish<false>, dim3(finishBlocks), dim3(kTraceBlock), 0, st, P, in, &s->counters[4],
-                                       counting ? work + 2 : nullptr);
+                    hipLaunchKernelGGL((k_finish<false, false>), grid, block, 0, st, P, in, &s->counters[4], counting ? work + 2 : nullptr);
                 }
             });
             uint32_t tally[2] = {0, 0};

@@ -392,7 +392,10 @@ KDN_D uint32_t ray_key(const TreeView& tv, const kdn_ray& r) {
 // shade path i at its hit: emission met, a shadow test toward one lamp
 // (its ray in s.shadow[i] and what it carries in s.srad; *shadow set) and
 // the scattered ray that continues the path (in s.rays[i]; returns true).
-template <bool One = false>
+// (Plain: the round has no guide of either kind and keeps no training
+// records, so the guided sampling is compiled out: the kernel keeps fewer
+// values live and spills fewer registers.)
+template <bool One = false, bool Plain = false>
 KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
     PathState& s = P.s;
     const size_t N = s.capacity;
@@ -418,7 +421,7 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
     const float tHit = hit.kind == KDN_HIT_MISSED ? kInf : hit.distance;
     const bool exactBefore = (meta & FlagExact) != 0;
     const float pdfBefore = s.pdf[i];
-    const bool training = P.recs != nullptr && i % P.train_stride == 0;
+    const bool training = !Plain && P.recs != nullptr && i % P.train_stride == 0;
 
     // lamps met along the ray (metAlong, trace.cpp:352-374), MIS against
     // aiming: quads and spheres before the hit, suns where the ray escapes
@@ -481,14 +484,14 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
         const bool exact = m.kind == KDN_GLASS && m.exact;
         // the guide's lobes at this vertex (the same test as its inference)
         const bool guidable = depth < kGuideDepths && m.kind != KDN_GLASS;
-        const bool guided = P.lobes != nullptr && guidable;
+        const bool guided = !Plain && P.lobes != nullptr && guidable;
         GuideLobes lobes;
         if (guided) load_lobes(P.lobes, N, i, lobes);
         const float share = guided ? lobes.select : 0.0f;
         // the pixel's plain guide steers the first vertex on its material
         // (steersPlace, guide.cpp:162-164)
         const float* hg = nullptr;
-        if (J.hg != nullptr && depth == 0 && !inside && s.pixel[i] != 0xFFFFFFFFu) {
+        if (!Plain && J.hg != nullptr && depth == 0 && !inside && s.pixel[i] != 0xFFFFFFFFu) {
             const float* g = J.hg + (size_t)s.pixel[i] * kHgFloats;
             if (g[0] > 0 && (uint32_t)g[1] == mi && m.kind != KDN_GLASS) hg = g;
         }
@@ -496,7 +499,7 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
         // deeper vertices of a broad surface: the survey's highlights filed
         // in their cell (the deep guide)
         HgDeepView deepView;
-        const bool deep = !hg && !guided && J.hg_deep != nullptr && depth >= 1 && !inside && m.kind != KDN_GLASS &&
+        const bool deep = !Plain && !hg && !guided && J.hg_deep != nullptr && depth >= 1 && !inside && m.kind != KDN_GLASS &&
                           m.width > kHgShinyWidth &&
                           hg_deep_view(J.hg_deep, x, n, out, mi, m, J.lamps, J.n_lamps, J.materials, &deepView);
         // aim at one lamp (aimedFrom, trace.cpp:896-947; aimLamp,
@@ -718,7 +721,7 @@ KDN_D void shadow_one(const PassParams& P, uint32_t q, const LaneRegs& lr, doubl
 // wavefront's launches, so the estimates do not change; counts segments
 // and shadow tests in tally[0] and tally[1].
 // (With One: the round has one job and counts no operations.)
-template <bool One>
+template <bool One, bool Plain>
 KDN_D void finish_lane(PassParams& P, uint32_t inQueue, uint32_t count, const LaneRegs& lr, uint32_t* tally, double* work) {
     PathState& s = P.s;
     const size_t N = s.capacity;
@@ -751,7 +754,7 @@ KDN_D void finish_lane(PassParams& P, uint32_t inQueue, uint32_t count, const La
             s.hits[i] = h;
             ++segments;
             bool shadow;
-            const bool alive = shade_path<One>(P, i, &shadow);
+            const bool alive = shade_path<One, Plain>(P, i, &shadow);
             if (shadow) {
                 ++shadows;
                 shadowTurn = true;

@@ -19,9 +19,6 @@ struct NodeDev {
     uint32_t level, x, y, z, kind, first_child, entry_first, entry_count;
 };
 
-// Registers of the trace's per-lane register file: the longest program a
-// surface leaf may hold (the octree build fails beyond it).
-constexpr int kMaxRegs = 384;
 // Slot registers of the march's LDS register file per lane: the most
 // values a program keeps live at once after the build's allocation (the
 // build fails beyond it), and the trace kernels' block size: one wave, so
@@ -60,7 +57,6 @@ struct SlotStep {
 // together), and the steps every program pool keeps readable past its
 // last program so that a fetch may run over the end.
 constexpr int kSlotFetch = 8;
-constexpr int kSsaFetch = 4;
 constexpr int kPoolPad = 16;
 
 enum NodeKind : uint32_t { KindEmpty = 0, KindFilled = 1, KindSurface = 2, KindBranch = 3 };

@@ -311,23 +311,6 @@ __global__ void k_object_sizes(uint32_t n, const kdn_object* objects, uint32_t* 
     if (i < n) sizes[i] = objects[i].count + 1;
 }
 
-// The longest program of a surface leaf: the trace's register file must
-// hold every one of them.
-__global__ void k_surface_program_max(const NodeDev* nodes, uint32_t n, const EntryDev* entries, const kdn_step* pool,
-                                      uint32_t* most) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const NodeDev node = nodes[i];
-    if (node.kind != KindSurface) return;
-    uint32_t longest = 0;
-    for (uint32_t j = 0; j < node.entry_count; ++j) {
-        const kdn_step h = pool[entries[node.entry_first + j].program];
-        const uint32_t L = (uint32_t)h.a | ((uint32_t)h.b << 16);
-        longest = L > longest ? L : longest;
-    }
-    atomicMax(most, longest);
-}
-
 // Slot registers for every program an entry names (one lane per entry): a
 // step's value takes the lowest free slot once the operands whose last use
 // it is have given theirs back, so a program needs as many slots as it
@@ -549,21 +532,6 @@ bool buildOctree(Ctx& c, const float lower[3], const float size[3], int depth, k
         t.level_first.push_back((uint32_t)childBase);
         t.level_count.push_back((uint32_t)nChildren);
         t.n_nodes = childBase + nChildren;
-    }
-    {
-        if (!ensure(bufC, 64, c.error, false)) return false;
-        uint32_t* most = (uint32_t*)bufC.ptr;
-        KDN_HIP(hipMemsetAsync(most, 0, 4, s));
-        hipLaunchKernelGGL(k_surface_program_max, dim3(blocksFor(t.n_nodes)), dim3(256), 0, s, (const NodeDev*)t.nodes.ptr,
-                           (uint32_t)t.n_nodes, (const EntryDev*)t.entries.ptr, (const kdn_step*)t.pool.ptr, most);
-        uint32_t longest = 0;
-        KDN_HIP(hipMemcpyAsync(&longest, most, 4, hipMemcpyDeviceToHost, s));
-        KDN_HIP(hipStreamSynchronize(s));
-        if (longest > (uint32_t)kMaxRegs) {
-            c.error = "a surface leaf's program of " + std::to_string(longest) + " steps passes the trace's register file of " +
-                      std::to_string(kMaxRegs) + " (a deeper octree cuts programs shorter)";
-            return false;
-        }
     }
     {
         // the march's programs on slot registers

@@ -11,8 +11,7 @@ __global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_extend(TreeView tv
     __shared__ uint32_t paths[kPathLevels * kTraceBlock];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    Dl buf[kMaxRegs];
-    const LaneRegs lr{buf, SlotRegs{slots + threadIdx.x, kTraceBlock}, PathRegs{paths + threadIdx.x, kTraceBlock}};
+    const LaneRegs lr{SlotRegs{slots + threadIdx.x, kTraceBlock}, PathRegs{paths + threadIdx.x, kTraceBlock}};
     kdn_hit hit;
     trace_ray(tv, rays[i], lr, hit);
     hits[i] = hit;

@@ -42,21 +42,6 @@ struct TreeView {
     double* work;  // when set, the FP32 operations of every evaluation are added here
 };
 
-struct Regs {
-    Dl* d;
-    KDN_D float& operator[](uint32_t i) const {
-        KDN_CHECK(i < (uint32_t)kMaxRegs);
-        return d[i].v;
-    }
-};
-struct DRegs {
-    Dl* d;
-    KDN_D Dl& operator[](uint32_t i) const {
-        KDN_CHECK(i < (uint32_t)kMaxRegs);
-        return d[i];
-    }
-};
-
 // A lane's slot registers: slot k at s[k * stride] (in LDS, one column per
 // lane of the block, so a wave's lanes touch consecutive banks).
 struct SlotRegs {
@@ -80,10 +65,8 @@ struct PathRegs {
 };
 
 // A lane's register files: the slot registers every march evaluation runs
-// in, its octree ancestry, and the private SSA registers of a landing's
-// dual evaluation and source walk.
+// in and its octree ancestry.
 struct LaneRegs {
-    Dl* ssa;
     SlotRegs slots;
     PathRegs path;
 };
@@ -126,44 +109,7 @@ KDN_D float eval_slots(const TreeView& tv, uint32_t at, const SlotRegs& r, float
         for (int k = 0; k + 1 < kSlotFetch; ++k) q[k] = q[k + 1];
         --left;
     }
-    return r[head.code >> 24];
-}
-
-// SSA steps of a program, fetched kSsaFetch at a time (as eval_slots).
-template <class F>
-KDN_D void for_each_step(const kdn_step* prog, uint32_t L, F&& f) {
-    kdn_step q[kSsaFetch];
-    uint32_t left = 0;
-    for (uint32_t i = 0; i < L; ++i) {
-        if (left == 0) {
-#pragma unroll
-            for (int k = 0; k < kSsaFetch; ++k) q[k] = prog[k];
-            prog += kSsaFetch;
-            left = kSsaFetch;
-        }
-        f(i, q[0]);
-#pragma unroll
-        for (int k = 0; k + 1 < kSsaFetch; ++k) q[k] = q[k + 1];
-        --left;
-    }
-}
-
-KDN_D float eval_program(const TreeView& tv, uint32_t at, const Regs& r, float px, float py, float pz) {
-    const kdn_step* pool = tv.pool;
-    const kdn_step h = pool[at];
-    if (tv.work) *tv.work += h.value;
-    const uint32_t L = (uint32_t)h.a | ((uint32_t)h.b << 16);
-    for_each_step(pool + at + 1, L, [&](uint32_t i, const kdn_step& st) { r[i] = point_step(st, r, px, py, pz); });
-    return r[h.c];
-}
-
-KDN_D void eval_dual(const TreeView& tv, uint32_t at, const DRegs& r, float px, float py, float pz, Dl* out) {
-    const kdn_step* pool = tv.pool;
-    const kdn_step h = pool[at];
-    if (tv.work) *tv.work += 4 * h.value;
-    const uint32_t L = (uint32_t)h.a | ((uint32_t)h.b << 16);
-    for_each_step(pool + at + 1, L, [&](uint32_t i, const kdn_step& st) { r[i] = dual_step(st, r, px, py, pz); });
-    *out = r[h.c];
+    return r[(head.code >> 24) & 15u];
 }
 
 struct LeafField {
@@ -210,16 +156,95 @@ KDN_D void faced_normal(const float g[3], const float d[3], bool inside, float n
     n[0] = u[0]; n[1] = u[1]; n[2] = u[2];
 }
 
+// The source of a value (program.cpp:442-502): the leaf that decides it,
+// and, where a smooth minimum blends two solids on its way, the other
+// leaf and the share the blend gives it.
+struct Source {
+    uint32_t leaf, other;
+    float share;
+};
+
+// A program on slot registers evaluated in duals at a point, each value's
+// source carried forward with it. The reference walks back from the
+// result through the joins (min, smooth-min, max, negate, select) to the
+// deciding step, keeping the outermost smooth-min blend it passes; a
+// value here takes its chosen operand's source at each join, the blend of
+// a smooth-min in its band replacing what the operand carried, so the
+// result's source is the walk's. With Dual false only the values and
+// sources are made. Registers are private: kSlots duals and sources a
+// lane, the most values a program keeps live.
+template <bool Dual>
+KDN_D Dl eval_sourced(const TreeView& tv, uint32_t at, float px, float py, float pz, Source* source) {
+    Dl reg[kSlots];
+    Source src[kSlots];
+    struct View {
+        Dl* r;
+        KDN_D Dl& operator[](uint32_t i) const {
+            KDN_CHECK(i < (uint32_t)kSlots);
+            return r[i];
+        }
+    } const view{reg};
+    struct Values {
+        const Dl* r;
+        KDN_D float operator[](uint32_t i) const {
+            KDN_CHECK(i < (uint32_t)kSlots);
+            return r[i].v;
+        }
+    } const values{reg};
+    const SlotStep head = tv.spool[at];
+    if (tv.work) *tv.work += (Dual ? 4 : 1) * head.value;
+    const uint32_t L = head.code & 0xFFFFFFu;
+    for (uint32_t i = 0; i < L; ++i) {
+        const uint32_t code = tv.spool[at + 1 + i].code;
+        kdn_step st;
+        st.op = (uint8_t)(code & 63u);
+        st.flags = 0;
+        st.a = (uint16_t)((code >> 10) & 15u);
+        st.b = (uint16_t)((code >> 14) & 15u);
+        st.c = (uint16_t)((code >> 18) & 15u);
+        st.value = tv.spool[at + 1 + i].value;
+        st.leaf = 0;
+        Dl v;
+        if (Dual) {
+            v = dual_step(st, view, px, py, pz);
+        } else {
+            v = dl(point_step(st, values, px, py, pz));
+        }
+        Source sv;
+        if (is_join(st.op)) {
+            const uint32_t next = chosen_operand(st, values);
+            sv = src[next & 15u];
+            if (st.op == KDN_SMOOTH_MINIMUM) {
+                const float radius = values[st.c];
+                const float apart = __builtin_fabsf(values[st.a] - values[st.b]);
+                const float sh = (radius > 0 && apart < radius) ? 0.5f * (radius - apart) / radius : 0.0f;
+                if (sh > 0) {
+                    sv.other = src[(next == st.a ? st.b : st.a) & 15u].leaf;
+                    sv.share = sh;
+                }
+            }
+        } else {
+            sv = Source{tv.pool[at + 1 + i].leaf, KDN_NO_LEAF, 0.0f};
+        }
+        const uint32_t dst = (code >> 6) & 15u;
+        reg[dst] = v;
+        src[dst] = sv;
+    }
+    const uint32_t result = (head.code >> 24) & 15u;
+    *source = src[result];
+    return reg[result];
+}
+
 // The hit at t from entry `winner` of leaf n. The march stops within its
 // tolerance of the surface; one Newton step along the ray, t - v / (g.d),
 // bounded and taken only where the ray is not grazing, brings the distance
 // onto the surface.
 KDN_D void land(const TreeView& tv, const NodeDev& n, uint32_t nodeIndex, uint32_t winner, const float o[3], const float d[3], float t,
-                bool inside, Dl* buf, kdn_hit& hit) {
+                bool inside, kdn_hit& hit) {
     const float px = o[0] + d[0] * t, py = o[1] + d[1] * t, pz = o[2] + d[2] * t;
     const uint32_t at = tv.entries[n.entry_first + winner].program;
-    Dl value;
-    eval_dual(tv, at, DRegs{buf}, px, py, pz, &value);
+    Source src;
+    const Dl value = eval_sourced<true>(tv, at, px, py, pz, &src);
     const float slope = value.g[0] * d[0] + value.g[1] * d[1] + value.g[2] * d[2];
     const float glen = __builtin_sqrtf(value.g[0] * value.g[0] + value.g[1] * value.g[1] + value.g[2] * value.g[2]);
     const float bound = kNewtonReach * (kTolAbs + kTolRel * t);
@@ -232,8 +257,9 @@ KDN_D void land(const TreeView& tv, const NodeDev& n, uint32_t nodeIndex, uint32
     hit.distance = tr;
     hit.point[0] = o[0] + d[0] * tr; hit.point[1] = o[1] + d[1] * tr; hit.point[2] = o[2] + d[2] * tr;
     faced_normal(value.g, d, inside, hit.normal);
-    const kdn_step h = tv.pool[at];
-    source_of(tv.pool + at + 1, h.c, Regs{buf}, &hit.leaf, &hit.other, &hit.share);
+    hit.leaf = src.leaf;
+    hit.other = src.other;
+    hit.share = src.share;
     hit.cell = nodeIndex;
 }
 
@@ -284,7 +310,6 @@ KDN_D float slope_of(float part) {
 // lane that marches its path's rays and its shadow rays by turns passes it
 // per ray, so that a wave's lanes run one march between them.)
 KDN_D void march_any(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr, kdn_hit& hit, int& iterations, const bool AnyHit) {
-    Dl* const buf = lr.ssa;
     const float o[3] = {ray.origin[0], ray.origin[1], ray.origin[2]};
     const float d[3] = {ray.direction[0], ray.direction[1], ray.direction[2]};
     const bool inside = ray.inside != 0;
@@ -362,7 +387,7 @@ KDN_D void march_any(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr,
                             return;
                         }
                         const uint32_t w = winner_at(tv, n, r, o, d, t, inset, f);
-                        land(tv, n, index, w, o, d, t, false, buf, hit);
+                        land(tv, n, index, w, o, d, t, false, hit);
                         return;
                     }
                     step = f.least;
@@ -373,16 +398,17 @@ KDN_D void march_any(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr,
                         if (past.least > 0) {
                             const LeafField here = leaf_field(tv, n, r, px, py, pz);
                             const uint32_t w = winner_at(tv, n, r, o, d, t, inset, here);
-                            land(tv, n, index, w, o, d, t, true, buf, hit);
+                            land(tv, n, index, w, o, d, t, true, hit);
                             return;
                         }
                         // another solid past the crossing: a hit where it is of
-                        // another material (sphereCrossing, sphere_trace.cpp:176-187)
+                        // another material (sphereCrossing, sphere_trace.cpp:176-187),
+                        // its field taken past the crossing; the landing's Newton
+                        // step from there puts the hit on that solid's surface
+                        // (the reference reports t, within its FP64 tolerance of it)
                         kdn_hit other;
-                        land(tv, n, index, past.arg, o, d, t + inset, true, buf, other);
+                        land(tv, n, index, past.arg, o, d, t + inset, true, other);
                         if (other.leaf != KDN_NO_LEAF && tv.leafMaterial[other.leaf] != ray.within) {
-                            other.distance = t;
-                            other.point[0] = px; other.point[1] = py; other.point[2] = pz;
                             hit = other;
                             return;
                         }
@@ -409,31 +435,19 @@ KDN_D void march_any(const TreeView& tv, const kdn_ray& ray, const LaneRegs& lr,
             hit.normal[0] = hit.normal[1] = hit.normal[2] = 0;
             hit.normal[axis] = d[axis] > 0 ? -1.0f : 1.0f;
             hit.cell = index;
-            // the solid's leaf from its program where the register file
-            // holds it (a filled cell's program may be longer than any
-            // surface leaf's; the build checks surface leaves only)
+            // the solid's leaf: the source of its first program's value
             if (!AnyHit && n.entry_count > 0) {
-                const uint32_t at = tv.entries[n.entry_first].program;
-                const kdn_step head = tv.pool[at];
-                if (((uint32_t)head.a | ((uint32_t)head.b << 16)) <= (uint32_t)kMaxRegs) {
-                    const Regs ssa{buf};
-                    (void)eval_program(tv, at, ssa, hit.point[0], hit.point[1], hit.point[2]);
-                    float sh;
-                    source_of(tv.pool + at + 1, head.c, ssa, &hit.leaf, &hit.other, &sh);
-                    hit.share = sh;
-                }
+                Source src;
+                (void)eval_sourced<false>(tv, tv.entries[n.entry_first].program, hit.point[0], hit.point[1], hit.point[2], &src);
+                hit.leaf = src.leaf;
+                hit.other = src.other;
+                hit.share = src.share;
             }
             return;
-        } else if (n.kind == KindEmpty && inside) {
-            // a ray from inside stands in empty space: it left the solid at t
-            hit.kind = KDN_HIT_SURFACE;
-            hit.distance = t;
-            hit.point[0] = o[0] + d[0] * t; hit.point[1] = o[1] + d[1] * t; hit.point[2] = o[2] + d[2] * t;
-            hit.normal[0] = hit.normal[1] = hit.normal[2] = 0;
-            hit.normal[axis] = d[axis] > 0 ? 1.0f : -1.0f;
-            hit.cell = index;
-            return;
         }
+        // (a ray from inside crosses Empty leaves as the reference's inside
+        // march crosses open space, sphere_trace.cpp:189-201: it lands
+        // where it next comes within its tolerance of a surface)
         ++iterations;
         if (tExit >= t1) return;
         t = fmax_(t, tExit);

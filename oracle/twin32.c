@@ -985,7 +985,6 @@ void or_free_octree(or_octree* t) {
 #define MAX_ITER 4096
 static const float kTolAbs = 1e-5f, kTolRel = 2e-6f, kTie = 1e-6f, kGraze = 0.02f;
 static const float kNewtonFacing = 0.01f, kNewtonReach = 100.0f;
-static const uint32_t kMaxRegs = 384; /* the GPU trace's register file */
 
 typedef struct {
     const or_octree* t;
@@ -1271,11 +1270,9 @@ static void trace_one(const View* v, const kdn_ray* ray, kdn_hit* hit) {
                         }
                         kdn_hit other;
                         land(v, &n, index, past.arg, o, d, t + inset, 1, &other);
+                        /* the crossing's hit keeps the landing's Newton step
+                         * onto the solid past it (the device's rule) */
                         if (other.leaf != KDN_NO_LEAF && v->leafMaterial[other.leaf] != ray->within) {
-                            other.distance = t;
-                            other.point[0] = px;
-                            other.point[1] = py;
-                            other.point[2] = pz;
                             *hit = other;
                             return;
                         }
@@ -1302,9 +1299,8 @@ static void trace_one(const View* v, const kdn_ray* ray, kdn_hit* hit) {
             hit->normal[0] = hit->normal[1] = hit->normal[2] = 0;
             hit->normal[axis] = d[axis] > 0 ? -1.0f : 1.0f;
             hit->cell = index;
-            /* the GPU's register file holds kMaxRegs values: a longer
-             * program of a filled cell leaves the solid's leaf unnamed */
-            if (n.entry_count > 0 && header_length(&v->t->pool[v->t->entries[n.entry_first].program]) <= kMaxRegs) {
+            /* the solid's leaf: the source of its first program's value */
+            if (n.entry_count > 0) {
                 const uint32_t at = v->t->entries[n.entry_first].program;
                 (void)eval_prog(v, at, hit->point[0], hit->point[1], hit->point[2]);
                 float sh;
@@ -1312,17 +1308,9 @@ static void trace_one(const View* v, const kdn_ray* ray, kdn_hit* hit) {
                 hit->share = sh;
             }
             return;
-        } else if (n.kind == 0 && inside) {
-            hit->kind = KDN_HIT_SURFACE;
-            hit->distance = t;
-            hit->point[0] = o[0] + d[0] * t;
-            hit->point[1] = o[1] + d[1] * t;
-            hit->point[2] = o[2] + d[2] * t;
-            hit->normal[0] = hit->normal[1] = hit->normal[2] = 0;
-            hit->normal[axis] = d[axis] > 0 ? 1.0f : -1.0f;
-            hit->cell = index;
-            return;
         }
+        /* a ray from inside crosses Empty leaves (the reference's inside march
+         * crosses open space, sphere_trace.cpp:189-201) */
         ++iterations;
         if (tExit >= t1) return;
         t = fmax_(t, tExit);
