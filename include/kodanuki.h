@@ -401,6 +401,16 @@ typedef struct kdn_denoise_frame {
 int kdn_denoise(kdn_ctx* ctx, const kdn_denoise_frame* frame, const kdn_denoise_pixel* pixels, float* rgb);
 /* The same on device buffers, on a stream (NULL: the context's). */
 int kdn_denoise_device(kdn_ctx* ctx, const kdn_denoise_frame* frame, const void* pixels, void* rgb, void* stream);
+/* kdn_denoise_device that allocates nothing (no counterpart in the
+ * reference): for callers that cannot allocate device memory, such as code
+ * under HIP graph capture or a serving loop's inner steps. The filter's
+ * working memory is a static array of every GPU the library runs on (76 MiB,
+ * held until the process ends), so frames are limited to 2^20 pixels
+ * (KDN_E_ARGUMENT beyond); pixels and rgb are the caller's device buffers.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (other streams, other threads) race and must be ordered by the
+ * caller. Results are the same bits as kdn_denoise_device's. */
+int kdn_denoise_device_static(kdn_ctx* ctx, const kdn_denoise_frame* frame, const void* pixels, void* rgb, void* stream);
 typedef struct kdn_denoise_stats {
     double gbuffer_ms;   /* primary rays and the history, HIP events */
     double filter_ms;    /* the filter's kernels */

@@ -207,7 +207,7 @@ EXPORTS = [
     "kdn_scene_region", "kdn_open", "kdn_close", "kdn_last_error", "kdn_global_error", "kdn_upload_scene",
     "kdn_build_octree", "kdn_octree_sizes", "kdn_octree_download", "kdn_extend", "kdn_extend_device",
     "kdn_render_pass", "kdn_read_accumulation", "kdn_read_image",
-    "kdn_denoise", "kdn_denoise_device", "kdn_denoise_accumulation",
+    "kdn_denoise", "kdn_denoise_device", "kdn_denoise_device_static", "kdn_denoise_accumulation",
     "kdn_guide_reset", "kdn_guide_configure", "kdn_guide_describe", "kdn_guide_params", "kdn_guide_set_params", "kdn_guide_infer",
     "kdn_guide_gradients", "kdn_guide_train",
     "kdn_serve_open", "kdn_serve_submit", "kdn_serve_step", "kdn_serve_status", "kdn_serve_result",
@@ -263,6 +263,7 @@ def _bind(lib):
         "kdn_read_image": (C.c_int, [P, P]),
         "kdn_denoise": (C.c_int, [P, C.POINTER(DenoiseFrame), P, P]),
         "kdn_denoise_device": (C.c_int, [P, C.POINTER(DenoiseFrame), P, P, P]),
+        "kdn_denoise_device_static": (C.c_int, [P, C.POINTER(DenoiseFrame), P, P, P]),
         "kdn_denoise_accumulation": (C.c_int, [P, P, C.POINTER(DenoiseStats)]),
         "kdn_highlight_guides": (C.c_int, [P, C.c_int32, C.c_int32, C.POINTER(HighlightStats)]),
         "kdn_highlight_guide_at": (C.c_int, [P, C.c_int32, C.c_int32, P]),
@@ -482,6 +483,17 @@ class Device:
         rgb = np.zeros((h, w, 3), np.float32)
         _check(library().kdn_denoise(self._h, C.byref(f), _ptr(px), _ptr(rgb)), self._h)
         return rgb
+
+    def denoise_device(self, width, height, pixels_ptr, rgb_ptr, forward, right, up, reach, centre=(0.5, 0.5),
+                       demodulated=False, static=False, stream=None):
+        """The filter on device buffers (pointers): width x height history
+        pixels in, rgb out. static: the entry point that allocates nothing
+        (frames up to 2^20 pixels)."""
+        f = DenoiseFrame(width, height, (C.c_double * 3)(*forward), (C.c_double * 3)(*right), (C.c_double * 3)(*up), reach,
+                         width / height, centre[0], centre[1], 1 if demodulated else 0, 0)
+        call = library().kdn_denoise_device_static if static else library().kdn_denoise_device
+        _check(call(self._h, C.byref(f), C.c_void_p(pixels_ptr), C.c_void_p(rgb_ptr), C.c_void_p(stream) if stream else None),
+               self._h)
 
     def denoise_accumulation(self, width, height, stats=False):
         rgb = np.zeros((height, width, 3), np.float32)

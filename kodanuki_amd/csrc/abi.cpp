@@ -406,6 +406,12 @@ int kdn_open(int device, kdn_ctx** out) {
         delete ctx;
         return fail("kdn_open: hipStreamCreate failed", KDN_E_DEVICE);
     }
+    if (!kdn::denoiseStaticScratch(ctx->c)) {
+        const std::string why = ctx->c.error;
+        hipStreamDestroy(ctx->c.stream);
+        delete ctx;
+        return fail("kdn_open: " + why, KDN_E_DEVICE);
+    }
     *out = ctx;
     return KDN_OK;
 }

@@ -79,6 +79,7 @@ struct Ctx {
     DevBuf hguides;                 // the plain guide: a packed guide per pixel
     int hguide_width = 0, hguide_height = 0;  // the image they were made for (0: none)
     size_t hguide_deep = 0;         // byte offset of the deep guides' filing in hguides (0: none)
+    float* denoise_static = nullptr;  // this device's static denoiser working memory (denoise.hip)
 };
 
 // Glass scattering is held off the device: renders with glass faulted the
@@ -106,6 +107,9 @@ void freeIntegrator(Ctx& c);
 bool guideReset(Ctx& c, uint32_t seed, const float lower[3], const float size[3]);
 bool guideReady(const Ctx& c);
 void freeGuide(Ctx& c);
+
+// denoise.hip: look up this device's static denoiser working memory
+bool denoiseStaticScratch(Ctx& c);
 
 // hguide.hip: the plain guide of every pixel of a width x height image
 bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats, DevBuf& out, DevBuf& work, size_t* deepOffset);

@@ -116,12 +116,19 @@ DenoiseLensDev lensOf(const kdn_denoise_frame& f) {
     return L;
 }
 
-// The filter on device buffers: pixels in, rgb out, scratch from the
-// context (19 floats a pixel).
-bool denoiseDevice(Ctx& c, const kdn_denoise_frame& f, const kdn_denoise_pixel* pixels, float* rgb, hipStream_t st) {
+// The filter's working memory for the entry point that allocates nothing:
+// its 19 fields of up to kStaticPixels pixels (76 MiB), held by every GPU
+// the library runs on until the process ends. Host code reaches it only
+// through the address hipGetSymbolAddress gives on each device
+// (denoiseStaticScratch); every call on a GPU shares that GPU's array.
+constexpr size_t kStaticPixels = size_t(1) << 20;
+constexpr size_t kDenoiseFields = 19;
+static __device__ __align__(256) float scratch[kStaticPixels * kDenoiseFields];
+
+// The filter on device buffers: pixels in, rgb out, its 19 float fields a
+// pixel in `base` (n floats each, one after another).
+bool denoiseFields(Ctx& c, const kdn_denoise_frame& f, const kdn_denoise_pixel* pixels, float* rgb, hipStream_t st, float* base) {
     const size_t n = (size_t)f.width * f.height;
-    if (!ensure(c.scratch[4], n * 19 * sizeof(float) + 256, c.error, false)) return false;
-    float* base = (float*)c.scratch[4].ptr;
     DenoiseBuffers B;
     B.width = f.width;
     B.height = f.height;
@@ -142,6 +149,13 @@ bool denoiseDevice(Ctx& c, const kdn_denoise_frame& f, const kdn_denoise_pixel* 
     return true;
 }
 
+// The filter with its fields in the context's scratch (grown on demand).
+bool denoiseDevice(Ctx& c, const kdn_denoise_frame& f, const kdn_denoise_pixel* pixels, float* rgb, hipStream_t st) {
+    const size_t n = (size_t)f.width * f.height;
+    if (!ensure(c.scratch[4], n * kDenoiseFields * sizeof(float) + 256, c.error, false)) return false;
+    return denoiseFields(c, f, pixels, rgb, st, (float*)c.scratch[4].ptr);
+}
+
 bool checkFrame(Ctx& c, const kdn_denoise_frame* f) {
     if (!f || f->width <= 0 || f->height <= 0 || f->width > 16384 || f->height > 16384) {
         c.error = "kdn_denoise: frame size out of 1..16384";
@@ -151,6 +165,20 @@ bool checkFrame(Ctx& c, const kdn_denoise_frame* f) {
 }
 
 }  // namespace
+
+// The device address of this GPU's static working memory, looked up once
+// per context with its device current (kdn_open calls it).
+bool denoiseStaticScratch(Ctx& c) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr) {
+        c.error = std::string("hipGetSymbolAddress of the denoiser's static working memory: ") + hipGetErrorString(e);
+        return false;
+    }
+    c.denoise_static = (float*)ptr;
+    return true;
+}
+
 }  // namespace kdn
 
 using namespace kdn;
@@ -164,6 +192,24 @@ int kdn_denoise_device(kdn_ctx* ctx, const kdn_denoise_frame* frame, const void*
     hipSetDevice(c.device);
     hipStream_t st = stream ? (hipStream_t)stream : c.stream;
     return denoiseDevice(c, *frame, (const kdn_denoise_pixel*)pixels, (float*)rgb, st) ? KDN_OK : KDN_E_DEVICE;
+}
+
+int kdn_denoise_device_static(kdn_ctx* ctx, const kdn_denoise_frame* frame, const void* pixels, void* rgb, void* stream) {
+    if (!ctx || !pixels || !rgb) return KDN_E_ARGUMENT;
+    Ctx& c = ctx->c;
+    if (!checkFrame(c, frame)) return KDN_E_ARGUMENT;
+    if ((size_t)frame->width * frame->height > kStaticPixels) {
+        c.error = "kdn_denoise_device_static: frame of " + std::to_string((size_t)frame->width * frame->height) +
+                  " pixels passes the static working memory's " + std::to_string(kStaticPixels) + " (kdn_denoise_device has no limit)";
+        return KDN_E_ARGUMENT;
+    }
+    if (c.denoise_static == nullptr) {
+        c.error = "kdn_denoise_device_static: the static working memory was not found at kdn_open";
+        return KDN_E_STATE;
+    }
+    hipSetDevice(c.device);
+    hipStream_t st = stream ? (hipStream_t)stream : c.stream;
+    return denoiseFields(c, *frame, (const kdn_denoise_pixel*)pixels, (float*)rgb, st, c.denoise_static) ? KDN_OK : KDN_E_DEVICE;
 }
 
 int kdn_denoise(kdn_ctx* ctx, const kdn_denoise_frame* frame, const kdn_denoise_pixel* pixels, float* rgb) {

@@ -47,6 +47,7 @@ extern "C" __attribute__((used, visibility("default"))) inline const uint64_t* k
 #define __global__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+#define __align__(n) alignas(n)
 // a block's shared memory: one array for every thread (threads run one
 // after another here, each in its own column)
 #define __shared__ static
@@ -83,6 +84,12 @@ inline hipError_t hipGetDeviceCount(int* n) {
     return hipSuccess;
 }
 inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
+// a __device__ variable is host memory here: its symbol is its address
+#define HIP_SYMBOL(x) (&(x))
+inline hipError_t hipGetSymbolAddress(void** p, const void* symbol) {
+    *p = const_cast<void*>(symbol);
+    return hipSuccess;
+}
 
 template <class T>
 inline hipError_t hipMalloc(T** p, size_t bytes) {
