@@ -243,7 +243,7 @@ def main():
     ap.add_argument("--no-guide", action="store_true", help="skip the guided renders")
     ap.add_argument("--guide-passes", type=int, default=8)
     ap.add_argument("--serve-jobs", type=int, default=32)
-    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals, jobs/s")
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[2.0, 8.0, 32.0], help="Poisson arrivals, jobs/s")
     ap.add_argument("--serve-seed", type=int, default=7)
     ap.add_argument("--serve-paths", type=int, default=1 << 23, help="paths of one round")
     ap.add_argument("--serve-budget-gib", type=int, default=192)
@@ -378,11 +378,12 @@ def main():
 
     serving = serving_guided = None
     if not args.no_serve:
-        serving = serving_stream(k, scenes, dev, args.serve_jobs, args.serve_rate, args.serve_seed, args.serve_paths,
-                                 args.serve_budget_gib << 30)
+        # the stream at each arrival rate (jobs/s), the same jobs each time
+        serving = [serving_stream(k, scenes, dev, args.serve_jobs, rate, args.serve_seed, args.serve_paths,
+                                  args.serve_budget_gib << 30) for rate in args.serve_rates]
         if not args.no_guide:
             # the same stream served with the plain guide on every job
-            serving_guided = serving_stream(k, scenes, dev, args.serve_jobs, args.serve_rate, args.serve_seed, args.serve_paths,
+            serving_guided = serving_stream(k, scenes, dev, args.serve_jobs, args.serve_rates[0], args.serve_seed, args.serve_paths,
                                             args.serve_budget_gib << 30, guided=True)
 
     cpu = None

@@ -132,3 +132,45 @@ def test_matches_fp64_oracle(device):
     noise = np.mean(blocks(vg / spp_cpu / 9) / (bc ** 2 + 0.01))
     note(f"[oracle] block relMSE {rel:.2e} (oracle noise {noise:.2e})")
     assert rel < 1e-3 + 3 * noise
+
+
+@pytest.mark.skipif(bool(EMULATION), reason="a million device samples and a 256x256 oracle render: device only")
+def test_config0_matches_fp64_oracle(device):
+    """The plain scene at its own size (32 primitives smooth-merged over a
+    plane, one quad lamp, 256x256, 16 spp, 6 bounces, octree depth 6)
+    against the FP64 oracle's render of the same 16 samples of every pixel
+    (the same counter-based numbers: paired estimates), and against an
+    independent 256-spp device render (per-pixel means within 3 standard
+    errors; 8x8 block means within relMSE 1e-3 of each other beyond the
+    oracle's own noise)."""
+    text = scenes.plain(0)
+    s = k.Scene(text)
+    cam, rs = s.settings()
+    W, H, spp = rs.width, rs.height, rs.samples
+    assert (W, H, spp, rs.surfaces) == (256, 256, 16, 7)
+    _, lower, size, g16, _, n16, _ = _render(device, text, 6, spp, w=W, h=H, surfaces=rs.surfaces, seed=rs.seed)
+    mg16 = g16 / n16[..., None]
+    _, _, _, ghi, sqhi, nhi, st = _render(device, text, 6, 256, w=W, h=H, surfaces=rs.surfaces, seed=rs.seed, first=1 << 20)
+    mhi = ghi / nhi[..., None]
+    var = np.maximum(sqhi / nhi[..., None] - mhi ** 2, 0)
+    A = O.SceneArrays(s)
+    cpu = O.render64(s, A, lower, size, W, H, spp, seed=rs.seed, surfaces=rs.surfaces)
+    mc = cpu / spp
+    se16 = np.sqrt(var / spp) + 1e-4
+    paired = np.abs(mg16 - mc) / se16
+    unpaired = np.abs(mhi - mc) / (np.sqrt(var / spp + var / 256) + 1e-4)
+    rel16 = np.mean((mg16 - mc) ** 2 / (mc ** 2 + 0.01))
+    note(f"[config0] gpu 16 spp vs oracle 16 spp (same numbers): within 3se {np.mean(paired < 3):.5f}, relMSE {rel16:.2e}; "
+         f"gpu 256 spp (other numbers) vs oracle: within 3se {np.mean(unpaired < 3):.5f}; means gpu {mhi.mean(axis=(0, 1))} "
+         f"oracle {mc.mean(axis=(0, 1))}")
+    assert np.mean(paired < 3) > 0.99
+    assert np.mean(unpaired < 3) > 0.98
+
+    def blocks(img):
+        return img.reshape(H // 8, 8, W // 8, 8, 3).mean(axis=(1, 3))
+
+    bg, bc = blocks(mhi), blocks(mc)
+    rel = np.mean((bg - bc) ** 2 / (bc ** 2 + 0.01))
+    noise = np.mean(blocks(var / spp / 64) / (bc ** 2 + 0.01))
+    note(f"[config0] 8x8 block relMSE {rel:.2e} (oracle noise {noise:.2e})")
+    assert rel < 1e-3 + 3 * noise

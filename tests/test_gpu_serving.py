@@ -219,3 +219,47 @@ def test_guided_jobs_in_a_mix_match_jobs_alone(device, room, scene_set):
         for i, j in enumerate(ids):
             assert server.status(j).state == k.JOB_DONE, server.job_error(j)
             assert _same(server.result(j), alone[i]), i
+
+
+@pytest.mark.skipif(bool(EMULATION), reason="scenes of up to 4096 primitives at depth 8: device only")
+def test_pool_stream_jobs_match_jobs_alone(device):
+    """The serving stream's own scene pool (eight seeded scenes of 32 to 4096
+    primitives, octree depths 6 to 8) with a seeded Poisson arrival order at
+    rates 2, 8 and 32 jobs/s: arrivals are mapped to server steps (a round
+    takes about a tenth of a second at these sizes), so the mix is fixed by
+    the seed; every job's accumulators equal the same job served alone."""
+    import random
+    pool = [(k.Scene(text), depth) for text, depth in scenes.serving_pool(glass=GLASS)]
+    sizes = [(64, 64), (128, 96), (160, 90), (256, 256)]
+    spps = [1, 4, 16]
+    rng = random.Random(5)
+    for rate in (2.0, 8.0, 32.0):
+        t, stream = 0.0, []
+        for _ in range(10):
+            t += rng.expovariate(rate)
+            w, h = rng.choice(sizes)
+            stream.append((int(t / 0.1), rng.randrange(len(pool)), w, h, rng.choice(spps), rng.randrange(1, 100)))
+        alone = []
+        with k.Server(device, max_paths=1 << 18) as server:
+            for _, si, w, h, spp, seed in stream:
+                scene, depth = pool[si]
+                j = server.submit(scene, w, h, spp, seed=seed, surfaces=6, depth=depth)
+                server.run([j])
+                assert server.status(j).state == k.JOB_DONE, server.job_error(j)
+                alone.append(server.result(j))
+                server.release(j)
+        with k.Server(device, max_paths=1 << 18) as server:
+            ids, pending, steps, most = [], list(stream), 0, 0
+            while pending or any(server.status(j).state not in (k.JOB_DONE, k.JOB_FAILED) for j in ids):
+                while pending and pending[0][0] <= steps:
+                    _, si, w, h, spp, seed = pending.pop(0)
+                    scene, depth = pool[si]
+                    ids.append(server.submit(scene, w, h, spp, seed=seed, surfaces=6, depth=depth))
+                st = server.step()
+                most = max(most, st.jobs_in_round)
+                steps += 1
+            for i, j in enumerate(ids):
+                assert server.status(j).state == k.JOB_DONE, server.job_error(j)
+                assert _same(server.result(j), alone[i]), (rate, i, stream[i])
+        note(f"[serving] pool stream at {rate} jobs/s: {len(stream)} jobs over {steps} steps, up to {most} jobs in a round, "
+             f"each equal alone and in the mix")
