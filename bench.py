@@ -66,7 +66,8 @@ def cpu_baseline(scene, lower, size, width, height, seed, surfaces, budget_s=12.
                       f"(FP64 sphere trace over every object program), {threads} threads, {dt:.1f} s"}
 
 
-def serving_stream(k, scenes, dev, jobs, rate, seed, max_paths, budget, sizes=None, spps=None, pool_texts=None, guided=False):
+def serving_stream(k, scenes, dev, jobs, rate, seed, max_paths, budget, sizes=None, spps=None, pool_texts=None, guided=False,
+                   alone=False):
     """The serving stream: `jobs` render jobs with Poisson arrivals at
     `rate` jobs/s, each a scene of the eight-scene pool (32 to 4096
     primitives), a size of {256^2, 512^2, 1024^2, 1920x1080} and
@@ -75,7 +76,8 @@ def serving_stream(k, scenes, dev, jobs, rate, seed, max_paths, budget, sizes=No
     to the end of the round that folds its last sample; scene builds are
     part of the first job of each scene. With guided, every job samples
     its first vertices with the plain guide (its survey at admission is
-    part of its latency)."""
+    part of its latency). With alone, the same jobs are also served one at
+    a time back to back, for the serving loop's efficiency."""
     import random
     rng = random.Random(seed)
     pool = []
@@ -133,6 +135,20 @@ def serving_stream(k, scenes, dev, jobs, rate, seed, max_paths, budget, sizes=No
     server.close()
     wall_s = (last_finish - base_ms) / 1e3
     lat = sorted(latency)
+    alone_s = None
+    if alone:
+        # the same jobs served one at a time, back to back with no arrival
+        # gaps (each its own rounds): the rate the device gives these jobs
+        # without sharing rounds, against which the stream's aggregate is
+        # the serving loop's efficiency
+        with k.Server(dev, budget_bytes=budget, max_paths=max_paths) as solo:
+            t0 = time.perf_counter()
+            for arrival, si, w, h, spp in stream:
+                sc, depth, rs = pool[si]
+                j = solo.submit(sc, w, h, spp, seed=rs.seed, surfaces=rs.surfaces, depth=depth, guided=guided)
+                solo.run([j])
+                solo.release(j)
+            alone_s = time.perf_counter() - t0
 
     def pct(q):
         return lat[min(len(lat) - 1, int(q * len(lat)))]
@@ -144,6 +160,8 @@ def serving_stream(k, scenes, dev, jobs, rate, seed, max_paths, budget, sizes=No
                         f"budget {budget / 2 ** 30:g} GiB" + ("; every job with the plain guide" if guided else ""),
             "p50_ms": pct(0.50), "p99_ms": pct(0.99), "mean_ms": sum(lat) / len(lat),
             "aggregate_msamples_s": total_samples / wall_s / 1e6, "samples": total_samples, "wall_s": wall_s,
+            "alone_msamples_s": None if alone_s is None else total_samples / alone_s / 1e6,
+            "efficiency": None if alone_s is None else alone_s / wall_s,
             "rounds": rounds, "mean_jobs_per_round": jobs_in_rounds / max(rounds, 1),
             "mean_paths_per_round": paths_in_rounds / max(rounds, 1), "max_resident_bytes": max_resident}
 
@@ -380,7 +398,7 @@ def main():
     if not args.no_serve:
         # the stream at each arrival rate (jobs/s), the same jobs each time
         serving = [serving_stream(k, scenes, dev, args.serve_jobs, rate, args.serve_seed, args.serve_paths,
-                                  args.serve_budget_gib << 30) for rate in args.serve_rates]
+                                  args.serve_budget_gib << 30, alone=rate == args.serve_rates[0]) for rate in args.serve_rates]
         if not args.no_guide:
             # the same stream served with the plain guide on every job
             serving_guided = serving_stream(k, scenes, dev, args.serve_jobs, args.serve_rates[0], args.serve_seed, args.serve_paths,

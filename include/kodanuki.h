@@ -184,7 +184,13 @@ typedef struct kdn_render_settings {
     float sky_zenith[3];
     float sky_horizon[3];
     float sky_ground[3];
+    int32_t batch;      /* render { batch }: samples a batch of a pixel's estimate holds (0: unset) */
+    float exposure;     /* grade { exposure } (1) */
+    int32_t curve;      /* grade { curve }: KDN_CURVE_FILMIC (the default) or KDN_CURVE_CLAMPED */
 } kdn_render_settings;
+
+#define KDN_CURVE_FILMIC 0
+#define KDN_CURVE_CLAMPED 1
 
 typedef struct kdn_scene_info {
     uint32_t steps;       /* steps over every object program */
@@ -328,7 +334,12 @@ typedef struct kdn_pass_desc {
                                bit 5: the plain guide: the first vertex
                                       of each path samples with its
                                       pixel's guide (kdn_highlight_guides
-                                      made for this image; not with bit 4) */
+                                      made for this image; not with bit 4);
+                               bit 6: keep every pixel's estimate as the
+                                      reference's frame does (batch means
+                                      and squares, kdn_read_estimates),
+                                      batches of `batch` samples */
+    uint32_t batch;         /* samples a batch holds (bit 6; 0: no batch is ever closed) */
 } kdn_pass_desc;
 
 typedef struct kdn_pass_stats {
@@ -361,6 +372,48 @@ int kdn_render_pass(kdn_ctx* ctx, const kdn_pass_desc* desc, kdn_pass_stats* sta
 int kdn_read_accumulation(kdn_ctx* ctx, double* rgb_sum, double* sq_sum, uint32_t* count);
 /* The mean image as linear RGB floats (width*height*3). */
 int kdn_read_image(kdn_ctx* ctx, float* rgb);
+
+/* A pixel's estimate as the reference's frame keeps it (Estimate,
+ * core/stopping.h:57-100): its samples' colours summed in sample order and
+ * counted, the colours of the batch being drawn, the count of whole
+ * batches with the sum of their means and of their means' squares, and the
+ * shared parts (a sample drawn alone shares its whole colour: its shared
+ * part is its channel mean; foldedSample and closedBatch,
+ * core/stopping.cpp:36-67). */
+typedef struct kdn_estimate {
+    double total[3];
+    int32_t taken;
+    int32_t batches;
+    double batch[3];
+    double means[3];
+    double squares[3];
+    double shared;
+    double shared_means;
+    double shared_squares;
+    double own_squares;
+} kdn_estimate;
+/* The estimates the passes with flags bit 6 kept (width*height, by rows
+ * from the top left); zeroed with the accumulators. */
+int kdn_read_estimates(kdn_ctx* ctx, kdn_estimate* estimates);
+/* Fold sample colours into estimates on the device, as a pass does:
+ * colours[(pixel * spp + sample) * 3 + channel], each pixel's samples in
+ * order, into estimates[pixel] (host buffers, read and written back), with
+ * batches of `batch` samples (the fold of the reference's foldedPixel,
+ * core/frame.cpp:221-235). */
+int kdn_fold_samples(kdn_ctx* ctx, const double* colours, uint32_t pixels, uint32_t spp, uint32_t batch, kdn_estimate* estimates);
+
+/* ---- images (host) ----------------------------------------------------------- */
+/* A linear RGB image (width*height*3 floats by rows from the top left)
+ * graded and written as a binary PPM (writePpm, shared/image.cpp:72-79):
+ * each value times the exposure, through the filmic curve (or clamped),
+ * to the power 1/2.2, rounded to a byte. A file that cannot be written is
+ * KDN_E_ARGUMENT with "path:1:1: cannot write" (kdn_global_error). */
+int kdn_image_write_ppm(const float* rgb, int32_t width, int32_t height, float exposure, int32_t curve, const char* path);
+/* The same bytes into memory (width*height*3), no file. */
+int kdn_image_encode(const float* rgb, int32_t width, int32_t height, float exposure, int32_t curve, uint8_t* bytes);
+/* The linear values as a little-endian PFM, bottom row first (writePfm,
+ * shared/image.cpp:88-96). */
+int kdn_image_write_pfm(const float* rgb, int32_t width, int32_t height, const char* path);
 
 /* ---- the denoiser ------------------------------------------------------------ */
 /*

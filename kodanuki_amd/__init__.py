@@ -73,7 +73,11 @@ class Camera(C.Structure):
 class RenderSettings(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("seed", C.c_int32), ("samples", C.c_int32),
                 ("surfaces", C.c_int32), ("far_", C.c_float), ("sky_zenith", C.c_float * 3),
-                ("sky_horizon", C.c_float * 3), ("sky_ground", C.c_float * 3)]
+                ("sky_horizon", C.c_float * 3), ("sky_ground", C.c_float * 3), ("batch", C.c_int32), ("exposure", C.c_float),
+                ("curve", C.c_int32)]
+
+
+CURVE_FILMIC, CURVE_CLAMPED = 0, 1
 
 
 class SceneInfo(C.Structure):
@@ -97,7 +101,15 @@ class EntryRecord(C.Structure):
 
 class PassDesc(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("first_sample", C.c_uint32), ("spp", C.c_uint32),
-                ("seed", C.c_uint32), ("surfaces", C.c_uint32), ("min_depth", C.c_uint32), ("flags", C.c_uint32)]
+                ("seed", C.c_uint32), ("surfaces", C.c_uint32), ("min_depth", C.c_uint32), ("flags", C.c_uint32),
+                ("batch", C.c_uint32)]
+
+
+# kdn_estimate: a pixel's estimate as the reference's frame keeps it
+ESTIMATE_DTYPE = np.dtype([("total", "<f8", 3), ("taken", "<i4"), ("batches", "<i4"), ("batch", "<f8", 3), ("means", "<f8", 3),
+                           ("squares", "<f8", 3), ("shared", "<f8"), ("shared_means", "<f8"), ("shared_squares", "<f8"),
+                           ("own_squares", "<f8")])
+assert ESTIMATE_DTYPE.itemsize == 136
 
 
 class PassStats(C.Structure):
@@ -110,7 +122,7 @@ class PassStats(C.Structure):
                 ("guide_steps", C.c_uint32)]
 
 
-PASS_KEEP, PASS_TIME, PASS_COUNT, PASS_NO_FINISH, PASS_GUIDE, PASS_HIGHLIGHT = 1, 2, 4, 8, 16, 32
+PASS_KEEP, PASS_TIME, PASS_COUNT, PASS_NO_FINISH, PASS_GUIDE, PASS_HIGHLIGHT, PASS_ESTIMATES = 1, 2, 4, 8, 16, 32, 64
 
 
 class GuideInfo(C.Structure):
@@ -206,7 +218,8 @@ EXPORTS = [
     "kdn_scene_leaves", "kdn_scene_nodes", "kdn_scene_materials", "kdn_scene_lights", "kdn_scene_settings",
     "kdn_scene_region", "kdn_open", "kdn_close", "kdn_last_error", "kdn_global_error", "kdn_upload_scene",
     "kdn_build_octree", "kdn_octree_sizes", "kdn_octree_download", "kdn_extend", "kdn_extend_device",
-    "kdn_render_pass", "kdn_read_accumulation", "kdn_read_image",
+    "kdn_render_pass", "kdn_read_accumulation", "kdn_read_image", "kdn_read_estimates", "kdn_fold_samples",
+    "kdn_image_write_ppm", "kdn_image_encode", "kdn_image_write_pfm",
     "kdn_denoise", "kdn_denoise_device", "kdn_denoise_device_static", "kdn_denoise_accumulation",
     "kdn_guide_reset", "kdn_guide_configure", "kdn_guide_describe", "kdn_guide_params", "kdn_guide_set_params", "kdn_guide_infer",
     "kdn_guide_gradients", "kdn_guide_train",
@@ -261,6 +274,11 @@ def _bind(lib):
         "kdn_render_pass": (C.c_int, [P, C.POINTER(PassDesc), C.POINTER(PassStats)]),
         "kdn_read_accumulation": (C.c_int, [P, P, P, P]),
         "kdn_read_image": (C.c_int, [P, P]),
+        "kdn_read_estimates": (C.c_int, [P, P]),
+        "kdn_fold_samples": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_uint32, P]),
+        "kdn_image_write_ppm": (C.c_int, [P, C.c_int32, C.c_int32, C.c_float, C.c_int32, C.c_char_p]),
+        "kdn_image_encode": (C.c_int, [P, C.c_int32, C.c_int32, C.c_float, C.c_int32, P]),
+        "kdn_image_write_pfm": (C.c_int, [P, C.c_int32, C.c_int32, C.c_char_p]),
         "kdn_denoise": (C.c_int, [P, C.POINTER(DenoiseFrame), P, P]),
         "kdn_denoise_device": (C.c_int, [P, C.POINTER(DenoiseFrame), P, P, P]),
         "kdn_denoise_device_static": (C.c_int, [P, C.POINTER(DenoiseFrame), P, P, P]),
@@ -460,11 +478,26 @@ class Device:
         _check(library().kdn_extend_device(self._h, C.c_void_p(rays_ptr), count, C.c_void_p(hits_ptr),
                                            C.c_void_p(stream) if stream else None), self._h)
 
-    def render_pass(self, width, height, spp, first_sample=0, seed=1, surfaces=7, min_depth=4, keep=False, flags=0):
-        d = PassDesc(width, height, first_sample, spp, seed, surfaces, min_depth, (PASS_KEEP if keep else 0) | flags)
+    def render_pass(self, width, height, spp, first_sample=0, seed=1, surfaces=7, min_depth=4, keep=False, flags=0, batch=0):
+        d = PassDesc(width, height, first_sample, spp, seed, surfaces, min_depth, (PASS_KEEP if keep else 0) | flags, batch)
         st = PassStats()
         _check(library().kdn_render_pass(self._h, C.byref(d), C.byref(st)), self._h)
         return st
+
+    def estimates(self, width, height):
+        """The per-pixel estimates passes with PASS_ESTIMATES kept: [height, width] of ESTIMATE_DTYPE."""
+        out = np.zeros((height, width), ESTIMATE_DTYPE)
+        _check(library().kdn_read_estimates(self._h, _ptr(out)), self._h)
+        return out
+
+    def fold_samples(self, colours, batch, estimates=None):
+        """colours [pixels, spp, 3] (float64) folded in sample order into
+        estimates [pixels] (ESTIMATE_DTYPE; fresh when None) on the device."""
+        col = np.ascontiguousarray(colours, np.float64)
+        pixels, spp, _ = col.shape
+        est = np.zeros(pixels, ESTIMATE_DTYPE) if estimates is None else np.ascontiguousarray(estimates, ESTIMATE_DTYPE).copy()
+        _check(library().kdn_fold_samples(self._h, _ptr(col), pixels, spp, batch, _ptr(est)), self._h)
+        return est
 
     def accumulation(self, width, height):
         rgb = np.zeros((height, width, 3), np.float64)
@@ -594,6 +627,27 @@ class Device:
         rgb = np.zeros((height, width, 3), np.float32)
         _check(library().kdn_read_image(self._h, _ptr(rgb)), self._h)
         return rgb
+
+
+def encode_image(rgb, exposure=1.0, curve=CURVE_FILMIC):
+    """A linear RGB image [height, width, 3] graded to bytes as the PPM holds them."""
+    img = np.ascontiguousarray(rgb, np.float32)
+    h, w, _ = img.shape
+    out = np.zeros((h, w, 3), np.uint8)
+    _check(library().kdn_image_encode(_ptr(img), w, h, exposure, curve, _ptr(out)))
+    return out
+
+
+def write_ppm(rgb, path, exposure=1.0, curve=CURVE_FILMIC):
+    img = np.ascontiguousarray(rgb, np.float32)
+    h, w, _ = img.shape
+    _check(library().kdn_image_write_ppm(_ptr(img), w, h, exposure, curve, os.fsencode(path)))
+
+
+def write_pfm(rgb, path):
+    img = np.ascontiguousarray(rgb, np.float32)
+    h, w, _ = img.shape
+    _check(library().kdn_image_write_pfm(_ptr(img), w, h, os.fsencode(path)))
 
 
 class Server:

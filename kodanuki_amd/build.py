@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libkodanuki_hip.so")
 
-SOURCES = ["abi.cpp", "scene.cpp", "tables.cpp", "scan.hip", "octree.hip", "trace.hip", "integrator.hip", "serve.hip", "guide.hip", "guide_net.hip",
+SOURCES = ["abi.cpp", "scene.cpp", "tables.cpp", "image.cpp", "scan.hip", "octree.hip", "trace.hip", "integrator.hip", "serve.hip", "guide.hip", "guide_net.hip",
            "denoise.hip", "hguide.hip", "sort.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

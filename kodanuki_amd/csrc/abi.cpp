@@ -168,6 +168,8 @@ void searchRegion(const kdn::Scene& scene, const DI box[3], const std::vector<ui
 
 namespace kdn {
 
+int setGlobalError(const std::string& m, int code) { return fail(m, code); }
+
 bool ensure(DevBuf& b, size_t bytes, std::string& err, bool keep) {
     if (b.bytes >= bytes) return true;
     size_t n = b.bytes ? b.bytes : (1u << 16);
@@ -619,6 +621,33 @@ int kdn_read_accumulation(kdn_ctx* ctx, double* rgb_sum, double* sq_sum, uint32_
         c.error = "kdn_read_accumulation: copy failed";
         return ctxFail(ctx, KDN_E_DEVICE);
     }
+    return KDN_OK;
+}
+
+int kdn_read_estimates(kdn_ctx* ctx, kdn_estimate* estimates) {
+    if (!ctx || !estimates) return fail("kdn_read_estimates: null argument", KDN_E_ARGUMENT);
+    kdn::Ctx& c = ctx->c;
+    if (!c.estimates) {
+        c.error = "kdn_read_estimates: no pass kept estimates (kdn_pass_desc.flags bit 6)";
+        return ctxFail(ctx, KDN_E_STATE);
+    }
+    hipSetDevice(c.device);
+    const size_t px = (size_t)c.width * c.height;
+    if (hipStreamSynchronize(c.stream) != hipSuccess ||
+        hipMemcpy(estimates, c.estimates, px * sizeof(kdn_estimate), hipMemcpyDeviceToHost) != hipSuccess) {
+        c.error = "kdn_read_estimates: copy failed";
+        return ctxFail(ctx, KDN_E_DEVICE);
+    }
+    return KDN_OK;
+}
+
+int kdn_fold_samples(kdn_ctx* ctx, const double* colours, uint32_t pixels, uint32_t spp, uint32_t batch, kdn_estimate* estimates) {
+    if (!ctx || (pixels && spp && (!colours || !estimates))) return fail("kdn_fold_samples: null argument", KDN_E_ARGUMENT);
+    if ((uint64_t)pixels * spp > (1ull << 28)) return fail("kdn_fold_samples: more than 2^28 samples", KDN_E_ARGUMENT);
+    if (pixels == 0 || spp == 0) return KDN_OK;
+    kdn::Ctx& c = ctx->c;
+    hipSetDevice(c.device);
+    if (!kdn::foldSamples(c, colours, pixels, spp, batch, estimates)) return ctxFail(ctx, KDN_E_DEVICE);
     return KDN_OK;
 }
 

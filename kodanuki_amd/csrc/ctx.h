@@ -61,6 +61,7 @@ struct Ctx {
     double* accum = nullptr;        // rgb sums, FP64, per pixel
     double* accum_sq = nullptr;     // luminance squares, FP64, per pixel
     uint32_t* accum_n = nullptr;    // samples per pixel
+    kdn_estimate* estimates = nullptr;  // per pixel, the reference's Estimate (passes with flags bit 6)
     PathState* paths = nullptr;     // host record of the path SoA in HBM
     size_t paths_bytes = 0;         // its one allocation
     size_t max_paths = 0;           // paths in flight per chunk (0: 2^23)
@@ -101,6 +102,7 @@ bool extendRays(Ctx& c, const kdn_ray* rays, uint32_t n, kdn_hit* hits, hipStrea
 
 // integrator.hip
 bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats);
+bool foldSamples(Ctx& c, const double* colours, uint32_t pixels, uint32_t spp, uint32_t batch, kdn_estimate* estimates);
 void freeIntegrator(Ctx& c);
 
 // guide.hip: the learned guide's parameters over a region, from a seed

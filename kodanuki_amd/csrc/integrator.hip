@@ -94,6 +94,28 @@ __global__ void k_sort_keys(PassParams P, const uint32_t* queue, uint32_t count,
     P.s.keys[q] = ray_key(job_of(P, i).tv, shadow ? P.s.shadow[i] : P.s.rays[i]);
 }
 
+// estimates[pixel] folded with the pixel's spp colours, in sample order
+__global__ void k_fold_samples(const double* colours, uint32_t pixels, uint32_t spp, uint32_t batch, kdn_estimate* estimates) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pixels) return;
+    kdn_estimate e = estimates[p];
+    for (uint32_t j = 0; j < spp; ++j) fold_estimate(e, colours + ((size_t)p * spp + j) * 3, batch);
+    estimates[p] = e;
+}
+
+bool foldSamples(Ctx& c, const double* colours, uint32_t pixels, uint32_t spp, uint32_t batch, kdn_estimate* estimates) {
+    const size_t cb = (size_t)pixels * spp * 3 * sizeof(double), eb = (size_t)pixels * sizeof(kdn_estimate);
+    if (!ensure(c.scratch[6], cb + 256, c.error, false) || !ensure(c.scratch[7], eb + 256, c.error, false)) return false;
+    KDN_HIP(hipMemcpy(c.scratch[6].ptr, colours, cb, hipMemcpyHostToDevice));
+    KDN_HIP(hipMemcpy(c.scratch[7].ptr, estimates, eb, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_fold_samples, dim3((pixels + 255) / 256), dim3(256), 0, c.stream, (const double*)c.scratch[6].ptr, pixels, spp,
+                       batch, (kdn_estimate*)c.scratch[7].ptr);
+    KDN_HIP(hipGetLastError());
+    KDN_HIP(hipStreamSynchronize(c.stream));
+    KDN_HIP(hipMemcpy(estimates, c.scratch[7].ptr, eb, hipMemcpyDeviceToHost));
+    return true;
+}
+
 __global__ void k_fold(PassParams P, uint32_t slice) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     fold_one(P, slice, t);
@@ -102,6 +124,8 @@ __global__ void k_fold(PassParams P, uint32_t slice) {
 #undef KDN_LANE_REGS
 
 void freeIntegrator(Ctx& c) {
+    if (c.estimates) (void)hipFree(c.estimates);
+    c.estimates = nullptr;
     if (c.paths) {
         (void)hipFree(c.paths->slice);
         delete c.paths;
@@ -462,22 +486,32 @@ bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
     hipStream_t st = c.stream;
     // accumulators
     const size_t px = (size_t)d.width * d.height;
-    if (!c.accum || c.width != d.width || c.height != d.height) {
+    const bool fresh = !c.accum || c.width != d.width || c.height != d.height;
+    if (fresh) {
         if (c.accum) hipFree(c.accum);
         if (c.accum_sq) hipFree(c.accum_sq);
         if (c.accum_n) hipFree(c.accum_n);
+        if (c.estimates) hipFree(c.estimates);
+        c.estimates = nullptr;
         KDN_HIP(hipMalloc(&c.accum, px * 3 * sizeof(double)));
         KDN_HIP(hipMalloc(&c.accum_sq, px * 3 * sizeof(double)));
         KDN_HIP(hipMalloc(&c.accum_n, px * sizeof(uint32_t)));
         c.width = d.width;
         c.height = d.height;
+    }
+    // the estimates are kept from the first pass that asks for them, zeroed
+    // with the accumulators
+    const bool newEstimates = (d.flags & 64) && !c.estimates;
+    if (newEstimates) KDN_HIP(hipMalloc(&c.estimates, px * sizeof(kdn_estimate)));
+    if (fresh || !(d.flags & 1)) {
         KDN_HIP(hipMemsetAsync(c.accum, 0, px * 3 * sizeof(double), st));
         KDN_HIP(hipMemsetAsync(c.accum_sq, 0, px * 3 * sizeof(double), st));
         KDN_HIP(hipMemsetAsync(c.accum_n, 0, px * sizeof(uint32_t), st));
-    } else if (!(d.flags & 1)) {
-        KDN_HIP(hipMemsetAsync(c.accum, 0, px * 3 * sizeof(double), st));
-        KDN_HIP(hipMemsetAsync(c.accum_sq, 0, px * 3 * sizeof(double), st));
-        KDN_HIP(hipMemsetAsync(c.accum_n, 0, px * sizeof(uint32_t), st));
+        if (c.estimates) KDN_HIP(hipMemsetAsync(c.estimates, 0, px * sizeof(kdn_estimate), st));
+    } else if (newEstimates) {
+        c.error = "kdn_render_pass: a pass that keeps the accumulators (bit 0) cannot start the estimates (bit 6): "
+                  "the passes before it kept none";
+        return false;
     }
     std::vector<JobDev> jobs{jobOf(c, d.width, d.height, d.seed, d.surfaces, d.min_depth)};
     if (d.flags & 32) {
@@ -495,7 +529,8 @@ bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
     kdn_pass_stats local{};
     for (uint32_t first = 0; first < d.spp; first += chunk) {
         const uint32_t n = std::min(chunk, d.spp - first);
-        const std::vector<SliceDev> slices{SliceDev{0, d.first_sample + first, n, 0, c.accum, c.accum_sq, c.accum_n}};
+        const std::vector<SliceDev> slices{SliceDev{0, d.first_sample + first, n, 0, c.accum, c.accum_sq, c.accum_n,
+                                                    (d.flags & 64) ? c.estimates : nullptr, d.batch}};
         if (!runRound(c, jobs, slices, d.flags, &local)) return false;
     }
     local.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();

@@ -66,6 +66,8 @@ struct SliceDev {
     double* accum;
     double* accum_sq;
     uint32_t* accum_n;
+    kdn_estimate* estimate;  // the job's per-pixel estimates (null: none kept)
+    uint32_t batch;          // samples a batch of an estimate holds
 };
 
 // A round: the slices of one or more jobs whose paths share the queues.
@@ -769,8 +771,39 @@ KDN_D void finish_lane(PassParams& P, uint32_t inQueue, uint32_t count, const La
     atomicAdd(&tally[1], shadows);
 }
 
+// A sample drawn alone folded into a pixel's estimate (foldedSample and
+// closedBatch, core/stopping.cpp:36-67): its colour added to the total and
+// the batch, its shared part (its whole colour's channel mean) to the
+// batch's; a whole batch closes into the sums of means and squares.
+KDN_D void fold_estimate(kdn_estimate& e, const double c[3], uint32_t batch) {
+    for (int k = 0; k < 3; ++k) {
+        e.total[k] = e.total[k] + c[k];
+        e.batch[k] = e.batch[k] + c[k];
+    }
+    e.taken += 1;
+    e.shared = e.shared + (c[0] + c[1] + c[2]) / 3;
+    if (batch > 0 && (uint32_t)e.taken % batch == 0) {
+        const double scale = 1.0 / (double)batch;
+        double mean[3];
+        for (int k = 0; k < 3; ++k) mean[k] = e.batch[k] * scale;
+        const double shared = e.shared / (double)batch;
+        const double own = (mean[0] + mean[1] + mean[2]) / 3 - shared;
+        for (int k = 0; k < 3; ++k) {
+            e.means[k] = e.means[k] + mean[k];
+            e.squares[k] = e.squares[k] + mean[k] * mean[k];
+            e.batch[k] = 0;
+        }
+        e.batches += 1;
+        e.shared_means = e.shared_means + shared;
+        e.shared_squares = e.shared_squares + shared * shared;
+        e.own_squares = e.own_squares + own * own;
+        e.shared = 0;
+    }
+}
+
 // fold: pixel slot t of slice k's first sample plane; its samples folded
-// into the job's FP64 accumulators in sample order
+// into the job's FP64 accumulators in sample order (and into its estimate
+// where the slice keeps them)
 KDN_D void fold_one(const PassParams& P, uint32_t k, uint32_t t) {
     const SliceDev& sl = P.slices[k];
     const JobDev& J = P.jobs[sl.job];
@@ -780,6 +813,8 @@ KDN_D void fold_one(const PassParams& P, uint32_t k, uint32_t t) {
     const uint32_t pixel = s.pixel[sl.first_path + t];
     if (pixel == 0xFFFFFFFFu) return;
     double sum[3] = {0, 0, 0}, sq[3] = {0, 0, 0};
+    kdn_estimate est;
+    if (sl.estimate) est = sl.estimate[pixel];
     for (uint32_t j = 0; j < sl.spp; ++j) {
         const size_t i = (size_t)sl.first_path + (size_t)j * J.plane + t;
         Lanes w;
@@ -795,7 +830,9 @@ KDN_D void fold_one(const PassParams& P, uint32_t k, uint32_t t) {
             sum[c] += rgb[c];
             sq[c] += rgb[c] * rgb[c];
         }
+        if (sl.estimate) fold_estimate(est, rgb, sl.batch);
     }
+    if (sl.estimate) sl.estimate[pixel] = est;
     for (int c = 0; c < 3; ++c) {
         sl.accum[3 * (size_t)pixel + c] += sum[c];
         sl.accum_sq[3 * (size_t)pixel + c] += sq[c];

@@ -197,7 +197,8 @@ public:
         scopes_.emplace_back();
         set("pi", num(kPi));
         set("time", num(0));
-        for (const char* axis : {"x", "y", "z"}) {
+        // bare words the settings take: axes, and a grade's curves
+        for (const char* axis : {"x", "y", "z", "filmic", "clamped"}) {
             Value w;
             w.kind = Value::Word;
             w.word = axis;
@@ -648,7 +649,8 @@ private:
                 else if (n == "samples") scene_.render.samples = int(x);
                 else if (n == "surfaces") scene_.render.surfaces = std::max(1, int(x));
                 else if (n == "far") scene_.render.far_ = float(x);
-                else if (n == "tolerance" || n == "batch" || n == "split" || n == "collisions") {}
+                else if (n == "batch") scene_.render.batch = int(x);
+                else if (n == "tolerance" || n == "split" || n == "collisions") {}
                 else fail(at, "render has no field " + n);
             });
         } else if (kind.text == "camera") {
@@ -663,6 +665,19 @@ private:
                     if (scalar(v) != 0) fail(at, "this loader takes a pinhole camera only");
                 } else {
                     fail(at, "camera has no field " + n);
+                }
+            });
+        } else if (kind.text == "grade") {
+            // readGrade (syntax/evaluator.cpp:1971-1985)
+            fields([&](const std::string& n, const Value& v, const Token& at) {
+                if (n == "exposure") {
+                    scene_.render.exposure = float(scalar(v));
+                } else if (n == "curve") {
+                    const std::string w = v.kind == Value::Word ? v.word : std::string();
+                    if (w != "filmic" && w != "clamped") fail(at, "a curve is filmic or clamped");
+                    scene_.render.curve = w == "filmic" ? KDN_CURVE_FILMIC : KDN_CURVE_CLAMPED;
+                } else {
+                    fail(at, "grade has no field " + n);
                 }
             });
         } else {
@@ -1116,6 +1131,8 @@ bool loadScene(const std::string& text, const std::string& baseDir, Scene& scene
     scene.render.samples = 16;
     scene.render.surfaces = 10;
     scene.render.far_ = 1000.0f;
+    scene.render.exposure = 1.0f;  // defaultGrade (syntax/evaluator.cpp:2735-2740)
+    scene.render.curve = KDN_CURVE_FILMIC;
     try {
         std::vector<Token> tokens = lex(text);
         Loader loader(tokens, scene);

@@ -351,7 +351,7 @@ int kdn_serve_step(kdn_server* s, kdn_serve_stats* stats) {
                     jobs.back().hg_deep = j.hgDeep ? (const kdn::HgDeepHeader*)((const char*)j.hg.ptr + j.hgDeep) : nullptr;
                     owners.push_back(j.id);
                 }
-                slices.push_back(kdn::SliceDev{(uint32_t)jobIndex[r], nextOf[r], spp, (uint32_t)paths, j.accum, j.accumSq, j.accumN});
+                slices.push_back(kdn::SliceDev{(uint32_t)jobIndex[r], nextOf[r], spp, (uint32_t)paths, j.accum, j.accumSq, j.accumN, nullptr, 0});
                 paths += (uint64_t)spp * plane;
                 nextOf[r] += spp;
                 added = true;
