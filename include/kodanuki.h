@@ -187,6 +187,8 @@ typedef struct kdn_render_settings {
     int32_t batch;      /* render { batch }: samples a batch of a pixel's estimate holds (0: unset) */
     float exposure;     /* grade { exposure } (1) */
     int32_t curve;      /* grade { curve }: KDN_CURVE_FILMIC (the default) or KDN_CURVE_CLAMPED */
+    float tolerance;    /* render { tolerance }: the noise a pixel may keep, in steps of a byte (0: unset) */
+    int32_t split;      /* render { split }: samples that share a first vertex (0: unset) */
 } kdn_render_settings;
 
 #define KDN_CURVE_FILMIC 0
@@ -338,8 +340,19 @@ typedef struct kdn_pass_desc {
                                bit 6: keep every pixel's estimate as the
                                       reference's frame does (batch means
                                       and squares, kdn_read_estimates),
-                                      batches of `batch` samples */
+                                      batches of `batch` samples;
+                               bit 7: the Sobol member: an eye path's first
+                                      28 coordinates off the Owen-scrambled
+                                      Sobol sequence (sampler/sobol), the
+                                      rest hashed as without it;
+                               bit 8: the count pass of adaptive stopping:
+                                      each pixel draws its allotment less its
+                                      trial (kdn_stopping_allot), samples
+                                      first_sample.. (spp unused); with bits
+                                      0 and 6 the estimates restart from the
+                                      trial's totals (sumStart) */
     uint32_t batch;         /* samples a batch holds (bit 6; 0: no batch is ever closed) */
+    uint32_t round_paths;   /* the most paths of one round (0: 2^23; at least 4096) */
 } kdn_pass_desc;
 
 typedef struct kdn_pass_stats {
@@ -367,6 +380,47 @@ typedef struct kdn_pass_stats {
 } kdn_pass_stats;
 
 int kdn_render_pass(kdn_ctx* ctx, const kdn_pass_desc* desc, kdn_pass_stats* stats);
+
+/* Adaptive stopping: the reference's stopping/confidence_interval member
+ * (confidence_interval.h:36-106, its Hooks trialSamples .. allottedSplit,
+ * called by core/frame.cpp:28-62). A frame draws a trial of two batches in
+ * every pixel (a pass with flags bit 6), judges each pixel's trial against
+ * the tolerance, allots each pixel a count from its eight neighbours'
+ * judgements (never its own), and draws the rest (a pass with flags
+ * bits 0 and 8). Every sample here draws its own first vertex: split 1. */
+typedef struct kdn_stopping {
+    int32_t samples;    /* the most samples a pixel draws in a frame */
+    int32_t batch;      /* samples a batch holds */
+    double tolerance;   /* half the interval's width, in steps of a byte after the grade */
+    float exposure;     /* the grade the noise is judged through */
+    int32_t curve;      /* KDN_CURVE_FILMIC or KDN_CURVE_CLAMPED */
+    int32_t split;      /* samples that share a first vertex: 1 */
+} kdn_stopping;
+/* effortFor (core/stopping.cpp:23-34): the ceiling and batch a tolerance
+ * asks for where the scene names none. */
+int kdn_stopping_effort(double tolerance, int32_t* samples, int32_t* batch);
+/* confidenceTrial (confidence_interval.cpp:26-29): samples every pixel's
+ * trial draws, 0 for none (then the frame is one pass of `samples`). */
+int kdn_stopping_trial(const kdn_stopping* stopping, int32_t* samples);
+/* Judge every pixel's trial (the estimates a pass with flags bit 6 kept:
+ * confidenceWanted and splitSpread) and allot each its count
+ * (confidenceAllotted, splitFor) on the device; the counts stay on the
+ * device for the next pass with flags bit 8, and *total is that pass's
+ * paths (the samples every pixel draws beyond its trial). */
+int kdn_stopping_allot(kdn_ctx* ctx, const kdn_stopping* stopping, int32_t trial, uint64_t* total);
+/* Read the last allotment: per pixel (by rows) the samples its own trial
+ * wants (infinite where no count would do), the spread of its shared part
+ * and of the rest (2 floats), and its allotted count; any may be null. */
+int kdn_read_stopping(kdn_ctx* ctx, float* wanted, float* spreads, int32_t* allotted);
+
+/* An eye path's numbers as the integrator draws them (numberAt,
+ * plugins/transport/paths/sampler.cpp:15-30): for each sample index, the
+ * number at `coordinate` of the stream, off the Sobol sequence
+ * (sobolNumber, plugins/sampler/sobol/sobol.h:36) where `sobol` is set and
+ * the coordinate is below 28, hashed (math/random.cpp hashedNumber)
+ * otherwise; computed on the device. At most 2^24 indices. */
+int kdn_sequence_numbers(kdn_ctx* ctx, uint64_t stream, const uint32_t* indices, uint32_t count, uint32_t coordinate, int32_t sobol,
+                         double* out);
 /* Read the FP64 accumulators: per pixel the sum of linear RGB (3 doubles),
  * the sum of its squares per channel (3 doubles) and the sample count. */
 int kdn_read_accumulation(kdn_ctx* ctx, double* rgb_sum, double* sq_sum, uint32_t* count);

@@ -74,7 +74,7 @@ class RenderSettings(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("seed", C.c_int32), ("samples", C.c_int32),
                 ("surfaces", C.c_int32), ("far_", C.c_float), ("sky_zenith", C.c_float * 3),
                 ("sky_horizon", C.c_float * 3), ("sky_ground", C.c_float * 3), ("batch", C.c_int32), ("exposure", C.c_float),
-                ("curve", C.c_int32)]
+                ("curve", C.c_int32), ("tolerance", C.c_float), ("split", C.c_int32)]
 
 
 CURVE_FILMIC, CURVE_CLAMPED = 0, 1
@@ -102,7 +102,7 @@ class EntryRecord(C.Structure):
 class PassDesc(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("first_sample", C.c_uint32), ("spp", C.c_uint32),
                 ("seed", C.c_uint32), ("surfaces", C.c_uint32), ("min_depth", C.c_uint32), ("flags", C.c_uint32),
-                ("batch", C.c_uint32)]
+                ("batch", C.c_uint32), ("round_paths", C.c_uint32)]
 
 
 # kdn_estimate: a pixel's estimate as the reference's frame keeps it
@@ -123,6 +123,42 @@ class PassStats(C.Structure):
 
 
 PASS_KEEP, PASS_TIME, PASS_COUNT, PASS_NO_FINISH, PASS_GUIDE, PASS_HIGHLIGHT, PASS_ESTIMATES = 1, 2, 4, 8, 16, 32, 64
+PASS_SOBOL, PASS_COUNTED = 128, 256
+
+
+class Stopping(C.Structure):
+    """kdn_stopping: the confidence-interval member's settings (the
+    reference's Stopping, core/stopping.h:102-132), split 1."""
+    _fields_ = [("samples", C.c_int32), ("batch", C.c_int32), ("tolerance", C.c_double), ("exposure", C.c_float),
+                ("curve", C.c_int32), ("split", C.c_int32)]
+
+
+def stopping_effort(tolerance):
+    """effortFor: (samples, batch) a tolerance asks for where the scene names none."""
+    samples, batch = C.c_int32(), C.c_int32()
+    _check(library().kdn_stopping_effort(C.c_double(tolerance), C.byref(samples), C.byref(batch)))
+    return samples.value, batch.value
+
+
+def stopping_of(settings, split=1):
+    """The Stopping a scene's render settings ask for (frameStopping,
+    core/frame.cpp:304-317): effortFor's ceiling and batch where the scene
+    names none; every sample here draws its own first vertex (split 1)."""
+    samples, batch = stopping_effort(settings.tolerance)
+    if settings.batch > 0:
+        batch = settings.batch
+    if settings.samples > 0:
+        samples = settings.samples
+    if split > 0 and batch % split:
+        raise KodanukiError(f"a split of {split} needs batches of a multiple of it, not {batch}")
+    return Stopping(samples, batch, settings.tolerance, settings.exposure, settings.curve, split)
+
+
+def stopping_trial(stopping):
+    """confidenceTrial: samples each pixel's trial draws (0: none)."""
+    n = C.c_int32()
+    _check(library().kdn_stopping_trial(C.byref(stopping), C.byref(n)))
+    return n.value
 
 
 class GuideInfo(C.Structure):
@@ -219,6 +255,7 @@ EXPORTS = [
     "kdn_scene_region", "kdn_open", "kdn_close", "kdn_last_error", "kdn_global_error", "kdn_upload_scene",
     "kdn_build_octree", "kdn_octree_sizes", "kdn_octree_download", "kdn_extend", "kdn_extend_device",
     "kdn_render_pass", "kdn_read_accumulation", "kdn_read_image", "kdn_read_estimates", "kdn_fold_samples",
+    "kdn_stopping_effort", "kdn_stopping_trial", "kdn_stopping_allot", "kdn_read_stopping", "kdn_sequence_numbers",
     "kdn_image_write_ppm", "kdn_image_encode", "kdn_image_write_pfm",
     "kdn_denoise", "kdn_denoise_device", "kdn_denoise_device_static", "kdn_denoise_accumulation",
     "kdn_guide_reset", "kdn_guide_configure", "kdn_guide_describe", "kdn_guide_params", "kdn_guide_set_params", "kdn_guide_infer",
@@ -276,6 +313,11 @@ def _bind(lib):
         "kdn_read_image": (C.c_int, [P, P]),
         "kdn_read_estimates": (C.c_int, [P, P]),
         "kdn_fold_samples": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_uint32, P]),
+        "kdn_stopping_effort": (C.c_int, [C.c_double, P, P]),
+        "kdn_stopping_trial": (C.c_int, [P, P]),
+        "kdn_stopping_allot": (C.c_int, [P, P, C.c_int32, P]),
+        "kdn_read_stopping": (C.c_int, [P, P, P, P]),
+        "kdn_sequence_numbers": (C.c_int, [P, C.c_uint64, P, C.c_uint32, C.c_uint32, C.c_int32, P]),
         "kdn_image_write_ppm": (C.c_int, [P, C.c_int32, C.c_int32, C.c_float, C.c_int32, C.c_char_p]),
         "kdn_image_encode": (C.c_int, [P, C.c_int32, C.c_int32, C.c_float, C.c_int32, P]),
         "kdn_image_write_pfm": (C.c_int, [P, C.c_int32, C.c_int32, C.c_char_p]),
@@ -478,8 +520,10 @@ class Device:
         _check(library().kdn_extend_device(self._h, C.c_void_p(rays_ptr), count, C.c_void_p(hits_ptr),
                                            C.c_void_p(stream) if stream else None), self._h)
 
-    def render_pass(self, width, height, spp, first_sample=0, seed=1, surfaces=7, min_depth=4, keep=False, flags=0, batch=0):
-        d = PassDesc(width, height, first_sample, spp, seed, surfaces, min_depth, (PASS_KEEP if keep else 0) | flags, batch)
+    def render_pass(self, width, height, spp, first_sample=0, seed=1, surfaces=7, min_depth=4, keep=False, flags=0, batch=0,
+                    round_paths=0):
+        d = PassDesc(width, height, first_sample, spp, seed, surfaces, min_depth, (PASS_KEEP if keep else 0) | flags, batch,
+                     round_paths)
         st = PassStats()
         _check(library().kdn_render_pass(self._h, C.byref(d), C.byref(st)), self._h)
         return st
@@ -498,6 +542,46 @@ class Device:
         est = np.zeros(pixels, ESTIMATE_DTYPE) if estimates is None else np.ascontiguousarray(estimates, ESTIMATE_DTYPE).copy()
         _check(library().kdn_fold_samples(self._h, _ptr(col), pixels, spp, batch, _ptr(est)), self._h)
         return est
+
+    # ---- adaptive stopping and the sequence ----
+    def stopping_allot(self, stopping, trial):
+        """Judge the trial estimates and allot every pixel's count on the
+        device; returns the count pass's paths."""
+        total = C.c_uint64()
+        _check(library().kdn_stopping_allot(self._h, C.byref(stopping), trial, C.byref(total)), self._h)
+        return total.value
+
+    def stopping_state(self, width, height):
+        """The last allotment: (wanted [h, w] f32, spreads [h, w, 2] f32, allotted [h, w] i32)."""
+        wanted = np.zeros((height, width), np.float32)
+        spreads = np.zeros((height, width, 2), np.float32)
+        allotted = np.zeros((height, width), np.int32)
+        _check(library().kdn_read_stopping(self._h, _ptr(wanted), _ptr(spreads), _ptr(allotted)), self._h)
+        return wanted, spreads, allotted
+
+    def render_stopped(self, width, height, stopping, seed=1, surfaces=7, min_depth=4, flags=0, round_paths=0):
+        """A still frame as the reference's frame draws it with the
+        confidence-interval member (core/frame.cpp:28-62, 435): a trial of two
+        batches in every pixel, then each pixel's allotment less its trial;
+        one pass of stopping.samples where the member draws no trial.
+        Returns (trial, paths of the count pass, [pass stats])."""
+        trial = stopping_trial(stopping)
+        if trial <= 0:
+            return 0, 0, [self.render_pass(width, height, stopping.samples, 0, seed, surfaces, min_depth, flags=flags | PASS_ESTIMATES,
+                                           batch=stopping.batch)]
+        first = self.render_pass(width, height, trial, 0, seed, surfaces, min_depth, flags=flags | PASS_ESTIMATES, batch=stopping.batch)
+        total = self.stopping_allot(stopping, trial)
+        rest = self.render_pass(width, height, 0, trial, seed, surfaces, min_depth, keep=True,
+                                flags=flags | PASS_ESTIMATES | PASS_COUNTED, batch=stopping.batch, round_paths=round_paths)
+        return trial, total, [first, rest]
+
+    def sequence_numbers(self, stream, indices, coordinate, sobol=True):
+        """numberAt of an eye path on the device: float64 per sample index."""
+        idx = np.ascontiguousarray(indices, np.uint32)
+        out = np.zeros(len(idx), np.float64)
+        _check(library().kdn_sequence_numbers(self._h, C.c_uint64(stream), _ptr(idx), len(idx), coordinate, 1 if sobol else 0,
+                                              _ptr(out)), self._h)
+        return out
 
     def accumulation(self, width, height):
         rgb = np.zeros((height, width, 3), np.float64)

@@ -237,6 +237,9 @@ void releaseAll(Ctx& c) {
     freeGuide(c);
     freeBuf(c.hguides);
     freeBuf(c.sort_tmp);
+    for (DevBuf* b : {&c.stop_wanted, &c.stop_spreads, &c.stop_allotted, &c.stop_counts, &c.stop_offsets, &c.stop_tmp}) freeBuf(*b);
+    c.stop_width = c.stop_height = 0;
+    c.stop_total = 0;
     c.hguide_width = c.hguide_height = 0;
     c.hguide_deep = 0;
     if (c.materials_dev) hipFree(c.materials_dev);
@@ -648,6 +651,52 @@ int kdn_fold_samples(kdn_ctx* ctx, const double* colours, uint32_t pixels, uint3
     kdn::Ctx& c = ctx->c;
     hipSetDevice(c.device);
     if (!kdn::foldSamples(c, colours, pixels, spp, batch, estimates)) return ctxFail(ctx, KDN_E_DEVICE);
+    return KDN_OK;
+}
+
+int kdn_stopping_effort(double tolerance, int32_t* samples, int32_t* batch) {
+    if (!samples || !batch) return fail("kdn_stopping_effort: null argument", KDN_E_ARGUMENT);
+    kdn::stoppingEffort(tolerance, samples, batch);
+    return KDN_OK;
+}
+
+int kdn_stopping_trial(const kdn_stopping* stopping, int32_t* samples) {
+    if (!stopping || !samples) return fail("kdn_stopping_trial: null argument", KDN_E_ARGUMENT);
+    std::string err;
+    if (!kdn::stoppingCheck(*stopping, err)) return fail("kdn_stopping_trial: " + err, KDN_E_ARGUMENT);
+    *samples = kdn::stoppingTrial(*stopping);
+    return KDN_OK;
+}
+
+int kdn_stopping_allot(kdn_ctx* ctx, const kdn_stopping* stopping, int32_t trial, uint64_t* total) {
+    if (!ctx || !stopping) return fail("kdn_stopping_allot: null argument", KDN_E_ARGUMENT);
+    kdn::Ctx& c = ctx->c;
+    std::string err;
+    if (!kdn::stoppingCheck(*stopping, err)) {
+        c.error = "kdn_stopping_allot: " + err;
+        return ctxFail(ctx, KDN_E_ARGUMENT);
+    }
+    hipSetDevice(c.device);
+    if (!kdn::stoppingAllot(c, *stopping, trial, total)) return ctxFail(ctx, c.estimates ? KDN_E_ARGUMENT : KDN_E_STATE);
+    return KDN_OK;
+}
+
+int kdn_read_stopping(kdn_ctx* ctx, float* wanted, float* spreads, int32_t* allotted) {
+    if (!ctx) return fail("kdn_read_stopping: null argument", KDN_E_ARGUMENT);
+    kdn::Ctx& c = ctx->c;
+    hipSetDevice(c.device);
+    if (!kdn::readStopping(c, wanted, spreads, allotted)) return ctxFail(ctx, KDN_E_STATE);
+    return KDN_OK;
+}
+
+int kdn_sequence_numbers(kdn_ctx* ctx, uint64_t stream, const uint32_t* indices, uint32_t count, uint32_t coordinate, int32_t sobol,
+                         double* out) {
+    if (!ctx || (count && (!indices || !out))) return fail("kdn_sequence_numbers: null argument", KDN_E_ARGUMENT);
+    if (count > (1u << 24)) return fail("kdn_sequence_numbers: more than 2^24 numbers", KDN_E_ARGUMENT);
+    if (count == 0) return KDN_OK;
+    kdn::Ctx& c = ctx->c;
+    hipSetDevice(c.device);
+    if (!kdn::sequenceNumbers(c, stream, indices, count, coordinate, sobol != 0, out)) return ctxFail(ctx, KDN_E_DEVICE);
     return KDN_OK;
 }
 

@@ -81,6 +81,12 @@ struct Ctx {
     int hguide_width = 0, hguide_height = 0;  // the image they were made for (0: none)
     size_t hguide_deep = 0;         // byte offset of the deep guides' filing in hguides (0: none)
     float* denoise_static = nullptr;  // this device's static denoiser working memory (denoise.hip)
+    // adaptive stopping (stopping.hip): each pixel's judged trial, its
+    // allotment, and the count pass's per-slot counts and dense offsets
+    DevBuf stop_wanted, stop_spreads, stop_allotted, stop_counts, stop_offsets, stop_tmp;
+    uint64_t stop_total = 0;        // the count pass's paths
+    int stop_width = 0, stop_height = 0;  // the image allotted (0: none)
+    int32_t stop_trial = 0;
 };
 
 // Glass scattering is held off the device: renders with glass faulted the
@@ -103,12 +109,23 @@ bool extendRays(Ctx& c, const kdn_ray* rays, uint32_t n, kdn_hit* hits, hipStrea
 // integrator.hip
 bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats);
 bool foldSamples(Ctx& c, const double* colours, uint32_t pixels, uint32_t spp, uint32_t batch, kdn_estimate* estimates);
+// an eye path's numbers (numberAt) of `count` sample indices at one coordinate
+bool sequenceNumbers(Ctx& c, uint64_t stream, const uint32_t* indices, uint32_t count, uint32_t coordinate, bool sobol, double* out);
 void freeIntegrator(Ctx& c);
 
 // guide.hip: the learned guide's parameters over a region, from a seed
 bool guideReset(Ctx& c, uint32_t seed, const float lower[3], const float size[3]);
 bool guideReady(const Ctx& c);
 void freeGuide(Ctx& c);
+
+// stopping.hip: effortFor, confidenceTrial, and the device's judge and
+// allotment of every pixel's count from the trial estimates
+void stoppingEffort(double tolerance, int32_t* samples, int32_t* batch);
+int32_t stoppingTrial(const kdn_stopping& s);
+bool stoppingCheck(const kdn_stopping& s, std::string& err);
+bool stoppingAllot(Ctx& c, const kdn_stopping& s, int32_t trial, uint64_t* total);
+bool stoppingSumStart(Ctx& c);
+bool readStopping(Ctx& c, float* wanted, float* spreads, int32_t* allotted);
 
 // denoise.hip: look up this device's static denoiser working memory
 bool denoiseStaticScratch(Ctx& c);
@@ -120,6 +137,9 @@ bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats, DevBuf&
 // *sorted is vals or valsAlt, whichever holds the sorted values
 bool sortPairsU32(uint32_t* keys, uint32_t* keysAlt, uint32_t* vals, uint32_t* valsAlt, uint32_t n, int endBit, DevBuf& tmp,
                   hipStream_t s, std::string& err, uint32_t** sorted);
+
+// sort.hip: exclusive prefix sum of n u64 values into out (rocPRIM's)
+bool scanU64(const uint64_t* in, uint64_t* out, size_t n, DevBuf& tmp, hipStream_t s, std::string& err);
 
 // scan.hip: exclusive prefix sum of n u32 values into out, returns the total
 bool scanU32(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* total, DevBuf& tmp, hipStream_t s, std::string& err);

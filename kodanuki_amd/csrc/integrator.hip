@@ -103,6 +103,24 @@ __global__ void k_fold_samples(const double* colours, uint32_t pixels, uint32_t 
     estimates[p] = e;
 }
 
+__global__ void k_sequence_numbers(uint64_t stream, const uint32_t* indices, uint32_t count, uint32_t coordinate, bool sobol,
+                                   double* out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) out[k] = sequence_number(sobol, stream, indices[k], coordinate);
+}
+
+bool sequenceNumbers(Ctx& c, uint64_t stream, const uint32_t* indices, uint32_t count, uint32_t coordinate, bool sobol, double* out) {
+    const size_t ib = (size_t)count * sizeof(uint32_t), ob = (size_t)count * sizeof(double);
+    if (!ensure(c.scratch[6], ib, c.error, false) || !ensure(c.scratch[7], ob, c.error, false)) return false;
+    KDN_HIP(hipMemcpy(c.scratch[6].ptr, indices, ib, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_sequence_numbers, dim3((count + 255) / 256), dim3(256), 0, c.stream, stream,
+                       (const uint32_t*)c.scratch[6].ptr, count, coordinate, sobol, (double*)c.scratch[7].ptr);
+    KDN_HIP(hipGetLastError());
+    KDN_HIP(hipStreamSynchronize(c.stream));
+    KDN_HIP(hipMemcpy(out, c.scratch[7].ptr, ob, hipMemcpyDeviceToHost));
+    return true;
+}
+
 bool foldSamples(Ctx& c, const double* colours, uint32_t pixels, uint32_t spp, uint32_t batch, kdn_estimate* estimates) {
     const size_t cb = (size_t)pixels * spp * 3 * sizeof(double), eb = (size_t)pixels * sizeof(kdn_estimate);
     if (!ensure(c.scratch[6], cb + 256, c.error, false) || !ensure(c.scratch[7], eb + 256, c.error, false)) return false;
@@ -269,7 +287,7 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
     if (!ensureColour(c)) return false;
     hipStream_t st = c.stream;
     const SliceDev& lastSlice = slices.back();
-    const uint32_t total = lastSlice.first_path + lastSlice.spp * jobs[lastSlice.job].plane;
+    const uint32_t total = lastSlice.first_path + slice_paths(lastSlice, jobs[lastSlice.job].plane);
     if (!allocPaths(c, total)) return false;
     // the round's job and slice tables in HBM
     const size_t jobBytes = jobs.size() * sizeof(JobDev), sliceBytes = slices.size() * sizeof(SliceDev);
@@ -513,7 +531,17 @@ bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
                   "the passes before it kept none";
         return false;
     }
+    // the count pass: every pixel its allotment less its trial
+    const bool counted = (d.flags & 256) != 0;
+    if (counted) {
+        if (c.stop_width != d.width || c.stop_height != d.height || fresh) {
+            c.error = "kdn_render_pass: no allotment for this image (kdn_stopping_allot after its trial pass)";
+            return false;
+        }
+        if ((d.flags & 64) && (d.flags & 1) && !stoppingSumStart(c)) return false;
+    }
     std::vector<JobDev> jobs{jobOf(c, d.width, d.height, d.seed, d.surfaces, d.min_depth)};
+    jobs[0].sobol = (d.flags & 128) ? 1u : 0u;
     if (d.flags & 32) {
         // the plain guide, of the image the guides were made for
         if ((d.flags & 16) || c.hguide_width != d.width || c.hguide_height != d.height) {
@@ -524,10 +552,20 @@ bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
         jobs[0].hg_deep = c.hguide_deep ? (const HgDeepHeader*)((const char*)c.hguides.ptr + c.hguide_deep) : nullptr;
     }
     const uint32_t plane = jobs[0].plane;
-    const size_t maxPaths = c.max_paths ? c.max_paths : (size_t)1 << 23;
+    const size_t maxPaths = d.round_paths ? std::max<size_t>(d.round_paths, 4096) : c.max_paths ? c.max_paths : (size_t)1 << 23;
     const uint32_t chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(d.spp, maxPaths / plane));
     kdn_pass_stats local{};
-    for (uint32_t first = 0; first < d.spp; first += chunk) {
+    if (counted) {
+        // rounds of at most maxPaths of the dense paths, pixel by pixel
+        for (uint64_t base = 0; base < c.stop_total; base += maxPaths) {
+            SliceDev sl{0, d.first_sample, 0, 0, c.accum, c.accum_sq, c.accum_n, (d.flags & 64) ? c.estimates : nullptr, d.batch};
+            sl.offsets = (const uint64_t*)c.stop_offsets.ptr;
+            sl.base = base;
+            sl.paths = (uint32_t)std::min<uint64_t>(maxPaths, c.stop_total - base);
+            if (!runRound(c, jobs, std::vector<SliceDev>{sl}, d.flags, &local)) return false;
+        }
+    }
+    for (uint32_t first = 0; !counted && first < d.spp; first += chunk) {
         const uint32_t n = std::min(chunk, d.spp - first);
         const std::vector<SliceDev> slices{SliceDev{0, d.first_sample + first, n, 0, c.accum, c.accum_sq, c.accum_n,
                                                     (d.flags & 64) ? c.estimates : nullptr, d.batch}};

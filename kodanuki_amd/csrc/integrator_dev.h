@@ -39,8 +39,110 @@ KDN_D uint64_t dimension(int event, int draw) {
     if (draw < LightU) return (uint64_t)draw;
     return (uint64_t)(4 + event * 8 + draw - LightU);
 }
-KDN_D float number(uint64_t stream, uint32_t sample, int event, int draw) {
-    return (float)hashed(stream, sample, dimension(event, draw));
+
+// ---- the Sobol member (plugins/sampler/sobol/sobol.cpp) -------------------------
+// The first 28 coordinates of an eye path (the pixel, the lens and three
+// events) come off an Owen-scrambled Sobol sequence, the rest stay hashed
+// (sampler.cpp:15-30). Each coordinate is its own Sobol dimension with Joe
+// and Kuo's direction numbers over 16 bits; the 6 shared coordinates take
+// dimensions 0..5 and the later ones take dimensions 0.. again on a second
+// index shuffle (sobol.h:13-21). The sample's place is shuffled by a
+// Laine-Karras hash of its reversed bits and every value scrambled the same
+// way, seeded by the upper half of the stream (sobol.cpp:129-148).
+constexpr uint32_t kSobolDimensions = 28;
+constexpr uint32_t kSharedCoordinates = 6;
+
+// The unscrambled values by place nibble: entry [d][q][k] XORs the
+// bit-reversed direction columns 4q..4q+3 of dimension d that the nibble k
+// selects, so a 16-bit place takes four loads from a 7 KB table that stays
+// in the vector L1.
+struct SobolNibbles {
+    uint32_t v[kSobolDimensions][4][16];
+};
+constexpr uint32_t reversed_bits(uint32_t v) {
+    v = (v << 16) | (v >> 16);
+    v = ((v & 0x00FF00FFu) << 8) | ((v & 0xFF00FF00u) >> 8);
+    v = ((v & 0x0F0F0F0Fu) << 4) | ((v & 0xF0F0F0F0u) >> 4);
+    v = ((v & 0x33333333u) << 2) | ((v & 0xCCCCCCCCu) >> 2);
+    return ((v & 0x55555555u) << 1) | ((v & 0xAAAAAAAAu) >> 1);
+}
+constexpr SobolNibbles sobol_nibbles() {
+    // degree, primitive polynomial's inner coefficients and the first
+    // direction numbers of each dimension (Joe and Kuo, new-joe-kuo-6.21201)
+    constexpr uint32_t rules[kSobolDimensions][9] = {
+        {0, 0, 1, 1, 1, 1, 1, 1, 1},      {1, 0, 1, 0, 0, 0, 0, 0, 0},      {2, 1, 1, 3, 0, 0, 0, 0, 0},
+        {3, 1, 1, 3, 1, 0, 0, 0, 0},      {3, 2, 1, 1, 1, 0, 0, 0, 0},      {4, 1, 1, 1, 3, 3, 0, 0, 0},
+        {4, 4, 1, 3, 5, 13, 0, 0, 0},     {5, 2, 1, 1, 5, 5, 17, 0, 0},     {5, 4, 1, 1, 5, 5, 5, 0, 0},
+        {5, 7, 1, 1, 7, 11, 19, 0, 0},    {5, 11, 1, 1, 5, 1, 1, 0, 0},     {5, 13, 1, 1, 1, 3, 11, 0, 0},
+        {5, 14, 1, 3, 5, 5, 31, 0, 0},    {6, 1, 1, 3, 3, 9, 7, 49, 0},     {6, 13, 1, 1, 1, 15, 21, 21, 0},
+        {6, 16, 1, 3, 1, 13, 27, 49, 0},  {6, 19, 1, 1, 1, 15, 7, 5, 0},    {6, 22, 1, 3, 1, 15, 13, 25, 0},
+        {6, 25, 1, 1, 5, 5, 19, 61, 0},   {7, 1, 1, 3, 7, 11, 23, 15, 103}, {7, 4, 1, 3, 7, 13, 13, 15, 69},
+        {7, 7, 1, 1, 3, 13, 7, 35, 63},   {7, 8, 1, 3, 5, 9, 1, 25, 53},    {7, 14, 1, 3, 1, 13, 9, 35, 107},
+        {7, 19, 1, 3, 1, 5, 27, 61, 31},  {7, 21, 1, 1, 5, 11, 19, 41, 61}, {7, 28, 1, 3, 5, 3, 3, 13, 69},
+        {7, 31, 1, 1, 7, 13, 1, 19, 1}};
+    SobolNibbles t{};
+    for (uint32_t d = 0; d < kSobolDimensions; ++d) {
+        const uint32_t degree = rules[d][0], poly = rules[d][1];
+        uint32_t m[16] = {};
+        uint32_t column[16] = {};
+        for (uint32_t b = 0; b < 16; ++b) {
+            if (degree == 0 || b < degree) {
+                m[b] = rules[d][2 + (degree == 0 ? 0 : b)];
+            } else {
+                uint32_t n = m[b - degree] ^ (m[b - degree] << degree);
+                for (uint32_t k = 1; k < degree; ++k)
+                    if ((poly >> (degree - 1 - k)) & 1u) n ^= m[b - k] << k;
+                m[b] = n;
+            }
+            column[b] = reversed_bits(m[b] << (31 - b));
+        }
+        for (uint32_t q = 0; q < 4; ++q)
+            for (uint32_t k = 0; k < 16; ++k) {
+                uint32_t x = 0;
+                for (uint32_t j = 0; j < 4; ++j)
+                    if ((k >> j) & 1u) x ^= column[4 * q + j];
+                t.v[d][q][k] = x;
+            }
+    }
+    return t;
+}
+static __constant__ SobolNibbles kSobol = sobol_nibbles();
+
+KDN_D uint32_t stirred(uint32_t v) {
+    v ^= v >> 16;
+    v *= 0x7FEB352Du;
+    v ^= v >> 15;
+    v *= 0x846CA68Bu;
+    return v ^ (v >> 16);
+}
+KDN_D uint32_t laine_karras(uint32_t v, uint32_t seed) {
+    v += seed;
+    v ^= v * 0x6C50B47Cu;
+    v ^= v * 0xB82F1E52u;
+    v ^= v * 0xC7AFE638u;
+    return v ^ (v * 0x8D22F6E6u);
+}
+KDN_D uint32_t reversed_sixteen(uint32_t v) { return reversed_bits(v & 0xFFFFu) >> 16; }
+// sobolNumber for a coordinate below kSobolDimensions, as a double
+KDN_D double sobol_number(uint64_t stream, uint32_t index, uint32_t coordinate) {
+    const uint32_t scramble = (uint32_t)(stream >> 32);
+    const bool shared = coordinate < kSharedCoordinates;
+    const uint32_t place =
+        reversed_sixteen(laine_karras(reversed_sixteen(index), stirred(scramble + (shared ? 0u : 0x68E31DA4u))));
+    const uint32_t seed = stirred(scramble + coordinate * 0xA136AAADu + 0x2545F491u);
+    const uint32_t (*t)[16] = kSobol.v[shared ? coordinate : coordinate - kSharedCoordinates];
+    const uint32_t value = t[0][place & 15u] ^ t[1][(place >> 4) & 15u] ^ t[2][(place >> 8) & 15u] ^ t[3][(place >> 12) & 15u];
+    return (double)reversed_bits(laine_karras(value, seed)) * 0x1p-32;
+}
+// numberAt (sampler.cpp:15-30): an eye path's coordinate, off the Sobol
+// sequence where the job draws with it and the coordinate is one of its
+// 28, hashed otherwise
+KDN_D double sequence_number(bool sobol, uint64_t stream, uint32_t sample, uint64_t coordinate) {
+    return sobol && coordinate < kSobolDimensions ? sobol_number(stream, sample, (uint32_t)coordinate)
+                                                 : hashed(stream, sample, coordinate);
+}
+KDN_D float number(bool sobol, uint64_t stream, uint32_t sample, int event, int draw) {
+    return (float)sequence_number(sobol, stream, sample, dimension(event, draw));
 }
 
 // ---- wavelengths (colour.cpp:236-257, spectrum.cpp) ----------------------------

@@ -42,6 +42,7 @@ struct JobDev {
     uint32_t plane;        // paths of one sample of every pixel (8x8 tiles)
     uint64_t seed_bits;
     uint32_t surfaces, min_depth;
+    uint32_t sobol;        // the first 28 coordinates off the Sobol sequence (0: all hashed)
     float eye[3], forward[3], right[3], up[3];
     float reach, aspect;
     float sky_zenith[3], sky_horizon[3], sky_ground[3];
@@ -68,7 +69,17 @@ struct SliceDev {
     uint32_t* accum_n;
     kdn_estimate* estimate;  // the job's per-pixel estimates (null: none kept)
     uint32_t batch;          // samples a batch of an estimate holds
+    // A counted slice (offsets set): pixel slot p draws offsets[p + 1] -
+    // offsets[p] samples from first_sample on, its own count, and the
+    // pass's paths are those samples pixel by pixel, densely numbered; this
+    // slice holds the dense paths [base, base + paths) of them (spp unused).
+    const uint64_t* offsets;
+    uint64_t base;
+    uint32_t paths;
 };
+
+// The paths of a slice.
+__host__ __device__ inline uint32_t slice_paths(const SliceDev& sl, uint32_t plane) { return sl.offsets ? sl.paths : sl.spp * plane; }
 
 // A round: the slices of one or more jobs whose paths share the queues.
 struct PassParams {
@@ -124,7 +135,23 @@ KDN_D void generate_one(PassParams& P, uint32_t i) {
     const SliceDev& sl = P.slices[k];
     const JobDev& J = P.jobs[sl.job];
     const uint32_t local = i - sl.first_path;
-    const uint32_t sIdx = local / J.plane, p = local % J.plane;
+    uint32_t sIdx, p;
+    if (sl.offsets) {
+        // the slot whose samples hold dense path g: the last p with
+        // offsets[p] <= g (offsets[0] = 0 <= g < offsets[plane])
+        const uint64_t g = sl.base + local;
+        uint32_t lo = 0, hi = J.plane;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (sl.offsets[mid] <= g) lo = mid;
+            else hi = mid;
+        }
+        p = lo;
+        sIdx = (uint32_t)(g - sl.offsets[lo]);
+    } else {
+        sIdx = local / J.plane;
+        p = local % J.plane;
+    }
     s.slice[i] = k;
     uint32_t x, y;
     pixel_of(J, p, &x, &y);
@@ -141,7 +168,7 @@ KDN_D void generate_one(PassParams& P, uint32_t i) {
     const double start = hashed(stream, ~0ull, 0);
     const Lanes w = sampled_wavelengths(wavelength_share(start, sample), P.colour);
     // filmDirection (core/camera.cpp:32-38) at the jittered film point
-    const double fx = x + hashed(stream, sample, PixelX), fy = y + hashed(stream, sample, PixelY);
+    const double fx = x + sequence_number(J.sobol, stream, sample, PixelX), fy = y + sequence_number(J.sobol, stream, sample, PixelY);
     const float u = (float)(2 * fx / J.width - 1), v = (float)(1 - 2 * fy / J.height);
     V3 d = v3(J.forward[0], J.forward[1], J.forward[2]) + v3(J.right[0], J.right[1], J.right[2]) * (u * J.reach * J.aspect) +
            v3(J.up[0], J.up[1], J.up[2]) * (v * J.reach);
@@ -507,11 +534,11 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
         // aim at one lamp (aimedFrom, trace.cpp:896-947; aimLamp,
         // light.cpp:359-370), MIS against meeting
         if (!exact && J.n_lamps > 0) {
-            const float uPick = number(stream, sample, depth, Pick);
+            const float uPick = number(J.sobol, stream, sample, depth, Pick);
             uint32_t j = 0;
             while (j + 1 < J.n_lamps && uPick >= J.lamps[j].pick_cdf) ++j;
             const LampDev& L = J.lamps[j];
-            const float u = number(stream, sample, depth, LightU), v = number(stream, sample, depth, LightV);
+            const float u = number(J.sobol, stream, sample, depth, LightU), v = number(J.sobol, stream, sample, depth, LightV);
             const V3 at = v3(L.at[0], L.at[1], L.at[2]);
             if (L.kind == KDN_LIGHT_QUAD) {
                 const V3 y = at + v3(L.along[0], L.along[1], L.along[2]) * u + v3(L.across[0], L.across[1], L.across[2]) * v;
@@ -599,8 +626,8 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
         }
         // scatter (scatteredFrom, trace.cpp:1020-1074)
         if (depth + 1 < (int)J.surfaces) {
-            const float uLobe = number(stream, sample, depth, Lobe), uSide = number(stream, sample, depth, Side);
-            const float du = number(stream, sample, depth, DirectionU), dv = number(stream, sample, depth, DirectionV);
+            const float uLobe = number(J.sobol, stream, sample, depth, Lobe), uSide = number(J.sobol, stream, sample, depth, Side);
+            const float du = number(J.sobol, stream, sample, depth, DirectionU), dv = number(J.sobol, stream, sample, depth, DirectionV);
             const Scatter sc = m.kind == KDN_GLASS ? glass_scatter(m, w, n, out, inside, uSide)
                                : guided ? guided_scatter(m, w, n, out, lobes, share, guide_number(stream, sample, depth, 0),
                                                          guide_number(stream, sample, depth, 1), uLobe, du, dv)
@@ -623,7 +650,7 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
                     bool survives = true;
                     if (depth >= (int)J.min_depth) {
                         const float survival = fmin_(0.95f, fmax_(fmax_(carried[0], carried[1]), fmax_(carried[2], carried[3])));
-                        if (!(number(stream, sample, depth, Survival) < survival)) {
+                        if (!(number(J.sobol, stream, sample, depth, Survival) < survival)) {
                             survives = false;
                         } else {
                             for (int l = 0; l < kLanes; ++l) carried[l] /= survival;
@@ -810,13 +837,25 @@ KDN_D void fold_one(const PassParams& P, uint32_t k, uint32_t t) {
     if (t >= J.plane) return;
     const PathState& s = P.s;
     const size_t N = s.capacity;
-    const uint32_t pixel = s.pixel[sl.first_path + t];
+    // the slot's paths in this slice: n of them from first, stride apart
+    size_t first = (size_t)sl.first_path + t, stride = J.plane;
+    uint32_t n = sl.spp;
+    if (sl.offsets) {
+        const uint64_t end = sl.base + sl.paths;
+        const uint64_t lo = sl.offsets[t] > sl.base ? sl.offsets[t] : sl.base;
+        const uint64_t hi = sl.offsets[t + 1] < end ? sl.offsets[t + 1] : end;
+        if (lo >= hi) return;
+        first = (size_t)sl.first_path + (size_t)(lo - sl.base);
+        stride = 1;
+        n = (uint32_t)(hi - lo);
+    }
+    const uint32_t pixel = s.pixel[first];
     if (pixel == 0xFFFFFFFFu) return;
     double sum[3] = {0, 0, 0}, sq[3] = {0, 0, 0};
     kdn_estimate est;
     if (sl.estimate) est = sl.estimate[pixel];
-    for (uint32_t j = 0; j < sl.spp; ++j) {
-        const size_t i = (size_t)sl.first_path + (size_t)j * J.plane + t;
+    for (uint32_t j = 0; j < n; ++j) {
+        const size_t i = first + (size_t)j * stride;
         Lanes w;
         float rad[kLanes];
         for (int l = 0; l < kLanes; ++l) {
@@ -837,7 +876,7 @@ KDN_D void fold_one(const PassParams& P, uint32_t k, uint32_t t) {
         sl.accum[3 * (size_t)pixel + c] += sum[c];
         sl.accum_sq[3 * (size_t)pixel + c] += sq[c];
     }
-    sl.accum_n[pixel] += sl.spp;
+    sl.accum_n[pixel] += n;
 }
 
 }  // namespace kdn

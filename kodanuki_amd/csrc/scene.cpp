@@ -650,7 +650,9 @@ private:
                 else if (n == "surfaces") scene_.render.surfaces = std::max(1, int(x));
                 else if (n == "far") scene_.render.far_ = float(x);
                 else if (n == "batch") scene_.render.batch = int(x);
-                else if (n == "tolerance" || n == "split" || n == "collisions") {}
+                else if (n == "tolerance") scene_.render.tolerance = float(x);
+                else if (n == "split") scene_.render.split = int(x);
+                else if (n == "collisions") {}
                 else fail(at, "render has no field " + n);
             });
         } else if (kind.text == "camera") {

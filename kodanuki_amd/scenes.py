@@ -210,6 +210,21 @@ paint(steel) at([0, 0, 0.01]) box([1.2, 1.2, 0.02]);
 """
 
 
+def stopping_table(width=64, height=40, tolerance=6, samples=64, batch=8):
+    """A matte table top with a plastic and a metal ball under a quad lamp
+    and a dim sky: quiet floor and sky pixels, noisy highlights and
+    shadow edges, the spread adaptive stopping divides its samples by."""
+    return f"""import "standard.sdf";
+render {{ width = {width}; height = {height}; seed = 1; tolerance = {tolerance}; samples = {samples}; batch = {batch}; surfaces = 6; far = 8; }}
+camera {{ eye = [0, -4.2, 1.6]; target = [0, 0, 0.35]; up = [0, 0, 1]; fov = 0.7; }}
+sky gradient {{ zenith = [0.25, 0.3, 0.45]; horizon = [0.45, 0.45, 0.45]; ground = [0.1, 0.1, 0.1]; }}
+light quad {{ at = [0.8, -1.4, 2.2]; along = [0.4, 0, 0]; across = [0, 0.4, 0]; power = [9, 9, 8]; }}
+paint(matte([0.6, 0.6, 0.58])) at([0, 0, -0.1]) box([6, 6, 0.2]);
+paint(plastic([0.8, 0.25, 0.1], roughness = 0.3)) at([-0.6, 0, 0.5]) sphere(0.5);
+paint(metal([0.9, 0.8, 0.5], roughness = 0.1)) at([0.65, 0.3, 0.4]) sphere(0.4);
+"""
+
+
 SERVING_COUNTS = (32, 64, 128, 256, 512, 1024, 2048, 4096)
 
 

@@ -94,12 +94,20 @@ typedef struct or_render_scene {
 void or_render64(const or_render_scene* scene, int width, int height, int x0, int y0, int x1, int y1, uint32_t first,
                  uint32_t spp, uint32_t seed, uint32_t surfaces, uint32_t min_depth, double* rgb, int threads);
 
+/* The same with the Sobol member drawing an eye path's first 28
+ * coordinates where sobol is set. */
+void or_render64_sequenced(const or_render_scene* scene, int width, int height, int x0, int y0, int x1, int y1, uint32_t first,
+                           uint32_t spp, uint32_t seed, uint32_t surfaces, uint32_t min_depth, int sobol, double* rgb, int threads);
+
 /* Reference helpers checked on their own. */
+double or_sobol_number(uint64_t stream, uint64_t index, uint64_t coordinate);
 uint64_t or_mixed_bits(uint64_t v);
 double or_hashed_number(uint64_t stream, uint64_t index, uint64_t coordinate);
 void or_sampled_wavelengths(double u, double nanometres[4], double density[4]);
 void or_colour_of(const double nanometres[4], const double density[4], const double spectrum[4], double rgb[3]);
 void or_spectrum_of(const double rgb[3], const double nanometres[4], double out[4]);
+/* The GGX escape table of a width (VIEWS = 32 entries) and its average. */
+void or_escape_of(double width, double table[32], double* average);
 
 #ifdef __cplusplus
 }

@@ -207,6 +207,8 @@ static void escape_of(double width, double* table, double* avg) {
         *avg += 2 * table[view] * leaving / VIEWS;
     }
 }
+/* escapeOf (energy.cpp:100-127), exported for the pins. */
+void or_escape_of(double width, double table[32], double* average) { escape_of(width, table, average); }
 static double escape_at(const Mat* m, double c) {
     const double place = clampd(c, 0, 1) * VIEWS - 0.5;
     int lo = (int)floor(place);
@@ -359,11 +361,79 @@ static int meet_quad(const Lamp* L, V o, V d, double tmax, double* t, double* co
 }
 static double power_weight(double a, double b) { return a * a + b * b > 0 ? a * a / (a * a + b * b) : 0; }
 
-/* ---- numbers (path.cpp:22-37, random.cpp:19-21) ---- */
-static double number(uint64_t stream, uint32_t sample, int event, int draw) {
-    const uint64_t dim = draw < 4 ? (uint64_t)draw : (uint64_t)(4 + event * 8 + draw - 4);
-    return or_hashed_number(stream, sample, dim);
+/* ---- the Sobol member (plugins/sampler/sobol/sobol.cpp:8-156) ----
+ * 28 coordinates, each its own dimension of Joe and Kuo's sequence over
+ * 16 bits; coordinates from 6 on take the dimensions again on a second
+ * index shuffle; place and value scrambled by Laine-Karras hashes seeded by
+ * the upper half of the stream. */
+static uint32_t stir32(uint32_t v) {
+    v ^= v >> 16;
+    v *= 0x7FEB352Du;
+    v ^= v >> 15;
+    v *= 0x846CA68Bu;
+    return v ^ (v >> 16);
 }
+static uint32_t rev32(uint32_t v) {
+    uint32_t r = 0;
+    for (int b = 0; b < 32; ++b) r |= ((v >> b) & 1u) << (31 - b);
+    return r;
+}
+static uint32_t rev16(uint32_t v) { return rev32(v & 0xFFFFu) >> 16; }
+static uint32_t lk_scramble(uint32_t v, uint32_t seed) {
+    v += seed;
+    v ^= v * 0x6C50B47Cu;
+    v ^= v * 0xB82F1E52u;
+    v ^= v * 0xC7AFE638u;
+    return v ^ (v * 0x8D22F6E6u);
+}
+static const uint32_t kSobolRules[28][9] = {
+    {0, 0, 1, 1, 1, 1, 1, 1, 1},      {1, 0, 1, 0, 0, 0, 0, 0, 0},      {2, 1, 1, 3, 0, 0, 0, 0, 0},
+    {3, 1, 1, 3, 1, 0, 0, 0, 0},      {3, 2, 1, 1, 1, 0, 0, 0, 0},      {4, 1, 1, 1, 3, 3, 0, 0, 0},
+    {4, 4, 1, 3, 5, 13, 0, 0, 0},     {5, 2, 1, 1, 5, 5, 17, 0, 0},     {5, 4, 1, 1, 5, 5, 5, 0, 0},
+    {5, 7, 1, 1, 7, 11, 19, 0, 0},    {5, 11, 1, 1, 5, 1, 1, 0, 0},     {5, 13, 1, 1, 1, 3, 11, 0, 0},
+    {5, 14, 1, 3, 5, 5, 31, 0, 0},    {6, 1, 1, 3, 3, 9, 7, 49, 0},     {6, 13, 1, 1, 1, 15, 21, 21, 0},
+    {6, 16, 1, 3, 1, 13, 27, 49, 0},  {6, 19, 1, 1, 1, 15, 7, 5, 0},    {6, 22, 1, 3, 1, 15, 13, 25, 0},
+    {6, 25, 1, 1, 5, 5, 19, 61, 0},   {7, 1, 1, 3, 7, 11, 23, 15, 103}, {7, 4, 1, 3, 7, 13, 13, 15, 69},
+    {7, 7, 1, 1, 3, 13, 7, 35, 63},   {7, 8, 1, 3, 5, 9, 1, 25, 53},    {7, 14, 1, 3, 1, 13, 9, 35, 107},
+    {7, 19, 1, 3, 1, 5, 27, 61, 31},  {7, 21, 1, 1, 5, 11, 19, 41, 61}, {7, 28, 1, 3, 5, 3, 3, 13, 69},
+    {7, 31, 1, 1, 7, 13, 1, 19, 1}};
+static uint32_t g_sobol_columns[28][16];
+static pthread_once_t g_sobol_once = PTHREAD_ONCE_INIT;
+static void build_sobol(void) {
+    for (int d = 0; d < 28; ++d) {
+        const uint32_t degree = kSobolRules[d][0], poly = kSobolRules[d][1];
+        uint32_t m[16];
+        for (uint32_t b = 0; b < 16; ++b) {
+            if (degree == 0 || b < degree) {
+                m[b] = kSobolRules[d][2 + (degree == 0 ? 0 : b)];
+            } else {
+                uint32_t n = m[b - degree] ^ (m[b - degree] << degree);
+                for (uint32_t k = 1; k < degree; ++k)
+                    if ((poly >> (degree - 1 - k)) & 1u) n ^= m[b - k] << k;
+                m[b] = n;
+            }
+            g_sobol_columns[d][b] = rev32(m[b] << (31 - b));
+        }
+    }
+}
+double or_sobol_number(uint64_t stream, uint64_t index, uint64_t coordinate) {
+    pthread_once(&g_sobol_once, build_sobol);
+    const uint32_t scramble = (uint32_t)(stream >> 32), c = (uint32_t)coordinate;
+    const uint32_t place = rev16(lk_scramble(rev16((uint32_t)index), stir32(scramble + (c < 6 ? 0u : 0x68E31DA4u))));
+    const uint32_t seed = stir32(scramble + c * 0xA136AAADu + 0x2545F491u);
+    const uint32_t* col = g_sobol_columns[c < 6 ? c : c - 6];
+    uint32_t value = 0;
+    for (int b = 0; b < 16; ++b)
+        if ((place >> b) & 1u) value ^= col[b];
+    return (double)rev32(lk_scramble(value, seed)) * 0x1p-32;
+}
+
+/* ---- numbers (path.cpp:22-37, sampler.cpp:15-30, random.cpp:19-21) ---- */
+static double sequence_number(int sobol, uint64_t stream, uint32_t sample, uint64_t dim) {
+    return sobol && dim < 28 ? or_sobol_number(stream, sample, dim) : or_hashed_number(stream, sample, dim);
+}
+#define number(stream, sample, event, draw) \
+    sequence_number(J->sobol, (stream), (sample), (draw) < 4 ? (uint64_t)(draw) : (uint64_t)(4 + (event) * 8 + (draw) - 4))
 enum { PIXEL_X = 0, PIXEL_Y = 1, LIGHT_U = 4, LIGHT_V = 5, DIR_U = 6, DIR_V = 7, LOBE = 8, SIDE = 9, SURVIVAL = 10, PICK = 11 };
 
 typedef struct {
@@ -375,6 +445,7 @@ typedef struct {
     int width, height, x0, y0, x1, y1;
     uint32_t first, spp, surfaces, min_depth;
     uint64_t seed_bits;
+    int sobol;
     V eye, fwd, right, up;
     double reach, aspect;
     double* rgb;
@@ -542,8 +613,14 @@ static void* worker(void* arg) {
 
 void or_render64(const or_render_scene* scene, int width, int height, int x0, int y0, int x1, int y1, uint32_t first, uint32_t spp,
                  uint32_t seed, uint32_t surfaces, uint32_t min_depth, double* rgb, int threads) {
+    or_render64_sequenced(scene, width, height, x0, y0, x1, y1, first, spp, seed, surfaces, min_depth, 0, rgb, threads);
+}
+
+void or_render64_sequenced(const or_render_scene* scene, int width, int height, int x0, int y0, int x1, int y1, uint32_t first,
+                           uint32_t spp, uint32_t seed, uint32_t surfaces, uint32_t min_depth, int sobol, double* rgb, int threads) {
     Job J;
     memset(&J, 0, sizeof(J));
+    J.sobol = sobol;
     J.scene = scene;
     J.nmat = scene->material_count;
     J.mats = (Mat*)calloc(J.nmat ? J.nmat : 1, sizeof(Mat));

@@ -23,4 +23,14 @@ bool sortPairsU32(uint32_t* keys, uint32_t* keysAlt, uint32_t* vals, uint32_t* v
     return true;
 }
 
+bool scanU64(const uint64_t* in, uint64_t* out, size_t n, DevBuf&, hipStream_t, std::string&) {
+    uint64_t acc = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const uint64_t v = in[i];
+        out[i] = acc;
+        acc += v;
+    }
+    return true;
+}
+
 }  // namespace kdn

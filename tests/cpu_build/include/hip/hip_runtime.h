@@ -43,6 +43,7 @@ extern "C" __attribute__((used, visibility("default"))) inline const uint64_t* k
 #endif
 
 #define __device__
+#define __constant__
 #define __host__
 #define __global__
 #define __forceinline__ inline

@@ -53,6 +53,9 @@ def lib():
         L.or_mixed_bits.argtypes = [C.c_uint64]
         L.or_hashed_number.restype = C.c_double
         L.or_hashed_number.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+        L.or_sobol_number.restype = C.c_double
+        L.or_sobol_number.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+        L.or_escape_of.argtypes = [C.c_double, P, P]
         _lib = L
     return _lib
 
@@ -170,11 +173,13 @@ class RenderScene(C.Structure):
                 ("camera", k.Camera), ("sky_zenith", C.c_float * 3), ("sky_horizon", C.c_float * 3), ("sky_ground", C.c_float * 3)]
 
 
-def render64(scene, arrays, lower, size, width, height, spp, first=0, seed=1, surfaces=7, min_depth=4, window=None, threads=None):
-    """The FP64 path tracer over a window of the image: rgb sums (h, w, 3)."""
+def render64(scene, arrays, lower, size, width, height, spp, first=0, seed=1, surfaces=7, min_depth=4, window=None, threads=None,
+             sobol=False):
+    """The FP64 path tracer over a window of the image: rgb sums (h, w, 3);
+    with sobol, an eye path's first 28 coordinates off the Sobol member."""
     L = lib()
-    L.or_render64.argtypes = [C.POINTER(RenderScene), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32,
-                              C.c_uint32, C.c_uint32, C.c_uint32, P, C.c_int]
+    L.or_render64_sequenced.argtypes = [C.POINTER(RenderScene), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32,
+                                        C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, P, C.c_int]
     x0, y0, x1, y1 = window or (0, 0, width, height)
     mats = scene.materials()
     grey = k.Material()
@@ -206,6 +211,6 @@ def render64(scene, arrays, lower, size, width, height, spp, first=0, seed=1, su
     R.sky_horizon[:] = rs.sky_horizon[:]
     R.sky_ground[:] = rs.sky_ground[:]
     rgb = np.zeros((y1 - y0, x1 - x0, 3), np.float64)
-    L.or_render64(C.byref(R), width, height, x0, y0, x1, y1, first, spp, seed, surfaces, min_depth, ptr(rgb),
-                  threads or min(16, os.cpu_count() or 1))
+    L.or_render64_sequenced(C.byref(R), width, height, x0, y0, x1, y1, first, spp, seed, surfaces, min_depth, 1 if sobol else 0,
+                            ptr(rgb), threads or min(16, os.cpu_count() or 1))
     return rgb
