@@ -350,7 +350,11 @@ typedef struct kdn_pass_desc {
                                       trial (kdn_stopping_allot), samples
                                       first_sample.. (spp unused); with bits
                                       0 and 6 the estimates restart from the
-                                      trial's totals (sumStart) */
+                                      trial's totals (sumStart);
+                               bit 9: keep the frame record's surfaces:
+                                      each pixel's first surface of its last
+                                      sample (first_sample + spp - 1), for
+                                      kdn_temporal_frame (not with bit 8) */
     uint32_t batch;         /* samples a batch holds (bit 6; 0: no batch is ever closed) */
     uint32_t round_paths;   /* the most paths of one round (0: 2^23; at least 4096) */
 } kdn_pass_desc;
@@ -480,6 +484,23 @@ int kdn_image_write_pfm(const float* rgb, int32_t width, int32_t height, const c
  * albedo multiplied back and the filtered light faded into the held light
  * by 1 / (1 + length / 8).
  */
+/* What a camera ray met first (Surface, core/batch.h:244-279): distance
+ * along the ray (infinite on a miss), unit normal facing the ray, leaf
+ * (KDN_NO_LEAF on a miss) and albedo (glass white, zero on a miss). */
+typedef struct kdn_surface {
+    float distance;
+    float normal[3];
+    uint32_t leaf;
+    float albedo[3];
+} kdn_surface;
+
+/* A lens (Lens, core/camera.h:14-56, pinhole): the eye, its unit frame,
+ * tan(field of view / 2) and width / height. */
+typedef struct kdn_lens {
+    double eye[3], forward[3], right[3], up[3];
+    double reach, aspect;
+} kdn_lens;
+
 /* A pixel of the temporal history (TemporalPixel, temporal/temporal.h:18-70). */
 typedef struct kdn_denoise_pixel {
     double colour[3];    /* its light (its lighting where demodulated) */
@@ -527,6 +548,51 @@ typedef struct kdn_denoise_stats {
  * through the pixel centres gives distance, normal, leaf and albedo.
  * stats may be NULL. */
 int kdn_denoise_accumulation(kdn_ctx* ctx, float* rgb, kdn_denoise_stats* stats);
+
+/* ---- the temporal member ---------------------------------------------------- */
+/*
+ * temporalFrame (reconstruction/temporal/temporal.h:237) on the device, for
+ * a viewer's frames drawn with a G-buffer at factor 1 (every pixel drawn
+ * each frame through its centre; the history demodulated): each pixel's
+ * history reprojected through the G-buffer into the camera before and read
+ * from the pixels on its plane, blended with the frame's sample where that
+ * sample's first surface is the G-buffer's leaf, clamped to its 3x3
+ * neighbourhood; the shown light is the lighting times the G-buffer's
+ * albedo. The history stays resident in HBM between frames. The upscaling
+ * mode (shown pixels finer than the drawn ones) is not built.
+ */
+typedef struct kdn_temporal_desc {
+    uint32_t moved;      /* the camera moved since the frame before (ShownFrame.moved) */
+    uint32_t restarted;  /* drop every history (ShownFrame.restarted) */
+    uint32_t recounted;  /* the scene's leaves were renumbered: leaves match as sky or not (ShownFrame.recounted) */
+    uint32_t denoise;    /* run the a-trous passes on the history after it (kdn_denoise_device) */
+} kdn_temporal_desc;
+typedef struct kdn_temporal_stats {
+    double gbuffer_ms;   /* the G-buffer's primary rays */
+    double temporal_ms;  /* the temporal kernel */
+    double filter_ms;    /* the a-trous passes (desc.denoise) */
+} kdn_temporal_stats;
+/* Set the resident scene's camera for the passes and frames after it (a
+ * viewer's orbit). */
+int kdn_set_camera(kdn_ctx* ctx, const kdn_camera* camera);
+/* One viewer frame after a fresh pass (kdn_render_pass with flags bit 9,
+ * not keeping the accumulators): the G-buffer of the scene's camera, the
+ * frame's lights (the pass's means) and record, and the resident history
+ * -> the next history, and the shown (and, with desc.denoise, the
+ * denoised) image, width * height * 3 floats each (host; either may be
+ * NULL). stats may be NULL. */
+int kdn_temporal_frame(kdn_ctx* ctx, const kdn_temporal_desc* desc, float* shown, float* denoised, kdn_temporal_stats* stats);
+/* Read the resident history (width * height pixels by rows). */
+int kdn_temporal_history(kdn_ctx* ctx, kdn_denoise_pixel* pixels);
+/* The temporal step on given frames (host buffers): this frame's lens and
+ * the one before, its lights (width * height * 3 doubles), record and
+ * G-buffer surfaces, and the history before (NULL, or before_width x
+ * before_height pixels) -> the history after and the shown image. The
+ * resident history is neither read nor changed. */
+int kdn_temporal_step(kdn_ctx* ctx, int32_t width, int32_t height, const kdn_lens* lens, const kdn_lens* before,
+                      const kdn_temporal_desc* desc, const double* parts, const kdn_surface* record, const kdn_surface* gbuffer,
+                      const kdn_denoise_pixel* before_pixels, int32_t before_width, int32_t before_height,
+                      kdn_denoise_pixel* pixels, float* shown);
 
 /* ---- the learned guide ------------------------------------------------------ */
 /*

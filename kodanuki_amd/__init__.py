@@ -123,7 +123,7 @@ class PassStats(C.Structure):
 
 
 PASS_KEEP, PASS_TIME, PASS_COUNT, PASS_NO_FINISH, PASS_GUIDE, PASS_HIGHLIGHT, PASS_ESTIMATES = 1, 2, 4, 8, 16, 32, 64
-PASS_SOBOL, PASS_COUNTED = 128, 256
+PASS_SOBOL, PASS_COUNTED, PASS_RECORD = 128, 256, 512
 
 
 class Stopping(C.Structure):
@@ -183,6 +183,42 @@ class DenoiseFrame(C.Structure):
 
 class DenoiseStats(C.Structure):
     _fields_ = [("gbuffer_ms", C.c_double), ("filter_ms", C.c_double)]
+
+
+# kdn_surface: what a camera ray met first (the reference's Surface)
+SURFACE_DTYPE = np.dtype([("distance", "<f4"), ("normal", "<f4", 3), ("leaf", "<u4"), ("albedo", "<f4", 3)])
+assert SURFACE_DTYPE.itemsize == 32
+
+
+class Lens(C.Structure):
+    _fields_ = [("eye", C.c_double * 3), ("forward", C.c_double * 3), ("right", C.c_double * 3), ("up", C.c_double * 3),
+                ("reach", C.c_double), ("aspect", C.c_double)]
+
+
+def lens_of(eye, target, up, fov, width, height):
+    """The pinhole lens of a camera at an image's size (lensOf, core/camera.cpp:9-25)."""
+    eye = np.asarray(eye, np.float64)
+    forward = np.asarray(target, np.float64) - eye
+    forward = forward / np.sqrt(forward @ forward)
+    up = np.asarray(up, np.float64)
+    right = np.array([forward[1] * up[2] - forward[2] * up[1], forward[2] * up[0] - forward[0] * up[2],
+                      forward[0] * up[1] - forward[1] * up[0]])
+    right = right / np.sqrt(right @ right)
+    upv = np.array([right[1] * forward[2] - right[2] * forward[1], right[2] * forward[0] - right[0] * forward[2],
+                    right[0] * forward[1] - right[1] * forward[0]])
+    lens = Lens()
+    lens.eye[:], lens.forward[:], lens.right[:], lens.up[:] = list(eye), list(forward), list(right), list(upv)
+    lens.reach = float(np.tan(np.float64(np.float32(fov)) / 2))
+    lens.aspect = width / height
+    return lens
+
+
+class TemporalDesc(C.Structure):
+    _fields_ = [("moved", C.c_uint32), ("restarted", C.c_uint32), ("recounted", C.c_uint32), ("denoise", C.c_uint32)]
+
+
+class TemporalStats(C.Structure):
+    _fields_ = [("gbuffer_ms", C.c_double), ("temporal_ms", C.c_double), ("filter_ms", C.c_double)]
 
 
 # a pixel's packed plain guide: count, material, share, unused, then 12
@@ -258,6 +294,7 @@ EXPORTS = [
     "kdn_stopping_effort", "kdn_stopping_trial", "kdn_stopping_allot", "kdn_read_stopping", "kdn_sequence_numbers",
     "kdn_image_write_ppm", "kdn_image_encode", "kdn_image_write_pfm",
     "kdn_denoise", "kdn_denoise_device", "kdn_denoise_device_static", "kdn_denoise_accumulation",
+    "kdn_set_camera", "kdn_temporal_frame", "kdn_temporal_history", "kdn_temporal_step",
     "kdn_guide_reset", "kdn_guide_configure", "kdn_guide_describe", "kdn_guide_params", "kdn_guide_set_params", "kdn_guide_infer",
     "kdn_guide_gradients", "kdn_guide_train",
     "kdn_serve_open", "kdn_serve_submit", "kdn_serve_step", "kdn_serve_status", "kdn_serve_result",
@@ -325,6 +362,10 @@ def _bind(lib):
         "kdn_denoise_device": (C.c_int, [P, C.POINTER(DenoiseFrame), P, P, P]),
         "kdn_denoise_device_static": (C.c_int, [P, C.POINTER(DenoiseFrame), P, P, P]),
         "kdn_denoise_accumulation": (C.c_int, [P, P, C.POINTER(DenoiseStats)]),
+        "kdn_set_camera": (C.c_int, [P, P]),
+        "kdn_temporal_frame": (C.c_int, [P, P, P, P, P]),
+        "kdn_temporal_history": (C.c_int, [P, P]),
+        "kdn_temporal_step": (C.c_int, [P, C.c_int32, C.c_int32, P, P, P, P, P, P, P, C.c_int32, C.c_int32, P, P]),
         "kdn_highlight_guides": (C.c_int, [P, C.c_int32, C.c_int32, C.POINTER(HighlightStats)]),
         "kdn_highlight_guide_at": (C.c_int, [P, C.c_int32, C.c_int32, P]),
         "kdn_highlight_density": (C.c_int, [P, P, P, P, P, P, C.c_uint32, P]),
@@ -522,6 +563,7 @@ class Device:
 
     def render_pass(self, width, height, spp, first_sample=0, seed=1, surfaces=7, min_depth=4, keep=False, flags=0, batch=0,
                     round_paths=0):
+        self._image_size = (width, height)
         d = PassDesc(width, height, first_sample, spp, seed, surfaces, min_depth, (PASS_KEEP if keep else 0) | flags, batch,
                      round_paths)
         st = PassStats()
@@ -617,6 +659,50 @@ class Device:
         st = DenoiseStats()
         _check(library().kdn_denoise_accumulation(self._h, _ptr(rgb), C.byref(st) if stats else None), self._h)
         return (rgb, st) if stats else rgb
+
+    # ---- the temporal member (a viewer's frames) ----
+    def set_camera(self, eye, target, up, fov):
+        cam = Camera()
+        cam.eye[:], cam.target[:], cam.up[:] = list(eye), list(target), list(up)
+        cam.fov = fov
+        _check(library().kdn_set_camera(self._h, C.byref(cam)), self._h)
+
+    def temporal_frame(self, moved=False, restarted=False, recounted=False, denoise=False, stats=False):
+        """One viewer frame after render_pass(..., flags=PASS_RECORD): the
+        shown image [h, w, 3] (and the denoised one with denoise)."""
+        w, h = self._image_size
+        shown = np.zeros((h, w, 3), np.float32)
+        denoised = np.zeros((h, w, 3), np.float32) if denoise else None
+        d = TemporalDesc(int(moved), int(restarted), int(recounted), int(denoise))
+        st = TemporalStats()
+        _check(library().kdn_temporal_frame(self._h, C.byref(d), _ptr(shown), _ptr(denoised) if denoise else None,
+                                            C.byref(st) if stats else None), self._h)
+        out = (shown, denoised) if denoise else shown
+        return (out, st) if stats else out
+
+    def temporal_history(self, width, height):
+        px = np.zeros((height, width), DENOISE_PIXEL_DTYPE)
+        _check(library().kdn_temporal_history(self._h, _ptr(px)), self._h)
+        return px
+
+    def temporal_step(self, lens, before_lens, parts, record, gbuffer, before=None, moved=False, restarted=False,
+                      recounted=False):
+        """The temporal step on given frames: parts [h, w, 3] f64, record and
+        gbuffer [h, w] SURFACE_DTYPE, before None or [bh, bw]
+        DENOISE_PIXEL_DTYPE -> (history [h, w], shown [h, w, 3])."""
+        parts = np.ascontiguousarray(parts, np.float64)
+        h, w, _ = parts.shape
+        record = np.ascontiguousarray(record, SURFACE_DTYPE)
+        gbuffer = np.ascontiguousarray(gbuffer, SURFACE_DTYPE)
+        bp = None if before is None else np.ascontiguousarray(before, DENOISE_PIXEL_DTYPE)
+        out = np.zeros((h, w), DENOISE_PIXEL_DTYPE)
+        shown = np.zeros((h, w, 3), np.float32)
+        d = TemporalDesc(int(moved), int(restarted), int(recounted), 0)
+        _check(library().kdn_temporal_step(self._h, w, h, C.byref(lens), C.byref(before_lens), C.byref(d), _ptr(parts), _ptr(record),
+                                           _ptr(gbuffer), _ptr(bp) if bp is not None else None,
+                                           bp.shape[1] if bp is not None else 0, bp.shape[0] if bp is not None else 0,
+                                           _ptr(out), _ptr(shown)), self._h)
+        return out, shown
 
     # ---- the plain (highlight) guide ----
     def highlight_guides(self, width, height):

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libkodanuki_hip.so")
 
 SOURCES = ["abi.cpp", "scene.cpp", "tables.cpp", "image.cpp", "scan.hip", "octree.hip", "trace.hip", "integrator.hip", "serve.hip", "guide.hip", "guide_net.hip",
-           "denoise.hip", "hguide.hip", "sort.hip", "stopping.hip"]
+           "denoise.hip", "hguide.hip", "sort.hip", "stopping.hip", "temporal.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

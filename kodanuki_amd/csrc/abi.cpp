@@ -237,7 +237,13 @@ void releaseAll(Ctx& c) {
     freeGuide(c);
     freeBuf(c.hguides);
     freeBuf(c.sort_tmp);
-    for (DevBuf* b : {&c.stop_wanted, &c.stop_spreads, &c.stop_allotted, &c.stop_counts, &c.stop_offsets, &c.stop_tmp}) freeBuf(*b);
+    for (DevBuf* b : {&c.stop_wanted, &c.stop_spreads, &c.stop_allotted, &c.stop_counts, &c.stop_offsets, &c.stop_tmp, &c.tm_history[0],
+                      &c.tm_history[1], &c.tm_gbuffer})
+        freeBuf(*b);
+    if (c.firsts) hipFree(c.firsts);
+    c.firsts = nullptr;
+    c.firsts_width = c.firsts_height = 0;
+    c.tm_width = c.tm_height = 0;
     c.stop_width = c.stop_height = 0;
     c.stop_total = 0;
     c.hguide_width = c.hguide_height = 0;
@@ -697,6 +703,51 @@ int kdn_sequence_numbers(kdn_ctx* ctx, uint64_t stream, const uint32_t* indices,
     kdn::Ctx& c = ctx->c;
     hipSetDevice(c.device);
     if (!kdn::sequenceNumbers(c, stream, indices, count, coordinate, sobol != 0, out)) return ctxFail(ctx, KDN_E_DEVICE);
+    return KDN_OK;
+}
+
+int kdn_set_camera(kdn_ctx* ctx, const kdn_camera* camera) {
+    if (!ctx || !camera) return fail("kdn_set_camera: null argument", KDN_E_ARGUMENT);
+    kdn::Ctx& c = ctx->c;
+    if (!c.integrator_ready) {
+        c.error = "kdn_set_camera: no scene on the device";
+        return ctxFail(ctx, KDN_E_STATE);
+    }
+    c.scene.camera = *camera;
+    return KDN_OK;
+}
+
+int kdn_temporal_frame(kdn_ctx* ctx, const kdn_temporal_desc* desc, float* shown, float* denoised, kdn_temporal_stats* stats) {
+    if (!ctx || !desc) return fail("kdn_temporal_frame: null argument", KDN_E_ARGUMENT);
+    kdn::Ctx& c = ctx->c;
+    hipSetDevice(c.device);
+    if (!kdn::temporalFrame(c, *desc, shown, denoised, stats)) return ctxFail(ctx, c.firsts ? KDN_E_DEVICE : KDN_E_STATE);
+    return KDN_OK;
+}
+
+int kdn_temporal_history(kdn_ctx* ctx, kdn_denoise_pixel* pixels) {
+    if (!ctx || !pixels) return fail("kdn_temporal_history: null argument", KDN_E_ARGUMENT);
+    kdn::Ctx& c = ctx->c;
+    hipSetDevice(c.device);
+    if (!kdn::temporalHistory(c, pixels)) return ctxFail(ctx, KDN_E_STATE);
+    return KDN_OK;
+}
+
+int kdn_temporal_step(kdn_ctx* ctx, int32_t width, int32_t height, const kdn_lens* lens, const kdn_lens* before,
+                      const kdn_temporal_desc* desc, const double* parts, const kdn_surface* record, const kdn_surface* gbuffer,
+                      const kdn_denoise_pixel* before_pixels, int32_t before_width, int32_t before_height,
+                      kdn_denoise_pixel* pixels, float* shown) {
+    if (!ctx || !lens || !before || !desc || !parts || !record || !gbuffer || !pixels || !shown)
+        return fail("kdn_temporal_step: null argument", KDN_E_ARGUMENT);
+    if (width <= 0 || height <= 0 || (size_t)width * (size_t)height > ((size_t)1 << 26))
+        return fail("kdn_temporal_step: an image of 1 to 2^26 pixels", KDN_E_ARGUMENT);
+    if (before_pixels && (before_width <= 0 || before_height <= 0))
+        return fail("kdn_temporal_step: a history needs a positive size", KDN_E_ARGUMENT);
+    kdn::Ctx& c = ctx->c;
+    hipSetDevice(c.device);
+    if (!kdn::temporalStep(c, width, height, *lens, *before, *desc, parts, record, gbuffer, before_pixels, before_width, before_height,
+                           pixels, shown))
+        return ctxFail(ctx, KDN_E_DEVICE);
     return KDN_OK;
 }
 

@@ -87,6 +87,14 @@ struct Ctx {
     uint64_t stop_total = 0;        // the count pass's paths
     int stop_width = 0, stop_height = 0;  // the image allotted (0: none)
     int32_t stop_trial = 0;
+    // the frame record's first surfaces (passes with flags bit 9) and the
+    // temporal member's resident history (temporal.hip)
+    kdn_surface* firsts = nullptr;
+    int firsts_width = 0, firsts_height = 0;
+    DevBuf tm_history[2], tm_gbuffer;
+    int tm_current = 0;
+    int tm_width = 0, tm_height = 0;  // the history's image (0: none)
+    kdn_lens tm_lens{};               // the lens of the frame before
 };
 
 // Glass scattering is held off the device: renders with glass faulted the
@@ -127,8 +135,18 @@ bool stoppingAllot(Ctx& c, const kdn_stopping& s, int32_t trial, uint64_t* total
 bool stoppingSumStart(Ctx& c);
 bool readStopping(Ctx& c, float* wanted, float* spreads, int32_t* allotted);
 
-// denoise.hip: look up this device's static denoiser working memory
+// temporal.hip: the temporal member's frame and step, and a camera's lens
+kdn_lens lensOfCamera(const kdn_camera& cam, int width, int height);
+bool temporalFrame(Ctx& c, const kdn_temporal_desc& d, float* shown, float* denoised, kdn_temporal_stats* stats);
+bool temporalHistory(Ctx& c, kdn_denoise_pixel* pixels);
+bool temporalStep(Ctx& c, int width, int height, const kdn_lens& lens, const kdn_lens& before, const kdn_temporal_desc& d,
+                  const double* parts, const kdn_surface* record, const kdn_surface* gbuffer, const kdn_denoise_pixel* beforePixels,
+                  int beforeWidth, int beforeHeight, kdn_denoise_pixel* pixels, float* shown);
+
+// denoise.hip: look up this device's static denoiser working memory; the
+// a-trous passes over a history on the device
 bool denoiseStaticScratch(Ctx& c);
+bool denoiseDevice(Ctx& c, const kdn_denoise_frame& f, const kdn_denoise_pixel* pixels, float* rgb, hipStream_t st);
 
 // hguide.hip: the plain guide of every pixel of a width x height image
 bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats, DevBuf& out, DevBuf& work, size_t* deepOffset);

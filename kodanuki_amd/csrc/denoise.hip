@@ -149,13 +149,6 @@ bool denoiseFields(Ctx& c, const kdn_denoise_frame& f, const kdn_denoise_pixel* 
     return true;
 }
 
-// The filter with its fields in the context's scratch (grown on demand).
-bool denoiseDevice(Ctx& c, const kdn_denoise_frame& f, const kdn_denoise_pixel* pixels, float* rgb, hipStream_t st) {
-    const size_t n = (size_t)f.width * f.height;
-    if (!ensure(c.scratch[4], n * kDenoiseFields * sizeof(float) + 256, c.error, false)) return false;
-    return denoiseFields(c, f, pixels, rgb, st, (float*)c.scratch[4].ptr);
-}
-
 bool checkFrame(Ctx& c, const kdn_denoise_frame* f) {
     if (!f || f->width <= 0 || f->height <= 0 || f->width > 16384 || f->height > 16384) {
         c.error = "kdn_denoise: frame size out of 1..16384";
@@ -165,6 +158,13 @@ bool checkFrame(Ctx& c, const kdn_denoise_frame* f) {
 }
 
 }  // namespace
+
+// The filter with its fields in the context's scratch (grown on demand).
+bool denoiseDevice(Ctx& c, const kdn_denoise_frame& f, const kdn_denoise_pixel* pixels, float* rgb, hipStream_t st) {
+    const size_t n = (size_t)f.width * f.height;
+    if (!ensure(c.scratch[4], n * kDenoiseFields * sizeof(float) + 256, c.error, false)) return false;
+    return denoiseFields(c, f, pixels, rgb, st, (float*)c.scratch[4].ptr);
+}
 
 // The device address of this GPU's static working memory, looked up once
 // per context with its device current (kdn_open calls it).

@@ -542,6 +542,22 @@ bool renderPass(Ctx& c, const kdn_pass_desc& d, kdn_pass_stats* stats) {
     }
     std::vector<JobDev> jobs{jobOf(c, d.width, d.height, d.seed, d.surfaces, d.min_depth)};
     jobs[0].sobol = (d.flags & 128) ? 1u : 0u;
+    if (d.flags & 512) {
+        // the frame record: each pixel's first surface of its last sample
+        if (counted || d.spp == 0) {
+            c.error = "kdn_render_pass: the frame record (bit 9) needs a plain pass of at least one sample";
+            return false;
+        }
+        if (!c.firsts || c.firsts_width != d.width || c.firsts_height != d.height) {
+            if (c.firsts) KDN_HIP(hipFree(c.firsts));
+            c.firsts = nullptr;
+            KDN_HIP(hipMalloc(&c.firsts, px * sizeof(kdn_surface)));
+            c.firsts_width = d.width;
+            c.firsts_height = d.height;
+        }
+        jobs[0].firsts = c.firsts;
+        jobs[0].record_sample = d.first_sample + d.spp - 1;
+    }
     if (d.flags & 32) {
         // the plain guide, of the image the guides were made for
         if ((d.flags & 16) || c.hguide_width != d.width || c.hguide_height != d.height) {

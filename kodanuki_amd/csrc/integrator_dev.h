@@ -145,6 +145,35 @@ KDN_D float number(bool sobol, uint64_t stream, uint32_t sample, int event, int 
     return (float)sequence_number(sobol, stream, sample, dimension(event, draw));
 }
 
+// ---- what a camera ray met ---------------------------------------------------------
+// The Surface a frame records of a camera ray (core/batch.h:244-279):
+// distance along the unit ray, the unit normal facing the ray, the leaf and
+// the albedo there (glass white; a miss infinitely far, no leaf, zero
+// normal and albedo). The G-buffer and the integrator's frame record both
+// take it from here.
+KDN_D kdn_surface surface_met(const kdn_hit& h, const kdn_ray& r, const uint32_t* leafMaterial, const MaterialDev* materials,
+                              uint32_t nMaterials) {
+    kdn_surface s;
+    if (h.kind != KDN_HIT_SURFACE) {
+        s.distance = kInf;
+        s.leaf = KDN_NO_LEAF;
+        for (int c = 0; c < 3; ++c) s.normal[c] = s.albedo[c] = 0;
+        return s;
+    }
+    s.distance = h.distance;
+    s.leaf = h.leaf;
+    double nv[3] = {h.normal[0], h.normal[1], h.normal[2]};
+    if (nv[0] * r.direction[0] + nv[1] * r.direction[1] + nv[2] * r.direction[2] > 0) {
+        for (int c = 0; c < 3; ++c) nv[c] = -nv[c];
+    }
+    const double nl = __builtin_sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+    for (int c = 0; c < 3; ++c) s.normal[c] = nl > 0 ? (float)(nv[c] / nl) : 0.0f;
+    const uint32_t mi = h.leaf != KDN_NO_LEAF ? leafMaterial[h.leaf] : nMaterials - 1;
+    const MaterialDev& m = materials[mi < nMaterials ? mi : nMaterials - 1];
+    for (int c = 0; c < 3; ++c) s.albedo[c] = m.kind == KDN_GLASS ? 1.0f : m.albedo[c];
+    return s;
+}
+
 // ---- wavelengths (colour.cpp:236-257, spectrum.cpp) ----------------------------
 struct Lanes {
     float nm[kLanes];

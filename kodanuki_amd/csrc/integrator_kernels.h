@@ -43,6 +43,9 @@ struct JobDev {
     uint64_t seed_bits;
     uint32_t surfaces, min_depth;
     uint32_t sobol;        // the first 28 coordinates off the Sobol sequence (0: all hashed)
+    // each pixel's first surface of sample record_sample (null: none kept)
+    kdn_surface* firsts;
+    uint32_t record_sample;
     float eye[3], forward[3], right[3], up[3];
     float reach, aspect;
     float sky_zenith[3], sky_horizon[3], sky_ground[3];
@@ -451,6 +454,9 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
     const bool exactBefore = (meta & FlagExact) != 0;
     const float pdfBefore = s.pdf[i];
     const bool training = !Plain && P.recs != nullptr && i % P.train_stride == 0;
+    // the frame record's first surface of the pixel's last sample
+    if (depth == 0 && J.firsts && sample == J.record_sample && s.pixel[i] != 0xFFFFFFFFu)
+        J.firsts[s.pixel[i]] = surface_met(hit, ray, J.leaf_material, J.materials, J.n_materials);
 
     // lamps met along the ray (metAlong, trace.cpp:352-374), MIS against
     // aiming: quads and spheres before the hit, suns where the ray escapes

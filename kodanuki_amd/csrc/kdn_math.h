@@ -19,6 +19,12 @@
 #include "kodanuki.h"
 
 #define KDN_D __device__ __forceinline__
+// Marks a kernel that synchronises its block (__syncthreads); the device
+// build needs no mark (the CPU build of the library runs such kernels'
+// blocks with their threads at once).
+#ifndef KDN_SYNCS_BLOCK
+#define KDN_SYNCS_BLOCK(kernel)
+#endif
 
 namespace kdn {
 

@@ -9,13 +9,18 @@
 // algorithm.
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
+#include "core/camera.h"
+#include "core/shown.h"
 #include "core/stopping.h"
 #include "math/colour.h"
 #include "math/random.h"
 #include "math/spectrum.h"
+#include "plugins/reconstruction/atrous/atrous.h"
+#include "plugins/reconstruction/temporal/temporal.h"
 #include "plugins/sampler/sobol/sobol.h"
 #include "plugins/transport/paths/material/energy.h"
 #include "shared/image.h"
@@ -77,7 +82,159 @@ Estimate unpacked(const double* in) {
 
 }  // namespace
 
+// Plain C views of a lens, a first surface and a history pixel.
+struct kr_lens {
+    double eye[3], forward[3], right[3], up[3];
+    double reach, aspect;
+    int32_t width, height;
+};
+struct kr_surface {
+    float distance;
+    float normal[3];
+    uint32_t leaf;
+    float albedo[3];
+};
+struct kr_pixel {
+    double colour[3];
+    double moment;
+    float length;
+    float distance;
+    double normal[3];
+    uint32_t leaf;
+    uint32_t pad;
+    double albedo[3];
+};
+
+namespace {
+
+math::Vector vectorOf(const double v[3]) { return {v[0], v[1], v[2]}; }
+
+Lens lensOf(const kr_lens& l) {
+    Lens lens{};
+    lens.eye = vectorOf(l.eye);
+    lens.forward = vectorOf(l.forward);
+    lens.right = vectorOf(l.right);
+    lens.up = vectorOf(l.up);
+    lens.reach = l.reach;
+    lens.aspect = l.aspect;
+    lens.aperture = 0;
+    lens.focus = 1;
+    lens.width = l.width;
+    lens.height = l.height;
+    return lens;
+}
+
+Surface surfaceOf(const kr_surface& s) {
+    Surface out{};
+    out.distance = s.distance;
+    out.normal = {s.normal[0], s.normal[1], s.normal[2]};
+    out.leaf = s.leaf;
+    out.material = s.leaf == noLeaf ? noMaterial : 0;
+    out.albedo = {s.albedo[0], s.albedo[1], s.albedo[2]};
+    out.roughness = 1;
+    out.onward = std::numeric_limits<float>::infinity();
+    return out;
+}
+
+TemporalPixel pixelOf(const kr_pixel& p) {
+    TemporalPixel t{};
+    t.colour = {p.colour[0], p.colour[1], p.colour[2]};
+    t.moment = p.moment;
+    t.length = p.length;
+    t.distance = p.distance;
+    t.normal = vectorOf(p.normal);
+    t.leaf = p.leaf;
+    t.albedo = {p.albedo[0], p.albedo[1], p.albedo[2]};
+    return t;
+}
+
+kr_pixel plainOf(const TemporalPixel& t) {
+    kr_pixel p{};
+    p.colour[0] = t.colour.red;
+    p.colour[1] = t.colour.green;
+    p.colour[2] = t.colour.blue;
+    p.moment = t.moment;
+    p.length = t.length;
+    p.distance = t.distance;
+    p.normal[0] = t.normal.x;
+    p.normal[1] = t.normal.y;
+    p.normal[2] = t.normal.z;
+    p.leaf = t.leaf;
+    p.albedo[0] = t.albedo.red;
+    p.albedo[1] = t.albedo.green;
+    p.albedo[2] = t.albedo.blue;
+    return p;
+}
+
+// A frame of a viewer sequence as the temporal and à-trous members read it,
+// at factor 1 (every pixel drawn each frame, through its centre).
+struct ViewerFrame {
+    Lens lens, before;
+    FrameRecord record;
+    std::vector<PixelSum> estimates;
+    std::vector<Surface> surfaces;
+};
+
+ViewerFrame viewerFrame(int width, int height, const kr_lens* lens, const kr_lens* before, const double* parts,
+                        const kr_surface* record, const kr_surface* gbuffer) {
+    ViewerFrame f;
+    f.lens = lensOf(*lens);
+    f.before = lensOf(*before);
+    const size_t n = (size_t)width * (size_t)height;
+    f.record.width = width;
+    f.record.height = height;
+    f.record.parts.resize(n);
+    f.record.surfaces.resize(n);
+    f.surfaces.resize(gbuffer ? n : 0);
+    for (size_t k = 0; k < n; ++k) {
+        f.record.parts[k].total = {parts[3 * k], parts[3 * k + 1], parts[3 * k + 2]};
+        f.record.surfaces[k] = surfaceOf(record[k]);
+        if (gbuffer) f.surfaces[k] = surfaceOf(gbuffer[k]);
+    }
+    return f;
+}
+
+TemporalState stateOf(int width, int height, int demodulated, const kr_pixel* pixels) {
+    TemporalState s = emptyTemporal();
+    s.width = width;
+    s.height = height;
+    s.demodulated = demodulated != 0;
+    s.film = stillFilm();
+    s.pixels.resize((size_t)width * (size_t)height);
+    for (size_t k = 0; k < s.pixels.size(); ++k) s.pixels[k] = pixelOf(pixels[k]);
+    return s;
+}
+
+}  // namespace
+
 extern "C" {
+
+// temporalFrame (temporal.h:237) of one frame at factor 1: the history
+// before (before_width x before_height pixels; 0 x 0 for none), this
+// frame's record (each pixel's light and its sample's first surface) and
+// G-buffer (null: none, the history then not demodulated) -> the history
+// after and the shown image.
+void kr_temporal_frame(int width, int height, const kr_lens* lens, const kr_lens* before, int moved, int restarted, int recounted,
+                       const double* parts, const kr_surface* record, const kr_surface* gbuffer, int before_width, int before_height,
+                       int before_demodulated, const kr_pixel* before_pixels, kr_pixel* out_pixels, float* out_image) {
+    const ViewerFrame f = viewerFrame(width, height, lens, before, parts, record, gbuffer);
+    const ShownFrame frame{width, height, f.lens, f.before, stillFilm(), moved != 0, restarted != 0, recounted != 0, 0u,
+                           f.record, f.estimates, f.surfaces, 1};
+    const Temporal t = temporalFrame(stateOf(before_width, before_height, before_demodulated, before_pixels), frame);
+    for (size_t k = 0; k < t.state.pixels.size(); ++k) out_pixels[k] = plainOf(t.state.pixels[k]);
+    for (size_t k = 0; k < t.image.rgb.size(); ++k) out_image[k] = t.image.rgb[k];
+}
+
+// denoisedFrame (atrous.h:462) of a history (the temporal state after a
+// frame), with no denoiser history before it -> the shown image.
+void kr_denoised_frame(int width, int height, const kr_lens* lens, const double* parts, const kr_surface* record,
+                       const kr_surface* gbuffer, int demodulated, const kr_pixel* pixels, float* out_image) {
+    const ViewerFrame f = viewerFrame(width, height, lens, lens, parts, record, gbuffer);
+    const ShownFrame frame{width, height, f.lens, f.before, stillFilm(), false, false, false, 0u, f.record, f.estimates, f.surfaces, 1};
+    const Denoised d = denoisedFrame(emptyDenoise(), stateOf(width, height, demodulated, pixels), frame);
+    for (size_t k = 0; k < d.image.rgb.size(); ++k) out_image[k] = d.image.rgb[k];
+}
+
 
 uint64_t kr_mixed_bits(uint64_t v) { return math::mixedBits(v); }
 

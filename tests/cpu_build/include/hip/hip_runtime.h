@@ -12,6 +12,11 @@
 #pragma once
 
 #include <chrono>
+#include <condition_variable>
+#include <mutex>
+#include <set>
+#include <thread>
+#include <vector>
 #include <cstdio>
 #include <cmath>
 #include <cstdint>
@@ -175,10 +180,82 @@ inline void kdn_emu_launch(void (*kernel)(P...), dim3 grid, dim3 block, A&&... a
     }
 }
 
+// Kernels that synchronise their block (__syncthreads) are marked with
+// KDN_SYNCS_BLOCK next to their definition. Their blocks run one after
+// another, each block's threads as OS threads at once (their __shared__
+// arrays shared as on the device), meeting at a real barrier; every other
+// kernel's threads run one after another.
+inline std::mutex& kdn_emu_marks_lock() {
+    static std::mutex m;
+    return m;
+}
+inline std::set<const void*>& kdn_emu_marks() {
+    static std::set<const void*> marks;
+    return marks;
+}
+inline bool kdn_emu_mark(const void* kernel) {
+    std::lock_guard<std::mutex> g(kdn_emu_marks_lock());
+    kdn_emu_marks().insert(kernel);
+    return true;
+}
+inline bool kdn_emu_marked(const void* kernel) {
+    std::lock_guard<std::mutex> g(kdn_emu_marks_lock());
+    return kdn_emu_marks().count(kernel) != 0;
+}
+#define KDN_SYNCS_BLOCK(kernel) static const bool kdn_emu_syncs_##kernel = kdn_emu_mark((const void*)&kernel);
+
+struct KdnEmuBarrier {
+    std::mutex m;
+    std::condition_variable cv;
+    uint64_t count = 0, waiting = 0, generation = 0;
+    void wait() {
+        std::unique_lock<std::mutex> lock(m);
+        const uint64_t g = generation;
+        if (++waiting == count) {
+            waiting = 0;
+            ++generation;
+            cv.notify_all();
+        } else {
+            cv.wait(lock, [&] { return generation != g; });
+        }
+    }
+};
+inline thread_local KdnEmuBarrier* kdn_emu_barrier = nullptr;
+inline void __syncthreads() {
+    if (!kdn_emu_barrier) {
+        std::fprintf(stderr, "emulation: a kernel calls __syncthreads but is not marked KDN_SYNCS_BLOCK\n");
+        std::abort();
+    }
+    kdn_emu_barrier->wait();
+}
+
 template <class... P, class... A>
 inline void kdn_emu_run(void (*kernel)(P...), dim3 grid, dim3 block, A&&... args) {
     gridDim = grid;
     blockDim = block;
+    if (kdn_emu_marked((const void*)kernel)) {
+        const uint64_t blocks = (uint64_t)grid.x * grid.y * grid.z, threads = (uint64_t)block.x * block.y * block.z;
+        for (uint64_t b = 0; b < blocks; ++b) {
+            KdnEmuBarrier barrier;
+            barrier.count = threads;
+            const dim3 at((uint32_t)(b % grid.x), (uint32_t)(b / grid.x % grid.y), (uint32_t)(b / grid.x / grid.y));
+            std::vector<std::thread> pool;
+            pool.reserve(threads);
+            for (uint64_t t = 0; t < threads; ++t) {
+                pool.emplace_back([&, t] {
+                    gridDim = grid;
+                    blockDim = block;
+                    blockIdx = at;
+                    threadIdx = dim3((uint32_t)(t % block.x), (uint32_t)(t / block.x % block.y), (uint32_t)(t / block.x / block.y));
+                    kdn_emu_barrier = &barrier;
+                    kernel(args...);
+                    kdn_emu_barrier = nullptr;
+                });
+            }
+            for (std::thread& th : pool) th.join();
+        }
+        return;
+    }
     const uint64_t blocks = (uint64_t)grid.x * grid.y * grid.z, threads = (uint64_t)block.x * block.y * block.z;
     const bool reverse = kdn_emu_reverse();
     for (uint64_t bi = 0; bi < blocks; ++bi) {

@@ -52,6 +52,45 @@ def ptr(a):
     return a.ctypes.data_as(P)
 
 
+class Lens(C.Structure):
+    _fields_ = [("eye", C.c_double * 3), ("forward", C.c_double * 3), ("right", C.c_double * 3), ("up", C.c_double * 3),
+                ("reach", C.c_double), ("aspect", C.c_double), ("width", C.c_int32), ("height", C.c_int32)]
+
+
+def lens_struct(lens, width, height):
+    L = Lens()
+    L.eye[:], L.forward[:], L.right[:], L.up[:] = lens[0], lens[1], lens[2], lens[3]
+    L.reach, L.aspect, L.width, L.height = lens[4], lens[5], width, height
+    return L
+
+
+def temporal_vectors(L):
+    sys.path.insert(0, ROOT)
+    from oracle import temporal_ref as T
+    from tests import temporal_frames as TF
+    L.kr_temporal_frame.argtypes = [I, I, P, P, I, I, I, P, P, P, I, I, I, P, P, P]
+    frames = TF.sequence(seed=3, width=24, height=16, frames=8)
+    out = {}
+    bw = bh = bd = 0
+    before = np.zeros((1, 1), T.PIXEL_DTYPE)
+    for k, fr in enumerate(frames):
+        W, H = fr["width"], fr["height"]
+        px = np.zeros((H, W), T.PIXEL_DTYPE)
+        img = np.zeros((H, W, 3), np.float32)
+        L.kr_temporal_frame(W, H, C.byref(lens_struct(fr["lens"], W, H)), C.byref(lens_struct(fr["before_lens"], W, H)), int(fr["moved"]),
+                            int(fr["restarted"]), int(fr["recounted"]), ptr(fr["parts"]), ptr(fr["record"]), ptr(fr["gbuffer"]), bw,
+                            bh, bd, ptr(before), ptr(px), ptr(img))
+        for name in ("parts", "record", "gbuffer"):
+            out[f"temporal{k}_{name}"] = fr[name]
+        out[f"temporal{k}_lens"] = np.array([*fr["lens"][0], *fr["lens"][1], *fr["lens"][2], *fr["lens"][3], fr["lens"][4],
+                                             fr["lens"][5]])
+        out[f"temporal{k}_flags"] = np.array([fr["moved"], fr["restarted"], fr["recounted"]], np.int32)
+        out[f"temporal{k}_pixels"] = px
+        out[f"temporal{k}_image"] = img
+        before, bw, bh, bd = px, W, H, 1
+    return out
+
+
 def main():
     L = ref()
     rng = np.random.default_rng(20261204)
@@ -137,6 +176,8 @@ def main():
     os.remove(tmp + ".ppm")
     os.remove(tmp + ".pfm")
     out["image"] = img
+    # temporalFrame over a synthetic viewer sequence (tests/temporal_frames.py)
+    out.update(temporal_vectors(L))
     np.savez_compressed(OUT, **out)
     print(f"wrote {OUT}: {os.path.getsize(OUT)} bytes")
 
