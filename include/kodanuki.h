@@ -255,7 +255,8 @@ int kdn_upload_scene(kdn_ctx* ctx, const kdn_scene* scene);
  * (plugins/geometry/tree/cut.cpp:117-348), the cell decision cutCell
  * (cut.cpp:446-476) and compaction of surviving cells. Stands in for
  * Geometry buildWorld → buildTree (core/plugin.h:168, tree/build.cpp:985)
- * with the split rule "Surface and level < depth_cap".
+ * with the split rule "Surface and level < depth_cap". The depth cap is
+ * 0..11 (the march keeps a ray's ancestry of 12 levels in LDS).
  */
 typedef struct kdn_octree_stats {
     uint32_t levels;                  /* depth_cap + 1 */
@@ -717,7 +718,7 @@ typedef struct kdn_job_desc {
     uint32_t seed;
     uint32_t surfaces;       /* the most surfaces a path stands on */
     uint32_t min_depth;      /* roulette starts at this depth */
-    int32_t depth;           /* the octree's depth cap */
+    int32_t depth;           /* the octree's depth cap, 1..11 */
     uint32_t flags;          /* bit 0: the plain guide (a survey of the job's
                                 image at admission; its guides are the job's
                                 memory, 736 B a pixel) */

@@ -203,7 +203,7 @@ uint64_t residentBytes(const Ctx& c) {
     uint64_t bytes = (uint64_t)s.n_steps * sizeof(kdn_step) + (uint64_t)s.n_objects * sizeof(kdn_object) +
                      (uint64_t)s.n_materials * sizeof(kdn_material) + (uint64_t)s.n_lights * sizeof(kdn_light) +
                      (uint64_t)s.n_leaves * 4;
-    bytes += c.tree.nodes.bytes + c.tree.entries.bytes + c.tree.pool.bytes + c.tree.spool.bytes;
+    bytes += c.tree.nodes.bytes + c.tree.entries.bytes + c.tree.pool.bytes + c.tree.spool.bytes + c.tree.mpool.bytes;
     for (const DevBuf& b : c.scratch) bytes += b.bytes;
     bytes += c.paths_bytes + c.round_bytes + c.hguides.bytes + c.sort_tmp.bytes;
     if (c.accum) bytes += (uint64_t)c.width * c.height * (6 * sizeof(double) + sizeof(uint32_t));
@@ -225,6 +225,7 @@ void releaseAll(Ctx& c) {
     freeBuf(c.tree.entries);
     freeBuf(c.tree.pool);
     freeBuf(c.tree.spool);
+    freeBuf(c.tree.mpool);
     if (c.tree.level_first_dev) hipFree(c.tree.level_first_dev);
     c.tree = OctreeDev{};
     for (DevBuf& b : c.scratch) freeBuf(b);
@@ -508,7 +509,7 @@ int kdn_upload_scene(kdn_ctx* ctx, const kdn_scene* s) {
 
 int kdn_build_octree(kdn_ctx* ctx, const float* lower3, const float* size3, int depth_cap, kdn_octree_stats* stats) {
     if (!ctx || !lower3 || !size3) return fail("kdn_build_octree: null argument", KDN_E_ARGUMENT);
-    if (depth_cap < 0 || depth_cap > 14) return fail("kdn_build_octree: depth cap out of 0..14", KDN_E_ARGUMENT);
+    if (depth_cap < 0 || depth_cap > kdn::kMaxDepth) return fail("kdn_build_octree: depth cap out of 0..11", KDN_E_ARGUMENT);
     // each side a power of two and each corner on the finest grid, so that
     // every cell box is exact in FP32
     for (int k = 0; k < 3; ++k) {

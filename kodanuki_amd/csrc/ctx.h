@@ -43,7 +43,8 @@ struct OctreeDev {
     DevBuf nodes;        // NodeDev, levels one after another
     DevBuf entries;      // EntryDev
     DevBuf pool;         // kdn_step programs with a header step each
-    DevBuf spool;        // the same programs on slot registers (the march's)
+    DevBuf spool;        // the same programs on slot registers (the landing's)
+    DevBuf mpool;        // the same with their literal steps folded away (the march's)
     uint64_t n_nodes = 0, n_entries = 0, n_pool = 0;
     uint32_t* level_first_dev = nullptr;  // device copy of level_first
 };

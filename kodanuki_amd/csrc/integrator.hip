@@ -35,7 +35,7 @@ __global__ void k_generate(PassParams P) {
 }
 
 #define KDN_LANE_REGS                                              \
-    __shared__ float slots[kSlots * kTraceBlock];                  \
+    __shared__ float slots[kLaneWords * kTraceBlock];              \
     __shared__ uint32_t paths[kPathLevels * kTraceBlock];          \
     const LaneRegs lr{SlotRegs{slots + threadIdx.x, kTraceBlock}, PathRegs{paths + threadIdx.x, kTraceBlock}}
 

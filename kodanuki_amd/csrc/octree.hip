@@ -315,11 +315,13 @@ __global__ void k_object_sizes(uint32_t n, const kdn_object* objects, uint32_t* 
 // step's value takes the lowest free slot once the operands whose last use
 // it is have given theirs back, so a program needs as many slots as it
 // keeps values live at once. The result lives to the end. Writes the
-// program as 8-byte slot steps at the same offsets of `spool`. A program
-// that needs more than kSlots raises *overflow. Entries share programs (a cut that changes nothing keeps its
+// program as 8-byte slot steps at the same offsets of `spool`, and again,
+// with its literal steps folded into the steps that read them, at the same
+// offsets of `mpool` (the march's programs). A program that needs more than
+// kSlots raises *overflow. Entries share programs (a cut that changes nothing keeps its
 // parent's), so the first lane to claim a program's bit in `claimed`
 // allocates it and the others leave it alone.
-__global__ void k_slot_programs(const EntryDev* entries, uint32_t nEntries, const kdn_step* pool, SlotStep* spool,
+__global__ void k_slot_programs(const EntryDev* entries, uint32_t nEntries, const kdn_step* pool, SlotStep* spool, SlotStep* mpool,
                                 uint16_t* lastUse, uint8_t* slotOf, uint32_t* claimed, uint32_t* overflow) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nEntries) return;
@@ -361,6 +363,66 @@ __global__ void k_slot_programs(const EntryDev* entries, uint32_t nEntries, cons
         if (last[i] == kUnread) freeSlots |= 1u << d;  // a value nothing reads
     }
     spool[at] = SlotStep{L | ((uint32_t)(L > 0 ? slot[h.c] : 0) << 24), h.value};
+
+    // The march's program: a coordinate or normal step is read from the
+    // lane's point words, a constant from the reading step's value (the
+    // first constant operand of a step; a second one stays in its slot), so
+    // only the steps that compute remain, on the same slots. A literal step
+    // is kept (as kMoveOp) where it is the result or some reader keeps it in
+    // its slot. (last[] is free again: it marks the kept literals.)
+    for (uint32_t i = 0; i < L; ++i) last[i] = 0;
+    for (uint32_t i = 0; i < L; ++i) {
+        const kdn_step st = prog[i];
+        const int n = operand_count(st.op);
+        bool immTaken = false;
+        for (int j = 0; j < n; ++j) {
+            const uint32_t src = j == 0 ? st.a : (j == 1 ? st.b : st.c);
+            const uint8_t sop = prog[src].op;
+            if (!is_literal(sop)) continue;
+            if (immTaken) last[src] = 1;
+            immTaken = true;
+        }
+    }
+    SlotStep* mout = mpool + at + 1;
+    uint32_t M = 0;
+    for (uint32_t i = 0; i < L; ++i) {
+        const kdn_step st = prog[i];
+        const bool kept = i == (uint32_t)h.c || last[i] != 0;
+        uint32_t code = 0;
+        float value = 0.0f;
+        if (is_literal(st.op) || point_word_of(st.op) != 0) {
+            if (!kept) continue;
+            // a kept literal: a copy of its own operand word or constant
+            code = kMoveOp;
+            const uint32_t w = point_word_of(st.op);
+            if (w) {
+                code |= w << 10;
+            } else {
+                code |= kImmA;
+                value = st.value;
+            }
+        } else {
+            code = st.op;
+            const int n = operand_count(st.op);
+            bool immTaken = false;
+            for (int j = 0; j < n; ++j) {
+                const uint32_t src = j == 0 ? st.a : (j == 1 ? st.b : st.c);
+                const uint8_t sop = prog[src].op;
+                uint32_t w = point_word_of(sop);
+                if (w == 0 && is_literal(sop) && !immTaken) {
+                    immTaken = true;
+                    code |= kImmA << j;
+                    value = prog[src].value;
+                } else if (w == 0) {
+                    w = slot[src];
+                }
+                code |= w << (10 + 5 * j);
+            }
+        }
+        code |= (uint32_t)slot[i] << 6;
+        mout[M++] = SlotStep{code, value};
+    }
+    mpool[at] = SlotStep{M | ((uint32_t)(L > 0 ? slot[h.c] : 0) << 24), h.value};
 }
 
 static inline uint32_t blocksFor(uint64_t n, uint32_t t = 256) { return (uint32_t)((n + t - 1) / t); }
@@ -536,6 +598,7 @@ bool buildOctree(Ctx& c, const float lower[3], const float size[3], int depth, k
     {
         // the march's programs on slot registers
         if (!growKeep(c, t.spool, (t.n_pool + kPoolPad) * sizeof(SlotStep), 0)) return false;
+        if (!growKeep(c, t.mpool, (t.n_pool + kPoolPad) * sizeof(SlotStep), 0)) return false;
         const size_t claimWords = (t.n_pool + 31) / 32;
         if (!ensure(bufS, t.n_pool * 3 + 256, c.error, false)) return false;
         if (!ensure(bufC, claimWords * 4 + 64, c.error, false)) return false;
@@ -545,8 +608,8 @@ bool buildOctree(Ctx& c, const float lower[3], const float size[3], int depth, k
         uint32_t* claimed = overflow + 16;
         KDN_HIP(hipMemsetAsync(bufC.ptr, 0, claimWords * 4 + 64, s));
         hipLaunchKernelGGL(k_slot_programs, dim3(blocksFor(t.n_entries)), dim3(256), 0, s, (const EntryDev*)t.entries.ptr,
-                           (uint32_t)t.n_entries, (const kdn_step*)t.pool.ptr, (SlotStep*)t.spool.ptr, lastUse, slotOf, claimed,
-                           overflow);
+                           (uint32_t)t.n_entries, (const kdn_step*)t.pool.ptr, (SlotStep*)t.spool.ptr, (SlotStep*)t.mpool.ptr, lastUse,
+                           slotOf, claimed, overflow);
         uint32_t longest = 0;
         KDN_HIP(hipMemcpyAsync(&longest, overflow, 4, hipMemcpyDeviceToHost, s));
         KDN_HIP(hipStreamSynchronize(s));

@@ -282,8 +282,8 @@ int kdn_serve_open(kdn_ctx* ctx, uint64_t budget_bytes, uint32_t max_paths, kdn_
 int kdn_serve_submit(kdn_server* s, const kdn_job_desc* job, uint64_t* id) {
     if (!s || !job || !job->scene || !id) return failed(s, "kdn_serve_submit: null argument", KDN_E_ARGUMENT);
     if (job->width <= 0 || job->height <= 0 || job->width > 16384 || job->height > 16384 || job->spp == 0 || job->surfaces == 0 ||
-        job->depth < 1 || job->depth > 14 || (job->flags & ~1u) != 0) {
-        return failed(s, "kdn_serve_submit: image size out of 1..16384, no samples, no surfaces, depth cap out of 1..14 or "
+        job->depth < 1 || job->depth > kdn::kMaxDepth || (job->flags & ~1u) != 0) {
+        return failed(s, "kdn_serve_submit: image size out of 1..16384, no samples, no surfaces, depth cap out of 1..11 or "
                          "unknown flags", KDN_E_ARGUMENT);
     }
     if ((uint64_t)kdn::planeOf(job->width, job->height) > s->maxPaths) {

@@ -7,6 +7,7 @@
 #pragma once
 
 #include "kdn_math.h"
+#include "layout.h"
 
 namespace kdn {
 
@@ -174,6 +175,17 @@ KDN_D int operand_count(uint8_t op) {
     }
 }
 KDN_D bool is_literal(uint8_t op) { return op == KDN_CONSTANT || op == KDN_ARGUMENT; }
+// The word of the march's lane register file a coordinate or normal step
+// reads (the point's x, y, z; a normal is zero at a point), 0 for others.
+KDN_D uint32_t point_word_of(uint8_t op) {
+    switch (op) {
+        case KDN_POINT_X: return kPointWord;
+        case KDN_POINT_Y: return kPointWord + 1;
+        case KDN_POINT_Z: return kPointWord + 2;
+        case KDN_NORMAL_X: case KDN_NORMAL_Y: case KDN_NORMAL_Z: return kPointWord + 3;
+        default: return 0;
+    }
+}
 
 // FP32 operations one point evaluation of a step takes: the algorithmic
 // work the march's roofline counts (loads of constants and coordinates

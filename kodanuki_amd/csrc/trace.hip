@@ -7,7 +7,7 @@
 namespace kdn {
 
 __global__ void __launch_bounds__(kTraceBlock, kTraceWaves) k_extend(TreeView tv, const kdn_ray* rays, uint32_t n, kdn_hit* hits) {
-    __shared__ float slots[kSlots * kTraceBlock];
+    __shared__ float slots[kLaneWords * kTraceBlock];
     __shared__ uint32_t paths[kPathLevels * kTraceBlock];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -23,6 +23,7 @@ TreeView treeView(const Ctx& c) {
     tv.entries = (const EntryDev*)c.tree.entries.ptr;
     tv.pool = (const kdn_step*)c.tree.pool.ptr;
     tv.spool = (const SlotStep*)c.tree.spool.ptr;
+    tv.mpool = (const SlotStep*)c.tree.mpool.ptr;
     tv.leafMaterial = c.scene.leaf_material;
     for (int k = 0; k < 3; ++k) {
         tv.lower[k] = c.tree.lower[k];

@@ -35,6 +35,7 @@ struct TreeView {
     const EntryDev* entries;
     const kdn_step* pool;
     const SlotStep* spool;  // the pool's programs on slot registers, same offsets
+    const SlotStep* mpool;  // the same with their literal steps folded away (the march's)
     const uint32_t* leafMaterial;
     float lower[3];
     float size[3];
@@ -48,7 +49,7 @@ struct SlotRegs {
     float* s;
     uint32_t stride;
     KDN_D float& operator[](uint32_t i) const {
-        KDN_CHECK(i < (uint32_t)kSlots);
+        KDN_CHECK(i < (uint32_t)kLaneWords);
         return s[i * stride];
     }
 };
@@ -71,43 +72,77 @@ struct LaneRegs {
     PathRegs path;
 };
 
-// A program on slot registers (the build's allocation, same offsets as the
-// SSA pool). Its steps are fetched kSlotFetch at a time, the header with
-// the first of them: the loads of a batch are in flight together, so a
-// lane waits once per batch instead of once per step (the pools keep
-// kPoolPad steps readable past their end for the last batch).
+// One step of a march program (its operands read from the lane's register
+// file or its constant): the arithmetic most steps are (sum, difference,
+// product, least, most, negation, magnitude, positive square, copy)
+// computed without a branch and chosen by opcode, so that a wave's lanes,
+// each in its own program, run the same instructions; square roots, smooth
+// minima and the rare operations under branches a wave skips where none of
+// its lanes takes them. The same operations in the same order as
+// point_step, so the same bits.
+KDN_D float march_step(uint32_t code, float value, const SlotRegs& r) {
+    const uint32_t op = code & 63u;
+    float a = r[(code >> 10) & 31u], b = r[(code >> 15) & 31u], c = r[(code >> 20) & 31u];
+    a = (code & kImmA) ? value : a;
+    b = (code & kImmB) ? value : b;
+    c = (code & kImmC) ? value : c;
+    const float sum = a + b, difference = a - b, product = a * b;
+    const float pos = fmax_(a, 0.0f);
+    float v = op == KDN_SUBTRACT ? difference : sum;
+    v = op == KDN_MULTIPLY ? product : v;
+    v = op == KDN_MINIMUM ? fmin_(a, b) : v;
+    v = op == KDN_MAXIMUM ? fmax_(a, b) : v;
+    v = op == KDN_NEGATE ? -a : v;
+    v = op == KDN_ABSOLUTE ? __builtin_fabsf(a) : v;
+    v = op == KDN_POSITIVE_SQUARE ? pos * pos : v;
+    v = op == kMoveOp ? a : v;
+    if (op == KDN_SQUARE_ROOT) {
+        v = __builtin_sqrtf(a);
+    } else if (op == KDN_SMOOTH_MINIMUM) {
+        v = p_smin(a, b, c);
+    } else if (op == KDN_DIVIDE || op == KDN_FLOOR || op == KDN_SINE || op == KDN_COSINE || op == KDN_SIGN ||
+               op == KDN_ARC_TANGENT || op == KDN_POWER || op == KDN_SELECT || (op >= KDN_LESS && op <= KDN_NOT)) {
+        const float w[3] = {a, b, c};
+        kdn_step st;
+        st.op = (uint8_t)op;
+        st.flags = 0;
+        st.a = 0;
+        st.b = 1;
+        st.c = 2;
+        st.value = value;
+        st.leaf = 0;
+        v = point_step(st, w, 0.0f, 0.0f, 0.0f);
+    }
+    return v;
+}
+
+// A march program (the build's slot allocation with its literal steps
+// folded away, same offsets as the SSA pool) at a point: the point goes
+// into the lane's point words, then the steps run in batches of kSlotFetch
+// fetched together (the loads of a batch are in flight at once, so a lane
+// waits once per batch; the pools keep kPoolPad steps readable past their
+// end for the last batch), each batch's steps unrolled.
 KDN_D float eval_slots(const TreeView& tv, uint32_t at, const SlotRegs& r, float px, float py, float pz) {
-    const SlotStep* p = tv.spool + at;
+    const SlotStep* p = tv.mpool + at;
+    const SlotStep head = p[0];
     SlotStep q[kSlotFetch];
 #pragma unroll
-    for (int k = 0; k < kSlotFetch; ++k) q[k] = p[k];
-    p += kSlotFetch;
-    const SlotStep head = q[0];
-#pragma unroll
-    for (int k = 0; k + 1 < kSlotFetch; ++k) q[k] = q[k + 1];
-    uint32_t left = kSlotFetch - 1;
+    for (int k = 0; k < kSlotFetch; ++k) q[k] = p[1 + k];
+    r[kPointWord] = px;
+    r[kPointWord + 1] = py;
+    r[kPointWord + 2] = pz;
+    r[kPointWord + 3] = 0.0f;
     if (tv.work) *tv.work += head.value;
     const uint32_t L = head.code & 0xFFFFFFu;
-    for (uint32_t i = 0; i < L; ++i) {
-        if (left == 0) {
+    for (uint32_t base = 0; base < L; base += kSlotFetch) {
+        if (base > 0) {
 #pragma unroll
-            for (int k = 0; k < kSlotFetch; ++k) q[k] = p[k];
-            p += kSlotFetch;
-            left = kSlotFetch;
+            for (int k = 0; k < kSlotFetch; ++k) q[k] = p[1 + base + k];
         }
-        const uint32_t code = q[0].code;
-        kdn_step st;
-        st.op = (uint8_t)(code & 63u);
-        st.flags = 0;
-        st.a = (uint16_t)((code >> 10) & 15u);
-        st.b = (uint16_t)((code >> 14) & 15u);
-        st.c = (uint16_t)((code >> 18) & 15u);
-        st.value = q[0].value;
-        st.leaf = 0;
-        r[(code >> 6) & 15u] = point_step(st, r, px, py, pz);
 #pragma unroll
-        for (int k = 0; k + 1 < kSlotFetch; ++k) q[k] = q[k + 1];
-        --left;
+        for (int k = 0; k < kSlotFetch; ++k) {
+            if (base + k < L) r[(q[k].code >> 6) & 15u] = march_step(q[k].code, q[k].value, r);
+        }
     }
     return r[(head.code >> 24) & 15u];
 }
