@@ -28,11 +28,42 @@ __global__ void k_dn_estimate(DenoiseBuffers B, int src) {
     estimate_pixel(B, src, (int)(at % B.width), (int)(at / B.width));
 }
 
-__global__ void k_dn_filter(DenoiseBuffers B, int src, int step) {
-    const uint32_t at = blockIdx.x * blockDim.x + threadIdx.x;
-    if (at >= (uint32_t)(B.width * B.height)) return;
-    filter_pixel(B, src, step, (int)(at % B.width), (int)(at / B.width));
+// A filter pass over LDS-staged tiles. At step s the taps of pixel
+// (a + s u, b + s v) are the pixels (a + s (u + i), b + s (v + j)), |i|, |j| <=
+// 2: the lattice of its residue (a, b) mod s. A block takes 16x16 lattice
+// points of one residue and stages the 20x20 lattice points their taps
+// reach (12 fields each, 19.2 KB) once, so each tap's guides and light are
+// read from HBM once a block rather than once a pixel that reads it; every
+// step's tile is the same size. Blocks: s^2 residues x tilesX x tilesY.
+constexpr int kFilterTile = 16;
+constexpr int kFilterWindow = kFilterTile + 4;
+constexpr int kTapFields = 12;
+
+__global__ void __launch_bounds__(kFilterTile * kFilterTile) k_dn_filter(DenoiseBuffers B, int src, int step, int tilesX, int tilesY) {
+    __shared__ float window[kTapFields][kFilterWindow * kFilterWindow];
+    const int tiles = tilesX * tilesY;
+    const int residue = (int)blockIdx.x / tiles, tile = (int)blockIdx.x % tiles;
+    const int ax = residue % step, ay = residue / step;
+    const int u0 = (tile % tilesX) * kFilterTile, v0 = (tile / tilesX) * kFilterTile;
+    for (int k = (int)threadIdx.x; k < kFilterWindow * kFilterWindow; k += kFilterTile * kFilterTile) {
+        const int xq = ax + step * (u0 - 2 + k % kFilterWindow), yq = ay + step * (v0 - 2 + k / kFilterWindow);
+        if (xq < 0 || xq >= B.width || yq < 0 || yq >= B.height) continue;  // outside the image: never read
+        const TapFields q = tap_of(B, src, (size_t)yq * B.width + xq);
+        const float f[kTapFields] = {q.x, q.y, q.z, q.nx, q.ny, q.nz, q.sky, q.red, q.green, q.blue, q.luminance, q.variance};
+        for (int n = 0; n < kTapFields; ++n) window[n][k] = f[n];
+    }
+    __syncthreads();
+    const int tx = (int)threadIdx.x % kFilterTile, ty = (int)threadIdx.x / kFilterTile;
+    const int xpix = ax + step * (u0 + tx), ypix = ay + step * (v0 + ty);
+    if (xpix >= B.width || ypix >= B.height) return;
+    filter_pixel_from(B, src, step, xpix, ypix, [&](int i, int j) {
+        const int k = (ty + 2 + j) * kFilterWindow + (tx + 2 + i);
+        KDN_CHECK(k >= 0 && k < kFilterWindow * kFilterWindow);
+        return TapFields{window[0][k], window[1][k], window[2][k], window[3][k],  window[4][k],  window[5][k],
+                         window[6][k], window[7][k], window[8][k], window[9][k], window[10][k], window[11][k]};
+    });
 }
+KDN_SYNCS_BLOCK(k_dn_filter)
 
 __global__ void k_dn_remodulate(DenoiseBuffers B, int src, const kdn_denoise_pixel* pixels, uint32_t demodulated, float* rgb) {
     const uint32_t at = blockIdx.x * blockDim.x + threadIdx.x;
@@ -141,7 +172,11 @@ bool denoiseFields(Ctx& c, const kdn_denoise_frame& f, const kdn_denoise_pixel* 
     hipLaunchKernelGGL(k_dn_estimate, grid, block, 0, st, B, 0);
     int src = 1;
     for (int k = 0; k < kDenoisePasses; ++k) {
-        hipLaunchKernelGGL(k_dn_filter, grid, block, 0, st, B, src, 1 << k);
+        const int step = 1 << k;
+        const int tilesX = ((f.width + step - 1) / step + kFilterTile - 1) / kFilterTile;
+        const int tilesY = ((f.height + step - 1) / step + kFilterTile - 1) / kFilterTile;
+        hipLaunchKernelGGL(k_dn_filter, dim3((uint32_t)(step * step * tilesX * tilesY)), dim3(kFilterTile * kFilterTile), 0, st, B, src,
+                           step, tilesX, tilesY);
         src ^= 1;
     }
     hipLaunchKernelGGL(k_dn_remodulate, grid, block, 0, st, B, src, pixels, f.demodulated, rgb);

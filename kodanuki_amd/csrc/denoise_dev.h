@@ -114,18 +114,33 @@ KDN_D void prepare_pixel(const DenoiseBuffers& B, const DenoiseLensDev& L, const
     B.variance[0][at] = variance;
 }
 
+// What the filter reads of a tap: its guides and its light.
+struct TapFields {
+    float x, y, z, nx, ny, nz, sky, red, green, blue, luminance, variance;
+};
+KDN_D TapFields tap_of(const DenoiseBuffers& B, int src, size_t q) {
+    return TapFields{B.x[q],   B.y[q],      B.z[q],        B.nx[q],        B.ny[q],        B.nz[q],
+                     B.sky[q], B.red[src][q], B.green[src][q], B.blue[src][q], B.luminance[src][q], B.variance[src][q]};
+}
+
 // The weight of tap q for pixel p (without the luminance term): the
 // normals' cosine to the 128th times exp(-plane - apart).
-KDN_D float facing_weight(const DenoiseBuffers& B, size_t p, size_t q) {
-    const float facing = B.nx[p] * B.nx[q] + B.ny[p] * B.ny[q] + B.nz[p] * B.nz[q] + B.sky[p] * B.sky[q];
+KDN_D float facing_of(float pnx, float pny, float pnz, float psky, const TapFields& q) {
+    const float facing = pnx * q.nx + pny * q.ny + pnz * q.nz + psky * q.sky;
     float power = facing > kFacingFloor ? facing : 0.0f;
     for (int n = 0; n < 7; ++n) power *= power;
     return power;
 }
+KDN_D float facing_weight(const DenoiseBuffers& B, size_t p, size_t q) {
+    return facing_of(B.nx[p], B.ny[p], B.nz[p], B.sky[p], tap_of(B, 0, q));
+}
 
+KDN_D float plane_of(float pnx, float pny, float pnz, float poffset, const TapFields& q, float inversePlane) {
+    const float along = q.x * pnx + q.y * pny + q.z * pnz;
+    return __builtin_fabsf(along - poffset) * inversePlane;
+}
 KDN_D float plane_exponent(const DenoiseBuffers& B, size_t p, size_t q, float inversePlane) {
-    const float along = B.x[q] * B.nx[p] + B.y[q] * B.ny[p] + B.z[q] * B.nz[p];
-    return __builtin_fabsf(along - B.offset[p]) * inversePlane;
+    return plane_of(B.nx[p], B.ny[p], B.nz[p], B.offset[p], tap_of(B, 0, q), inversePlane);
 }
 
 // The variance of a young pixel from the 7x7 pixels around it, weighed by
@@ -171,8 +186,11 @@ KDN_D void estimate_pixel(const DenoiseBuffers& B, int src, int xpix, int ypix) 
 }
 
 // One pass of the 5x5 B3 spline at a step (filteredChunk, atrous.cpp:
-// 178-300, 390-420): reads light src, writes light src ^ 1.
-KDN_D void filter_pixel(const DenoiseBuffers& B, int src, int step, int xpix, int ypix) {
+// 178-300, 390-420): reads light src, writes light src ^ 1. taps(i, j) gives
+// the fields of the tap i steps across and j steps down (asked for taps in
+// the image only): from the buffers, or from a block's staged tile.
+template <class Taps>
+KDN_D void filter_pixel_from(const DenoiseBuffers& B, int src, int step, int xpix, int ypix, const Taps& taps) {
     const size_t p = (size_t)ypix * B.width + xpix;
     const int dst = src ^ 1;
     // the deviation through a 3x3 gaussian of the variance
@@ -192,6 +210,7 @@ KDN_D void filter_pixel(const DenoiseBuffers& B, int src, int step, int xpix, in
     const float deviation = __builtin_sqrtf(var / wsum);
     const float inverseDeviation = 1 / (kDeviationSpread * deviation + kLuminanceFloor + kLuminanceShare * lum);
     const float inversePlane = B.inverseReach[p] * (1 / (float)step);
+    const float pnx = B.nx[p], pny = B.ny[p], pnz = B.nz[p], psky = B.sky[p], poffset = B.offset[p];
     float red = 0, green = 0, blue = 0, variance = 0, weight = 0;
     for (int j = -2; j <= 2; ++j) {
         const int yq = ypix + j * step;
@@ -199,15 +218,15 @@ KDN_D void filter_pixel(const DenoiseBuffers& B, int src, int step, int xpix, in
         for (int i = -2; i <= 2; ++i) {
             const int xq = xpix + i * step;
             if (xq < 0 || xq >= B.width) continue;
-            const size_t q = (size_t)yq * B.width + xq;
+            const TapFields q = taps(i, j);
             const float kernel = spline_weight(i) * spline_weight(j);
-            const float l = B.luminance[src][q];
+            const float l = q.luminance;
             const float apart = __builtin_fabsf(l - lum) * inverseDeviation;
-            const float w = kernel * facing_weight(B, p, q) * quick_exp(-plane_exponent(B, p, q, inversePlane) - apart);
-            red += w * B.red[src][q];
-            green += w * B.green[src][q];
-            blue += w * B.blue[src][q];
-            variance += w * w * B.variance[src][q];
+            const float w = kernel * facing_of(pnx, pny, pnz, psky, q) * quick_exp(-plane_of(pnx, pny, pnz, poffset, q, inversePlane) - apart);
+            red += w * q.red;
+            green += w * q.green;
+            blue += w * q.blue;
+            variance += w * w * q.variance;
             weight += w;
         }
     }
@@ -226,6 +245,13 @@ KDN_D void filter_pixel(const DenoiseBuffers& B, int src, int step, int xpix, in
         B.luminance[dst][p] = B.luminance[src][p];
         B.variance[dst][p] = B.variance[src][p];
     }
+}
+
+// The same pass reading every tap from the buffers.
+KDN_D void filter_pixel(const DenoiseBuffers& B, int src, int step, int xpix, int ypix) {
+    filter_pixel_from(B, src, step, xpix, ypix, [&](int i, int j) {
+        return tap_of(B, src, (size_t)(ypix + j * step) * B.width + (xpix + i * step));
+    });
 }
 
 // The filtered light times the albedo, faded into the held light by

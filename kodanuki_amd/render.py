@@ -179,7 +179,7 @@ def render(scene_path, config_path, out_dir=None, depth=8, device=0, dev=None):
             dev.render_pass(width, height, stopping.samples, seed=seed, surfaces=surfaces, flags=flags | k.PASS_ESTIMATES,
                             batch=stopping.batch)
         if plan["denoise"]:
-            shown, _ = dev.denoise_accumulation(width, height)
+            shown = dev.denoise_accumulation(width, height)
         else:
             est = dev.estimates(width, height)
             taken = np.maximum(est["taken"], 1).astype(np.float64)[..., None]
