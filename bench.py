@@ -18,8 +18,9 @@ else stands in for it. Beside it:
   (an instrumented pass over the first timed step's samples) over its mean
   launch time in the timed steps (HIP events on the pass's stream), against
   the FP32 vector peak; its HBM traffic from the committed counter runs.
-- `guided`, `denoise`, `serving`: the learned and plain guides at equal
-  relMSE, the denoiser, the serving stream.
+- `guided`, `denoise`, `viewer`, `serving`: the learned and plain guides at
+  equal relMSE, the denoiser, config 4's viewer sequence on the stress
+  scene, the serving stream.
 - `cpu_baseline`: the FP64 oracle renderer on a bounded window of the same
   image, on the host's cores.
 """
@@ -247,6 +248,52 @@ def guided_leg(k, dev, label, text, depth, W, H, spp, passes, config=None, guide
     return res
 
 
+def viewer_leg(k, scenes, dev, width, height, frames, depth=10):
+    """Config 4's viewer sequence: the stress scene (4096 primitives and a
+    64x64 canopy of repeated bubbles, its glass painted matte while glass is
+    held off the device) at octree depth 10, one sample a frame for
+    `frames` frames, the eye orbiting the target by 0.2 degrees a frame
+    with a seeded jitter of a centimetre, each frame shown through the
+    temporal member and denoised by the a-trous filter on its history."""
+    import math
+
+    import numpy as np
+    scene = k.Scene(scenes.stress(0, width=width, height=height, glass=False))
+    cam, rs = scene.settings()
+    lower, size = scene.region(depth)
+    dev.upload(scene)
+    tree = dev.build_octree(lower, size, depth)
+    eye, target, up = np.array(cam.eye[:], np.float64), np.array(cam.target[:], np.float64), list(cam.up)
+    rng = np.random.default_rng(4)
+    arm = eye - target
+    render_ms = temporal_ms = 0.0
+    start = time.perf_counter()
+    for f in range(frames):
+        a = math.radians(0.2) * f
+        rot = np.array([[math.cos(a), -math.sin(a), 0], [math.sin(a), math.cos(a), 0], [0, 0, 1]])
+        dev.set_camera(list(target + rot @ arm + rng.normal(0, 0.01, 3)), list(target), up, cam.fov)
+        t0 = time.perf_counter()
+        dev.render_pass(width, height, 1, first_sample=f, seed=rs.seed, surfaces=rs.surfaces, flags=k.PASS_RECORD)
+        t1 = time.perf_counter()
+        dev.temporal_frame(moved=f > 0, restarted=f == 0, denoise=True)
+        t2 = time.perf_counter()
+        render_ms += (t1 - t0) * 1e3
+        temporal_ms += (t2 - t1) * 1e3
+    seconds = time.perf_counter() - start
+    hist = dev.temporal_history(width, height)
+    dev.set_camera(list(eye), list(target), up, cam.fov)
+    out = {"workload": f"stress scene ({scene.info.leaves} leaves: 4096 primitives and a 64x64 canopy of repeated bubbles, "
+                       f"glass painted as white matte) {width}x{height}, octree depth {depth} ({tree.entries} entries, build "
+                       f"{tree.build_ms:.0f} ms), {rs.surfaces} surfaces, {frames} frames of 1 spp, orbit 0.2 deg a frame with "
+                       f"seeded jitter, temporal member and 4 a-trous passes on every frame",
+           "frames_per_s": frames / seconds, "msamples_s": frames * width * height / seconds / 1e6,
+           "render_ms_per_frame": render_ms / frames, "temporal_and_filter_ms_per_frame": temporal_ms / frames,
+           "median_history_length": float(np.median(hist["length"]))}
+    print(f"[viewer] {frames} frames {width}x{height}: {out['frames_per_s']:.2f} frames/s, render {out['render_ms_per_frame']:.1f} ms, "
+          f"temporal + filter {out['temporal_and_filter_ms_per_frame']:.1f} ms a frame", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -265,6 +312,9 @@ def main():
     ap.add_argument("--serve-seed", type=int, default=7)
     ap.add_argument("--serve-paths", type=int, default=1 << 23, help="paths of one round")
     ap.add_argument("--serve-budget-gib", type=int, default=192)
+    ap.add_argument("--no-viewer", action="store_true", help="skip config 4's viewer sequence")
+    ap.add_argument("--viewer-frames", type=int, default=120)
+    ap.add_argument("--viewer-size", type=int, nargs=2, default=[3840, 2160])
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (the known-size kernel that scales HBM counters)")
     args = ap.parse_args()
@@ -394,6 +444,10 @@ def main():
     print(f"[denoise] filter {fl:.2f} ms, G-buffer {gb:.2f} ms ({achieved:.0f} GB/s of unique bytes)", file=sys.stderr,
           flush=True)
 
+    viewer = None
+    if not args.no_viewer:
+        viewer = viewer_leg(k, scenes, dev, args.viewer_size[0], args.viewer_size[1], args.viewer_frames)
+
     serving = serving_guided = None
     if not args.no_serve:
         # the stream at each arrival rate (jobs/s), the same jobs each time
@@ -441,6 +495,7 @@ def main():
         "roofline": roofline,
         "guided": guided,
         "denoise": denoise,
+        "viewer": viewer,
         "serving": serving,
         "serving_guided": serving_guided,
         "cpu_baseline": cpu,

@@ -78,6 +78,7 @@ class RenderSettings(C.Structure):
 
 
 CURVE_FILMIC, CURVE_CLAMPED = 0, 1
+MATTE, PLASTIC, METAL, GLASS = 0, 1, 2, 3  # kdn_material.kind
 
 
 class SceneInfo(C.Structure):

@@ -793,6 +793,10 @@ private:
             return;
         }
         const Token& head = peek();
+        if (isIdent("repeat") && isSym("(", 1)) {
+            repeat(f, parent);
+            return;
+        }
         if (isIdent("merge") || isIdent("cut") || isIdent("clip")) {
             next();
             Join j = head.text == "merge" ? JoinMerge : head.text == "cut" ? JoinCut : JoinClip;
@@ -813,6 +817,67 @@ private:
         Args a = args();
         expect(";");
         attach(parent, leafNode(head, a, f));
+    }
+
+    // repeat(count = [nx, ny, nz], step = [sx, sy, sz]) { ... } (runRepeat,
+    // syntax/evaluator.cpp:2457-2577): the block's shapes, then copies of
+    // them at every other place of the grid, place (i, j, k) offset by
+    // i sx, j sy, k sz along the frame's axes (z outermost, x innermost),
+    // all merged with radius 0. A copy is the same tree with its leaves'
+    // origins moved.
+    void repeat(const Frame& f, std::vector<uint32_t>* parent) {
+        const Token& at = next();
+        Args a = args();
+        if (!a.positional.empty() || a.named.size() != 2 || !a.named.count("count") || !a.named.count("step")) {
+            fail(at, "repeat takes count and step");
+        }
+        const std::vector<double> counts = vec(a.named.at("count"), 3, at), step = vec(a.named.at("step"), 3, at);
+        int n[3];
+        double steps[3][3];
+        for (int axis = 0; axis < 3; ++axis) {
+            if (!(counts[axis] >= 1) || counts[axis] != std::floor(counts[axis])) fail(at, "a repeat count is a whole number of one or more");
+            n[axis] = (int)counts[axis];
+            for (int row = 0; row < 3; ++row) steps[axis][row] = f.A[row][axis] * step[axis];
+        }
+        if ((double)n[0] * n[1] * n[2] > 1e6) fail(at, "a repeat of more than a million copies");
+        std::vector<uint32_t> first;
+        block(f, &first, false);
+        if (first.empty()) fail(at, "this emits no shape");
+        std::vector<uint32_t> copies = first;
+        for (int k = 0; k < n[2]; ++k) {
+            for (int j = 0; j < n[1]; ++j) {
+                for (int i = 0; i < n[0]; ++i) {
+                    if (i == 0 && j == 0 && k == 0) continue;
+                    const int place[3] = {i, j, k};
+                    double offset[3];
+                    for (int row = 0; row < 3; ++row) offset[row] = steps[0][row] * place[0] + steps[1][row] * place[1] + steps[2][row] * place[2];
+                    for (uint32_t node : first) copies.push_back(copied(node, offset));
+                }
+            }
+        }
+        if (parent == nullptr || copies.size() == 1) {
+            for (uint32_t node : copies) attach(parent, node);
+        } else {
+            attach(parent, joined(JoinMerge, 0, copies));
+        }
+    }
+
+    uint32_t copied(uint32_t index, const double offset[3]) {
+        const Node node = scene_.nodes[index];
+        if (node.join != JoinLeaf) {
+            Node copy = node;
+            copy.children.clear();
+            for (uint32_t child : node.children) copy.children.push_back(copied(child, offset));
+            scene_.nodes.push_back(copy);
+            return uint32_t(scene_.nodes.size() - 1);
+        }
+        Leaf leaf = scene_.leaves[node.leaf];
+        for (int i = 0; i < 3; ++i) leaf.origin[i] += offset[i];
+        scene_.leaves.push_back(leaf);
+        Node copy = node;
+        copy.leaf = uint32_t(scene_.leaves.size() - 1);
+        scene_.nodes.push_back(copy);
+        return uint32_t(scene_.nodes.size() - 1);
     }
 
     uint32_t joined(Join j, double radius, const std::vector<uint32_t>& kids) {

@@ -171,6 +171,34 @@ def production(seed=0, count=512, width=1920, height=1080, samples=64, surfaces=
     return "\n".join(out) + "\n"
 
 
+def stress(seed=0, count=4096, grid=64, width=3840, height=2160, samples=256, surfaces=32, glass=True):
+    """The stress scene: the production scene grown to `count` primitives,
+    with a canopy of `grid` x `grid` domain-repeated instances above it
+    (the reference's repeat(count, step), syntax/evaluator.cpp:2457-2577):
+    each a thin-walled glass bubble (a shell 0.004 thick, cut from a sphere)
+    holding a nested dielectric core of another refraction. With
+    glass=False the shells and cores (and the production scene's glass) are
+    white and grey mattes, the geometry the same."""
+    base = production(seed, count=count, width=width, height=height, samples=samples, surfaces=surfaces, glass=glass)
+    lines = base.rstrip("\n").split("\n")
+    rng = Rng(seed + 2000)
+    a_shell, b_shell = rng.uniform(1.4, 1.7), rng.uniform(0.003, 0.01)
+    a_core, b_core = rng.uniform(1.3, 1.6), rng.uniform(0.003, 0.01)
+    if glass:
+        mats = [f"let shell = glass(refraction = {a_shell:.6g}, dispersion = {b_shell:.6g});",
+                f"let core = glass(refraction = {a_core:.6g}, dispersion = {b_core:.6g});"]
+    else:
+        mats = ["let shell = matte([0.9, 0.9, 0.9]);", "let core = matte([0.6, 0.6, 0.62]);"]
+    step = 11.52 / grid
+    r = 0.4 * step
+    canopy = [f"at({_v([-5.76 + step / 2, -5.76 + step / 2, 3.2])}) repeat(count = [{grid}, {grid}, 1], step = {_v([step, step, 0])}) {{",
+              f"\tpaint(shell) cut {{ sphere({r:.6g}); sphere({r - 0.004:.6g}); }}",
+              f"\tpaint(core) sphere({0.6 * r:.6g});",
+              "}"]
+    # materials before the shapes, the canopy before the floor
+    return "\n".join(lines[:-1] + mats + canopy + lines[-1:]) + "\n"
+
+
 def window_room(width=64, height=48, samples=16, surfaces=6):
     """A closed grey room lit only by a bright sky through a small square
     window in its ceiling (no lamps): every pixel's light comes through

@@ -18,9 +18,10 @@ from kodanuki_amd import scenes
 from tests import oracle_py as O
 
 
-@pytest.fixture(scope="module", params=["tiny", "plain", "production"])
+@pytest.fixture(scope="module", params=["tiny", "plain", "production", "stress"])
 def scene(request):
-    text = {"tiny": scenes.tiny(), "plain": scenes.plain(0), "production": scenes.production(0, count=64)}[request.param]
+    text = {"tiny": scenes.tiny(), "plain": scenes.plain(0), "production": scenes.production(0, count=64),
+            "stress": scenes.stress(0, count=64, grid=4)}[request.param]
     s = k.Scene(text)
     return s, O.SceneArrays(s)
 
