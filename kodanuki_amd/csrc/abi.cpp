@@ -226,6 +226,7 @@ void releaseAll(Ctx& c) {
     freeBuf(c.tree.pool);
     freeBuf(c.tree.spool);
     freeBuf(c.tree.mpool);
+    freeBuf(c.job_counters);
     if (c.tree.level_first_dev) hipFree(c.tree.level_first_dev);
     c.tree = OctreeDev{};
     for (DevBuf& b : c.scratch) freeBuf(b);
@@ -432,7 +433,18 @@ void kdn_close(kdn_ctx* ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->c.device);
     hipStreamSynchronize(ctx->c.stream);
+    for (hipStream_t& js : ctx->c.job_streams) {
+        if (js) (void)hipStreamSynchronize(js);
+    }
     kdn::releaseAll(ctx->c);
+    for (hipEvent_t& e : ctx->c.job_events) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
+    for (hipStream_t& js : ctx->c.job_streams) {
+        if (js) (void)hipStreamDestroy(js);
+        js = nullptr;
+    }
     hipStreamDestroy(ctx->c.stream);
     delete ctx;
 }

@@ -55,6 +55,13 @@ struct Ctx {
     int device = 0;
     std::string error;
     hipStream_t stream = nullptr;
+    // the streams a round of several jobs runs its jobs' finishing launches
+    // on side by side (default priority, made on first use), and their
+    // events; the job's own counters (count and tally) per job
+    static constexpr int kJobStreams = 4;
+    hipStream_t job_streams[kJobStreams] = {};
+    hipEvent_t job_events[kJobStreams + 1] = {};
+    DevBuf job_counters;
     SceneDev scene;
     OctreeDev tree;
     // integrator state

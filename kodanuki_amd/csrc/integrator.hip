@@ -94,6 +94,14 @@ __global__ void k_sort_keys(PassParams P, const uint32_t* queue, uint32_t count,
     P.s.keys[q] = ray_key(job_of(P, i).tv, shadow ? P.s.shadow[i] : P.s.rays[i]);
 }
 
+// the job of each queued path: the key that groups a round's queue by job
+// (a stable sort on it keeps each job's paths in their sorted order)
+__global__ void k_job_keys(PassParams P, const uint32_t* queue, uint32_t count) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    P.s.keys[q] = P.slices[P.s.slice[queue[q]]].job;
+}
+
 // estimates[pixel] folded with the pixel's spp colours, in sample order
 __global__ void k_fold_samples(const double* colours, uint32_t pixels, uint32_t spp, uint32_t batch, kdn_estimate* estimates) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -373,6 +381,89 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
         if (!ok && c.error.empty()) c.error = "sorting a ray queue failed";
         return ok;
     };
+    // A round of several jobs finishes each job's paths in the launch
+    // specialised for one job (its fields from the kernel's arguments, the
+    // same for every lane), the jobs' launches side by side on the job
+    // streams: the queue, sorted, is grouped by job with a stable sort (so
+    // each job's paths keep their order), each job gets its stretch of it
+    // and counters of its own. Every live path of a plain slice is queued
+    // (spp x width x height of them), so the stretches are known here.
+    bool everySlicePlain = true;
+    for (const SliceDev& sl : slices) everySlicePlain = everySlicePlain && sl.offsets == nullptr;
+    auto finishJobsApart = [&](uint32_t qi, uint32_t count) -> bool {
+        const uint32_t nJobs = (uint32_t)jobs.size();
+        std::vector<uint64_t> live(nJobs, 0);
+        for (const SliceDev& sl : slices) live[sl.job] += (uint64_t)sl.spp * jobs[sl.job].width * jobs[sl.job].height;
+        uint64_t sum = 0;
+        for (uint64_t n : live) sum += n;
+        if (sum != count) {
+            c.error = "a round's queue holds " + std::to_string(count) + " paths, its jobs' slices " + std::to_string(sum);
+            return false;
+        }
+        int bits = 1;
+        while ((1u << bits) < nJobs) ++bits;
+        bool ok = true;
+        timed(&stats->other_ms, [&] {
+            uint32_t* queue = s->queue[qi];
+            hipLaunchKernelGGL(k_job_keys, dim3((count + 255) / 256), dim3(256), 0, st, P, (const uint32_t*)queue, count);
+            uint32_t* sorted = nullptr;
+            ok = sortPairsU32(s->keys, s->keys_alt, queue, s->vals_alt, count, bits, c.sort_tmp, st, c.error, &sorted);
+            if (ok && sorted != queue) ok = hipMemcpyAsync(queue, sorted, (size_t)count * 4, hipMemcpyDeviceToDevice, st) == hipSuccess;
+        });
+        if (!ok) return false;
+        if (!ensure(c.job_counters, (size_t)nJobs * 64, c.error, false)) return false;
+        std::vector<uint32_t> counters((size_t)nJobs * 16, 0);
+        for (uint32_t j = 0; j < nJobs; ++j) counters[(size_t)j * 16 + qi] = (uint32_t)live[j];
+        uint32_t* jc = (uint32_t*)c.job_counters.ptr;
+        KDN_HIP(hipMemcpyAsync(jc, counters.data(), counters.size() * 4, hipMemcpyHostToDevice, st));
+        for (int k = 0; k < Ctx::kJobStreams; ++k) {
+            if (!c.job_streams[k]) KDN_HIP(hipStreamCreate(&c.job_streams[k]));
+        }
+        for (hipEvent_t& e : c.job_events) {
+            if (!e) KDN_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
+        KDN_HIP(hipEventRecord(c.job_events[Ctx::kJobStreams], st));
+        for (int k = 0; k < Ctx::kJobStreams; ++k) KDN_HIP(hipStreamWaitEvent(c.job_streams[k], c.job_events[Ctx::kJobStreams], 0));
+        const uint32_t resident = residentTraceBlocks(c);
+        bool launched = true;
+        timed(&stats->finish_ms, [&] {
+            uint64_t first = 0;
+            for (uint32_t j = 0; j < nJobs; ++j) {
+                PassParams Pj = P;
+                Pj.s.queue[qi] = s->queue[qi] + first;
+                Pj.s.counters = jc + (size_t)j * 16;
+                Pj.job0 = jobs[j];
+                first += live[j];
+                if (live[j] == 0) continue;
+                hipStream_t js = c.job_streams[j % Ctx::kJobStreams];
+                const dim3 grid(std::min<uint32_t>((uint32_t)((live[j] + kTraceBlock - 1) / kTraceBlock), resident)), block(kTraceBlock);
+                if (plain) {
+                    hipLaunchKernelGGL((k_finish<true, true>), grid, block, 0, js, Pj, qi, Pj.s.counters + 4, nullptr);
+                } else {
+                    hipLaunchKernelGGL((k_finish<true, false>), grid, block, 0, js, Pj, qi, Pj.s.counters + 4, nullptr);
+                }
+                launched = launched && hipGetLastError() == hipSuccess;
+            }
+            for (int k = 0; k < Ctx::kJobStreams; ++k) {
+                (void)hipEventRecord(c.job_events[k], c.job_streams[k]);
+                (void)hipStreamWaitEvent(st, c.job_events[k], 0);
+            }
+        });
+        if (!launched) {
+            c.error = "a job's finishing launch failed";
+            return false;
+        }
+        std::vector<uint32_t> tallies(counters.size(), 0);
+        KDN_HIP(hipMemcpyAsync(tallies.data(), jc, tallies.size() * 4, hipMemcpyDeviceToHost, st));
+        KDN_HIP(hipStreamSynchronize(st));
+        for (uint32_t j = 0; j < nJobs; ++j) {
+            stats->segments += tallies[(size_t)j * 16 + 4];
+            stats->shadow_rays += tallies[(size_t)j * 16 + 5];
+        }
+        ++stats->bounces;
+        stats->finish_launches += nJobs;
+        return true;
+    };
     if (guiding) {
         if (!guideReady(c) && !guideReset(c, 1, c.tree.lower, c.tree.size)) return false;
         if (!guideBeginRound(c, P, total)) return false;
@@ -394,6 +485,10 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
             // (camera rays too: grouped by direction octant, the finishing
             // lanes ran 1.05x faster at 1080p than in the 8x8 tiles' order)
             if (!sortQueue(in, count, 0)) return false;
+            if (jobs.size() > 1 && !counting && everySlicePlain) {
+                if (!finishJobsApart(in, count)) return false;
+                break;
+            }
             KDN_HIP(hipMemsetAsync(&s->counters[4], 0, 12, st));
             const uint32_t finishBlocks = std::min<uint32_t>((count + kTraceBlock - 1) / kTraceBlock, residentTraceBlocks(c));
             timed(&stats->finish_ms, [&] {
