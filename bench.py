@@ -2,7 +2,7 @@
 
 The metric (equal-MSE Msamples/s of the served, learned-guided renderer
 against the plain GPU tracer) is quoted on the glass-heavy scene (the
-32-primitive plain scene with 60% dispersive glass). Glass scattering is
+512-primitive production scene with 60% dispersive glass). Glass scattering is
 held off the device, so `value` is null and the line says why; nothing
 else stands in for it. Beside it:
 
@@ -487,7 +487,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "glass-heavy scene (the 32-primitive plain scene with 60% dispersive glass) at 1920x1080, "
+        "config": {"workload": "glass-heavy scene (the 512-primitive production scene with 60% dispersive glass) at 1920x1080, "
                               "served learned guide against the plain tracer at equal relMSE: not measured (see "
                               "'unmeasured'); the timed steps are the plain tracer on the production scene (see 'plain')",
                    "method": "plain"},

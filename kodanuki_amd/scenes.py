@@ -11,7 +11,7 @@ Workloads:
 - production(512): 512 primitives in groups nested by cut/clip/merge to
                depth 4, 40% matte / 30% rough metal / 30% dispersive glass,
                four quad lights and a gradient sky, 1920x1080.
-- glass_heavy: the plain scene with 60% glass.
+- glass_heavy: the production scene with 60% glass (config 2).
 - serving_pool: eight of the above, 32 to 4096 primitives.
 - window_room: a room lit by the sky through a small ceiling window.
 """
@@ -122,8 +122,10 @@ def plain(seed=0, count=32, glass=0.0, width=256, height=256, samples=16, surfac
 
 
 def glass_heavy(seed=0, **kw):
-    """The plain scene with 60% glass (caustics)."""
-    return plain(seed, glass=0.6, **kw)
+    """Config 2's scene: the production scene (config 1) with glass-heavy
+    caustics, 60% of its materials dielectric with Cauchy dispersion, the
+    rest matte and rough metal alike."""
+    return production(seed, kinds=(0.2, 0.2, 0.6), **kw)
 
 
 def _nest(rng, depth, leaves, materials):
@@ -141,8 +143,9 @@ def _nest(rng, depth, leaves, materials):
     return [head] + ["\t" + line for line in body] + ["}"]
 
 
-def production(seed=0, count=512, width=1920, height=1080, samples=64, surfaces=13, glass=True):
-    """The production scene: groups of 4 primitives nested to depth 4.
+def production(seed=0, count=512, width=1920, height=1080, samples=64, surfaces=13, glass=True, kinds=(0.4, 0.3, 0.3)):
+    """The production scene: groups of 4 primitives nested to depth 4, its
+    materials matte, rough metal and dispersive glass in the shares `kinds`.
     With glass=False its glass materials are white mattes (same geometry)."""
     rng = Rng(seed + 1000)
     out = ['import "standard.sdf";',
@@ -154,7 +157,7 @@ def production(seed=0, count=512, width=1920, height=1080, samples=64, surfaces=
         out.append(f"light quad {{ at = {_v([x, y, 5])}; along = [0, 1.2, 0]; across = [1.2, 0, 0]; power = {_v([p, p * 0.95, p * 0.9])}; }}")
     nm = 48
     for i in range(nm):
-        out.append(_material(rng, f"m{i}", [0.4, 0.3, 0.3], glass))
+        out.append(_material(rng, f"m{i}", list(kinds), glass))
     out.append("let floor = matte([0.5, 0.5, 0.48]);")
     groups = count // 4
     side = int(math.ceil(math.sqrt(groups)))
