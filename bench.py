@@ -68,7 +68,7 @@ def cpu_baseline(scene, lower, size, width, height, seed, surfaces, budget_s=12.
 
 
 def serving_stream(k, scenes, dev, jobs, rate, seed, max_paths, budget, sizes=None, spps=None, pool_texts=None, guided=False,
-                   alone=False):
+                   alone=False, learned=False):
     """The serving stream: `jobs` render jobs with Poisson arrivals at
     `rate` jobs/s, each a scene of the eight-scene pool (32 to 4096
     primitives), a size of {256^2, 512^2, 1024^2, 1920x1080} and
@@ -77,7 +77,9 @@ def serving_stream(k, scenes, dev, jobs, rate, seed, max_paths, budget, sizes=No
     to the end of the round that folds its last sample; scene builds are
     part of the first job of each scene. With guided, every job samples
     its first vertices with the plain guide (its survey at admission is
-    part of its latency). With alone, the same jobs are also served one at
+    part of its latency). With learned, every job samples with a learned
+    guide of its own, trained after each of its rounds on its own paths
+    (one slice of the job a round). With alone, the same jobs are also served one at
     a time back to back, for the serving loop's efficiency."""
     import random
     rng = random.Random(seed)
@@ -110,7 +112,7 @@ def serving_stream(k, scenes, dev, jobs, rate, seed, max_paths, budget, sizes=No
         while pending and pending[0][0] <= now:
             arrival, si, w, h, spp = pending.pop(0)
             sc, depth, rs = pool[si]
-            j = server.submit(sc, w, h, spp, seed=rs.seed, surfaces=rs.surfaces, depth=depth, guided=guided)
+            j = server.submit(sc, w, h, spp, seed=rs.seed, surfaces=rs.surfaces, depth=depth, guided=guided, learned=learned)
             open_jobs[j] = (arrival, si, w, h, spp)
         if not open_jobs:
             time.sleep(max(0.0, pending[0][0] - (time.perf_counter() - start)))
@@ -146,7 +148,7 @@ def serving_stream(k, scenes, dev, jobs, rate, seed, max_paths, budget, sizes=No
             t0 = time.perf_counter()
             for arrival, si, w, h, spp in stream:
                 sc, depth, rs = pool[si]
-                j = solo.submit(sc, w, h, spp, seed=rs.seed, surfaces=rs.surfaces, depth=depth, guided=guided)
+                j = solo.submit(sc, w, h, spp, seed=rs.seed, surfaces=rs.surfaces, depth=depth, guided=guided, learned=learned)
                 solo.run([j])
                 solo.release(j)
             alone_s = time.perf_counter() - t0
@@ -158,7 +160,8 @@ def serving_stream(k, scenes, dev, jobs, rate, seed, max_paths, budget, sizes=No
                         f"{', '.join(f'{w}x{h}' for w, h in sizes)}; spp from {', '.join(map(str, spps))}; scenes from "
                         f"{len(pool)} seeded scenes" + (" of 32 to 4096 primitives" if pool_texts is None else "") +
                         f" (glass painted as white matte); rounds of at most {max_paths} paths, "
-                        f"budget {budget / 2 ** 30:g} GiB" + ("; every job with the plain guide" if guided else ""),
+                        f"budget {budget / 2 ** 30:g} GiB" + ("; every job with the plain guide" if guided else "") +
+                        ("; every job with a learned guide of its own" if learned else ""),
             "p50_ms": pct(0.50), "p99_ms": pct(0.99), "mean_ms": sum(lat) / len(lat),
             "aggregate_msamples_s": total_samples / wall_s / 1e6, "samples": total_samples, "wall_s": wall_s,
             "alone_msamples_s": None if alone_s is None else total_samples / alone_s / 1e6,
@@ -448,7 +451,7 @@ def main():
     if not args.no_viewer:
         viewer = viewer_leg(k, scenes, dev, args.viewer_size[0], args.viewer_size[1], args.viewer_frames)
 
-    serving = serving_guided = None
+    serving = serving_guided = serving_learned = None
     if not args.no_serve:
         # the stream at each arrival rate (jobs/s), the same jobs each time
         serving = [serving_stream(k, scenes, dev, args.serve_jobs, rate, args.serve_seed, args.serve_paths,
@@ -457,6 +460,9 @@ def main():
             # the same stream served with the plain guide on every job
             serving_guided = serving_stream(k, scenes, dev, args.serve_jobs, args.serve_rates[0], args.serve_seed, args.serve_paths,
                                             args.serve_budget_gib << 30, guided=True)
+            # and with a learned guide of its own on every job
+            serving_learned = serving_stream(k, scenes, dev, args.serve_jobs, args.serve_rates[0], args.serve_seed, args.serve_paths,
+                                             args.serve_budget_gib << 30, learned=True)
 
     cpu = None
     if not args.no_cpu:
@@ -498,6 +504,7 @@ def main():
         "viewer": viewer,
         "serving": serving,
         "serving_guided": serving_guided,
+        "serving_learned": serving_learned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

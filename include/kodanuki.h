@@ -721,7 +721,12 @@ typedef struct kdn_job_desc {
     int32_t depth;           /* the octree's depth cap, 1..11 */
     uint32_t flags;          /* bit 0: the plain guide (a survey of the job's
                                 image at admission; its guides are the job's
-                                memory, 736 B a pixel) */
+                                memory, 736 B a pixel); bit 1: the learned
+                                guide (the job's own network over its scene's
+                                region from its seed, trained after each round
+                                on the job's own paths; one slice of the job a
+                                round; about 420 MB of the job's memory); not
+                                both */
     uint32_t pad;
 } kdn_job_desc;
 

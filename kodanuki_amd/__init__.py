@@ -258,6 +258,7 @@ class JobDesc(C.Structure):
 
 
 JOB_PLAIN_GUIDE = 1
+JOB_LEARNED_GUIDE = 2
 
 
 class JobStatus(C.Structure):
@@ -852,10 +853,13 @@ class Server:
     def __exit__(self, *a):
         self.close()
 
-    def submit(self, scene, width, height, spp, seed=1, surfaces=7, min_depth=4, depth=8, guided=False):
+    def submit(self, scene, width, height, spp, seed=1, surfaces=7, min_depth=4, depth=8, guided=False, learned=False):
         """A job; guided: its first vertices sampled with the plain guide
-        (a survey of its image when it is admitted)."""
-        d = JobDesc(scene.handle, width, height, spp, seed, surfaces, min_depth, depth, JOB_PLAIN_GUIDE if guided else 0, 0)
+        (a survey of its image when it is admitted); learned: its vertices
+        sampled with a learned guide of its own, trained on its own paths
+        after each round (one slice of the job a round)."""
+        flags = (JOB_PLAIN_GUIDE if guided else 0) | (JOB_LEARNED_GUIDE if learned else 0)
+        d = JobDesc(scene.handle, width, height, spp, seed, surfaces, min_depth, depth, flags, 0)
         job = C.c_uint64()
         self._check(library().kdn_serve_submit(self._h, C.byref(d), C.byref(job)))
         self._scenes[job.value] = scene

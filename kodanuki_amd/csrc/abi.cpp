@@ -205,7 +205,7 @@ uint64_t residentBytes(const Ctx& c) {
                      (uint64_t)s.n_leaves * 4;
     bytes += c.tree.nodes.bytes + c.tree.entries.bytes + c.tree.pool.bytes + c.tree.spool.bytes + c.tree.mpool.bytes;
     for (const DevBuf& b : c.scratch) bytes += b.bytes;
-    bytes += c.paths_bytes + c.round_bytes + c.hguides.bytes + c.sort_tmp.bytes;
+    bytes += c.paths_bytes + c.round_bytes + c.hguides.bytes + c.sort_tmp.bytes + guideRoundBytes(c);
     if (c.accum) bytes += (uint64_t)c.width * c.height * (6 * sizeof(double) + sizeof(uint32_t));
     return bytes;
 }

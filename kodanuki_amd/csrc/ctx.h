@@ -84,6 +84,7 @@ struct Ctx {
     size_t round_bytes = 0;
     bool scene_has_glass = false;   // a leaf is painted with a glass
     void* guide = nullptr;          // the learned guide (GuideState), made on first use
+    void* guide_round = nullptr;    // lobes and records of rounds whose jobs own their guides (serving)
     DevBuf sort_tmp;                // the ray queues' sort's temporary storage
     DevBuf hguides;                 // the plain guide: a packed guide per pixel
     int hguide_width = 0, hguide_height = 0;  // the image they were made for (0: none)
@@ -133,6 +134,14 @@ void freeIntegrator(Ctx& c);
 bool guideReset(Ctx& c, uint32_t seed, const float lower[3], const float size[3]);
 bool guideReady(const Ctx& c);
 void freeGuide(Ctx& c);
+// a served job's own learned guide: made over its scene's region from a
+// seed (null on failure, c.error set), its device bytes, its Adam steps
+void* guideOpen(Ctx& c, uint32_t seed, const float lower[3], const float size[3]);
+void guideClose(void* guide);
+uint64_t guideBytes();
+uint32_t guideSteps(const void* guide);
+// the device bytes of a context's rounds' lobes and records (serving)
+uint64_t guideRoundBytes(const Ctx& c);
 
 // stopping.hip: effortFor, confidenceTrial, and the device's judge and
 // allotment of every pixel's count from the trial estimates
@@ -183,7 +192,10 @@ namespace kdn {
 // loop's unit of work)
 struct JobDev;
 struct SliceDev;
-bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDev>& slices, uint32_t flags, kdn_pass_stats* stats);
+// guides: a learned guide of each job's own (null entries: none; a job
+// with one has one plain slice in the round)
+bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDev>& slices, uint32_t flags, kdn_pass_stats* stats,
+              const std::vector<void*>* guides = nullptr);
 uint32_t planeOf(int width, int height);
 // the path state for n paths in flight, allocated now (it grows on demand)
 bool reservePaths(Ctx& c, size_t n);

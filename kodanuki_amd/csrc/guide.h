@@ -75,6 +75,24 @@ KDN_D uint16_t to_bf16(float x) {
 }
 KDN_D float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// ---- grid gradients in fixed point -------------------------------------------
+// A grid gradient is summed as a 64-bit integer of 2^-32 units: integer sums
+// are the same in any order, so a training step gives the same gradients
+// however its atomics interleave. A contribution of magnitude 2^11 or more,
+// or not finite, adds nothing and marks the step wild (it is skipped, as a
+// step with a non-finite gradient is); so a step of at most 2^16 samples
+// (eight corners a level each) keeps every sum below 2^31 in magnitude.
+constexpr float kFixedUnit = 0x1p32f;
+KDN_D uint64_t fixed_of(float v, bool* wild) {
+    const float x = v * kFixedUnit;
+    if (!(__builtin_fabsf(x) < 0x1p43f)) {
+        *wild = true;
+        return 0;
+    }
+    return (uint64_t)(int64_t)x;
+}
+KDN_D float float_of_fixed(uint64_t a) { return (float)((double)(int64_t)a * 0x1p-32); }
+
 // ---- encoding -----------------------------------------------------------------
 KDN_D uint32_t grid_index(const GuideView& g, int level, uint32_t x, uint32_t y, uint32_t z) {
     if (g.dense >> level & 1u) {

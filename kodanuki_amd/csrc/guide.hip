@@ -45,6 +45,7 @@ struct GuideState {
     uint32_t seed = 0;
     float lower[3] = {0, 0, 0}, size[3] = {1, 1, 1};
     float* grid = nullptr;  // params, then Adam m, v, then gradients
+    uint64_t* grid_fixed = nullptr;  // the grid gradients as training sums them (fixed point, kept cleared)
     float* mlp = nullptr;   // the same for the layers
     uint16_t* mlp_bf = nullptr;
     float* scalars = nullptr;  // [0] weight scale, [1] loss, [2..] per-step skip flags (uint)
@@ -60,6 +61,12 @@ struct GuideState {
     float* mlpM() const { return mlp + kGuideMlpParams; }
     float* mlpV() const { return mlp + 2 * kGuideMlpParams; }
     float* mlpG() const { return mlp + 3 * kGuideMlpParams; }
+};
+
+// a round's lobes and records when its jobs' learned guides are their own
+// (serving)
+struct GuideRound {
+    DevBuf lobes, recs, samples, flags, offsets, scan_tmp;
 };
 
 constexpr int kScalarFlags = 2;
@@ -108,6 +115,15 @@ struct AdamStep {
     float lr, inv_scale, c1, c2;  // c1 = 1 - beta1^t, c2 = 1 - beta2^t
 };
 
+// the fixed-point grid gradients as floats, the sums cleared for the next
+// step
+__global__ void k_guide_unfix(uint64_t* fixed, float* grad, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        grad[i] = float_of_fixed(fixed[i]);
+        fixed[i] = 0;
+    }
+}
+
 // Adam on n parameters; gradients are cleared; with the step's flag
 // raised nothing moves. bf, when set, receives the BF16 copy.
 __global__ void k_guide_adam(float* p, float* m, float* v, float* g, uint64_t n, AdamStep a, const uint32_t* flag, uint16_t* bf) {
@@ -130,14 +146,14 @@ __global__ void k_guide_adam(float* p, float* m, float* v, float* g, uint64_t n,
 // the samples of a round's records: weight = radiance back along the
 // direction (final less snapshot, over the throughput, mean of the live
 // lanes) over its density; valid[r] = 1 for a sample that carries some
-__global__ void k_guide_collect(const float* recs, uint32_t nRec, uint32_t stride, const float* rad, size_t N, uint32_t product,
-                                float* samples, uint32_t* valid) {
+__global__ void k_guide_collect(const float* recs, uint32_t nRec, uint32_t first, uint32_t stride, const float* rad, size_t N,
+                                uint32_t product, float* samples, uint32_t* valid) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nRec) return;
     const float* rec = recs + (size_t)r * kRecFloats;
     uint32_t ok = 0;
     if (rec[kRecState] == 3.0f) {
-        const size_t i = (size_t)(r / kGuideDepths) * stride;
+        const size_t i = first + (size_t)(r / kGuideDepths) * stride;
         float sum = 0;
         int live = 0;
         // the incident radiance over the density, or (product) the
@@ -213,23 +229,67 @@ void viewOf(GuideState& G, uint32_t maxRes) {
     }
 }
 
-bool allocGuide(Ctx& c) {
-    if (c.guide) return true;
+void deleteGuide(GuideState* G) {
+    if (!G) return;
+    (void)hipFree(G->grid);
+    (void)hipFree(G->grid_fixed);
+    (void)hipFree(G->mlp);
+    (void)hipFree(G->mlp_bf);
+    (void)hipFree(G->scalars);
+    for (DevBuf* b : {&G->partial, &G->lobes, &G->recs, &G->samples, &G->flags, &G->offsets, &G->scan_tmp}) {
+        if (b->ptr) (void)hipFree(b->ptr);
+    }
+    delete G;
+}
+
+GuideState* newGuide(std::string& error) {
     GuideState* G = new GuideState();
     if (hipMalloc(&G->grid, 4 * kGuideGridParams * sizeof(float)) != hipSuccess ||
+        hipMalloc(&G->grid_fixed, kGuideGridParams * sizeof(uint64_t)) != hipSuccess ||
         hipMalloc(&G->mlp, 4 * (size_t)kGuideMlpParams * sizeof(float)) != hipSuccess ||
         hipMalloc(&G->mlp_bf, (size_t)kGuideMlpParams * sizeof(uint16_t)) != hipSuccess ||
         hipMalloc(&G->scalars, kScalarWords * sizeof(float)) != hipSuccess) {
-        c.error = "hipMalloc of the learned guide failed";
-        if (G->grid) (void)hipFree(G->grid);
-        if (G->mlp) (void)hipFree(G->mlp);
-        if (G->mlp_bf) (void)hipFree(G->mlp_bf);
-        delete G;
+        error = "hipMalloc of the learned guide failed";
+        deleteGuide(G);
+        return nullptr;
+    }
+    return G;
+}
+
+bool allocGuide(Ctx& c) {
+    if (c.guide) return true;
+    c.guide = newGuide(c.error);
+    return c.guide != nullptr;
+}
+
+// fresh parameters from a seed over a region, Adam's moments cleared
+bool resetGuide(GuideState& G, hipStream_t st, uint32_t seed, const float lower[3], const float size[3], std::string& error) {
+    G.seed = seed;
+    for (int k = 0; k < 3; ++k) {
+        G.lower[k] = lower[k];
+        G.size[k] = size[k];
+    }
+    viewOf(G, G.config.max_resolution);
+    G.view.share_cap = G.config.share;
+    G.view.learn_select = G.config.learn_select;
+    if (hipMemsetAsync(G.gridM(), 0, 3 * kGuideGridParams * sizeof(float), st) != hipSuccess ||
+        hipMemsetAsync(G.grid_fixed, 0, kGuideGridParams * sizeof(uint64_t), st) != hipSuccess ||
+        hipMemsetAsync(G.mlpM(), 0, 3 * (size_t)kGuideMlpParams * sizeof(float), st) != hipSuccess) {
+        error = "clearing the learned guide failed";
         return false;
     }
-    c.guide = G;
+    hipLaunchKernelGGL(k_guide_init, dim3((uint32_t)((kGuideGridParams + 255) / 256)), dim3(256), 0, st, G.grid, G.mlp, G.mlp_bf, seed);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+        error = "the learned guide's initialisation failed";
+        return false;
+    }
+    G.steps = 0;
+    G.config = defaultConfig();
+    G.loss_scale = 128.0f;
+    G.last_loss = 0;
     return true;
 }
+
 
 // Adam steps over the samples in HBM: batches of `batch` (modulo n) with
 // weights normalised by their mean; the last step's loss in G.last_loss.
@@ -249,10 +309,12 @@ bool trainSteps(Ctx& c, GuideState& G, const float* samples, uint32_t n, uint32_
     float* lossOf = G.scalars + kScalarFlags + kGuideStepsPerRound;  // per-step losses
     for (uint32_t k = 0; k < steps; ++k) {
         const uint32_t first = (uint32_t)(((uint64_t)(start + k) * batch) % n);
-        if (!launchGuideTrain(G.view, samples, n, first, batch, G.scalars, (float*)G.partial.ptr, G.gridG(), lossOf + k, st)) {
+        if (!launchGuideTrain(G.view, samples, n, first, batch, G.scalars, (float*)G.partial.ptr, G.grid_fixed, lossOf + k, flags + k,
+                              st)) {
             c.error = "the guide's training launch failed";
             return false;
         }
+        hipLaunchKernelGGL(k_guide_unfix, dim3(4096), dim3(256), 0, st, G.grid_fixed, G.gridG(), kGuideGridParams);
         hipLaunchKernelGGL(k_guide_reduce, dim3(kGuideMlpParams / 256), dim3(256), 0, st, (const float*)G.partial.ptr, tiles, G.mlpG(),
                            flags + k);
         hipLaunchKernelGGL(k_guide_check, dim3(2048), dim3(256), 0, st, (const float*)G.gridG(), kGuideGridParams, flags + k);
@@ -291,42 +353,22 @@ bool trainSteps(Ctx& c, GuideState& G, const float* samples, uint32_t n, uint32_
 
 bool guideReset(Ctx& c, uint32_t seed, const float lower[3], const float size[3]) {
     if (!allocGuide(c)) return false;
-    GuideState& G = *stateOf(c);
-    hipStream_t st = c.stream;
-    G.seed = seed;
-    for (int k = 0; k < 3; ++k) {
-        G.lower[k] = lower[k];
-        G.size[k] = size[k];
-    }
-    viewOf(G, G.config.max_resolution);
-    G.view.share_cap = G.config.share;
-    G.view.learn_select = G.config.learn_select;
-    KDN_HIP(hipMemsetAsync(G.gridM(), 0, 3 * kGuideGridParams * sizeof(float), st));
-    KDN_HIP(hipMemsetAsync(G.mlpM(), 0, 3 * (size_t)kGuideMlpParams * sizeof(float), st));
-    hipLaunchKernelGGL(k_guide_init, dim3((uint32_t)((kGuideGridParams + 255) / 256)), dim3(256), 0, st, G.grid, G.mlp, G.mlp_bf, seed);
-    KDN_HIP(hipGetLastError());
-    KDN_HIP(hipStreamSynchronize(st));
-    G.steps = 0;
-    G.config = defaultConfig();
-    G.loss_scale = 128.0f;
-    G.last_loss = 0;
-    return true;
+    return resetGuide(*stateOf(c), c.stream, seed, lower, size, c.error);
 }
 
 bool guideReady(const Ctx& c) { return c.guide != nullptr; }
 
 void freeGuide(Ctx& c) {
-    GuideState* G = (GuideState*)c.guide;
-    if (!G) return;
-    (void)hipFree(G->grid);
-    (void)hipFree(G->mlp);
-    (void)hipFree(G->mlp_bf);
-    (void)hipFree(G->scalars);
-    for (DevBuf* b : {&G->partial, &G->lobes, &G->recs, &G->samples, &G->flags, &G->offsets, &G->scan_tmp}) {
-        if (b->ptr) (void)hipFree(b->ptr);
-    }
-    delete G;
+    deleteGuide((GuideState*)c.guide);
     c.guide = nullptr;
+    GuideRound* R = (GuideRound*)c.guide_round;
+    if (R) {
+        for (DevBuf* b : {&R->lobes, &R->recs, &R->samples, &R->flags, &R->offsets, &R->scan_tmp}) {
+            if (b->ptr) (void)hipFree(b->ptr);
+        }
+        delete R;
+        c.guide_round = nullptr;
+    }
 }
 
 // Before a guided round: lobes for every path, records cleared.
@@ -357,23 +399,35 @@ bool guideInferQueue(Ctx& c, const PassParams& P, uint32_t queue, uint32_t count
 }
 
 // After a guided round: its records as samples, then Adam steps on them.
+// The samples of `nRec` records (those of the paths first, first +
+// stride, ...) packed in the buffers given, then Adam steps on them as the
+// guide's config asks; the counts added to stats.
+bool trainOnRecords(Ctx& c, GuideState& G, const float* recs, uint32_t nRec, uint32_t first, uint32_t stride, const float* rad,
+                    size_t N, DevBuf& flagsBuf, DevBuf& offsetsBuf, DevBuf& samplesBuf, DevBuf& scanTmp, kdn_pass_stats* stats);
+
 bool guideEndRound(Ctx& c, const PassParams& P, uint32_t paths, kdn_pass_stats* stats) {
     GuideState& G = *stateOf(c);
-    hipStream_t st = c.stream;
     const uint32_t stride = P.train_stride;
     const uint32_t nRec = ((paths + stride - 1) / stride) * kGuideDepths;
-    if (!ensure(G.flags, (size_t)nRec * 4 + 64, c.error, false)) return false;
-    if (!ensure(G.offsets, (size_t)nRec * 4 + 64, c.error, false)) return false;
-    if (!ensure(G.samples, (size_t)nRec * 2 * kSamFloats * sizeof(float), c.error, false)) return false;
-    float* raw = (float*)G.samples.ptr;
+    return trainOnRecords(c, G, (const float*)G.recs.ptr, nRec, 0, stride, (const float*)P.s.rad, P.s.capacity, G.flags, G.offsets,
+                          G.samples, G.scan_tmp, stats);
+}
+
+bool trainOnRecords(Ctx& c, GuideState& G, const float* recs, uint32_t nRec, uint32_t first, uint32_t stride, const float* rad,
+                    size_t N, DevBuf& flagsBuf, DevBuf& offsetsBuf, DevBuf& samplesBuf, DevBuf& scanTmp, kdn_pass_stats* stats) {
+    hipStream_t st = c.stream;
+    if (!ensure(flagsBuf, (size_t)nRec * 4 + 64, c.error, false)) return false;
+    if (!ensure(offsetsBuf, (size_t)nRec * 4 + 64, c.error, false)) return false;
+    if (!ensure(samplesBuf, (size_t)nRec * 2 * kSamFloats * sizeof(float), c.error, false)) return false;
+    float* raw = (float*)samplesBuf.ptr;
     float* packed = raw + (size_t)nRec * kSamFloats;
-    hipLaunchKernelGGL(k_guide_collect, dim3((nRec + 255) / 256), dim3(256), 0, st, (const float*)G.recs.ptr, nRec, stride, (const float*)P.s.rad,
-                       P.s.capacity, G.config.product, raw, (uint32_t*)G.flags.ptr);
+    hipLaunchKernelGGL(k_guide_collect, dim3((nRec + 255) / 256), dim3(256), 0, st, recs, nRec, first, stride, rad, N, G.config.product,
+                       raw, (uint32_t*)flagsBuf.ptr);
     uint64_t total = 0;
-    if (!scanU32((const uint32_t*)G.flags.ptr, (uint32_t*)G.offsets.ptr, nRec, &total, G.scan_tmp, st, c.error)) return false;
+    if (!scanU32((const uint32_t*)flagsBuf.ptr, (uint32_t*)offsetsBuf.ptr, nRec, &total, scanTmp, st, c.error)) return false;
     const uint32_t n = (uint32_t)total;
-    hipLaunchKernelGGL(k_guide_compact, dim3((nRec + 255) / 256), dim3(256), 0, st, (const float*)raw, (const uint32_t*)G.flags.ptr,
-                       (const uint32_t*)G.offsets.ptr, nRec, packed);
+    hipLaunchKernelGGL(k_guide_compact, dim3((nRec + 255) / 256), dim3(256), 0, st, (const float*)raw, (const uint32_t*)flagsBuf.ptr,
+                       (const uint32_t*)offsetsBuf.ptr, nRec, packed);
     KDN_HIP(hipGetLastError());
     G.last_samples = n;
     G.last_steps = 0;
@@ -388,6 +442,110 @@ bool guideEndRound(Ctx& c, const PassParams& P, uint32_t paths, kdn_pass_stats* 
         stats->guide_samples += n;
         stats->guide_steps += G.last_steps;
         stats->guide_loss = G.last_loss;
+    }
+    return true;
+}
+
+// ---- the learned guides of served jobs --------------------------------------
+// Each served job with the learned guide owns a GuideState over its scene's
+// region, trained after every round on its own paths' records only; a
+// round's lobes and records sit in the round context's GuideRound.
+
+void* guideOpen(Ctx& c, uint32_t seed, const float lower[3], const float size[3]) {
+    GuideState* G = newGuide(c.error);
+    if (!G) return nullptr;
+    if (!resetGuide(*G, c.stream, seed, lower, size, c.error)) {
+        deleteGuide(G);
+        return nullptr;
+    }
+    return G;
+}
+
+void guideClose(void* guide) { deleteGuide((GuideState*)guide); }
+
+uint64_t guideBytes() {
+    return 4 * kGuideGridParams * sizeof(float) + kGuideGridParams * sizeof(uint64_t) + 4 * (uint64_t)kGuideMlpParams * sizeof(float) +
+           (uint64_t)kGuideMlpParams * sizeof(uint16_t) + kScalarWords * sizeof(float) +
+           (uint64_t)guideTrainPartials(kGuideBatch) * kGuideMlpParams * sizeof(float);
+}
+
+uint32_t guideSteps(const void* guide) { return ((const GuideState*)guide)->steps; }
+
+uint64_t guideRoundBytes(const Ctx& c) {
+    const GuideRound* R = (const GuideRound*)c.guide_round;
+    if (!R) return 0;
+    return R->lobes.bytes + R->recs.bytes + R->samples.bytes + R->flags.bytes + R->offsets.bytes + R->scan_tmp.bytes;
+}
+
+bool guidesBeginRound(Ctx& c, PassParams& P, std::vector<JobDev>& jobs, const std::vector<SliceDev>& slices,
+                      const std::vector<void*>& guides) {
+    if (!c.guide_round) c.guide_round = new GuideRound();
+    GuideRound& R = *(GuideRound*)c.guide_round;
+    const size_t N = c.paths->capacity;
+    uint64_t blocks = 0;
+    bool anyLearned = false;
+    for (uint32_t j = 0; j < (uint32_t)jobs.size(); ++j) {
+        if (!guides[j]) continue;
+        const GuideState& G = *(const GuideState*)guides[j];
+        const SliceDev* one = nullptr;
+        for (const SliceDev& sl : slices) {
+            if (sl.job != j) continue;
+            if (one || sl.offsets) {
+                c.error = "a job with the learned guide has one plain slice a round";
+                return false;
+            }
+            one = &sl;
+        }
+        if (!one) continue;
+        const uint32_t n = slice_paths(*one, jobs[j].plane);
+        uint32_t stride = 1;
+        while ((uint64_t)stride * G.config.train_paths < n) stride *= 2;
+        jobs[j].learned = G.steps > 0 ? 1u : 0u;
+        jobs[j].train_stride = stride;
+        jobs[j].path_first = one->first_path;
+        jobs[j].rec_first = (uint32_t)blocks;
+        blocks += (n + stride - 1) / stride;
+        anyLearned = anyLearned || G.steps > 0;
+    }
+    if (anyLearned && !ensure(R.lobes, N * kGuideOutputs * sizeof(float), c.error, false)) return false;
+    const size_t recBytes = (size_t)std::max<uint64_t>(blocks, 1) * kGuideDepths * kRecFloats * sizeof(float);
+    if (!ensure(R.recs, recBytes, c.error, false)) return false;
+    KDN_HIP(hipMemsetAsync(R.recs.ptr, 0, recBytes, c.stream));
+    P.lobes = anyLearned ? (const float*)R.lobes.ptr : nullptr;
+    P.recs = (float*)R.recs.ptr;
+    P.train_stride = 0;
+    return true;
+}
+
+bool guidesInferQueue(Ctx& c, const PassParams& P, uint32_t queue, uint32_t first, uint32_t count, void* guide) {
+    if (!P.lobes || count == 0) return true;
+    GuideState& G = *(GuideState*)guide;
+    if (G.steps == 0) return true;  // its job samples by the BSDF this round
+    PassParams Q = P;
+    Q.s.queue[queue] = P.s.queue[queue] + first;
+    if (!launchGuideInferPaths(G.view, Q, queue, count, (float*)((GuideRound*)c.guide_round)->lobes.ptr, c.stream)) {
+        c.error = "the guide's inference launch failed";
+        return false;
+    }
+    return true;
+}
+
+bool guidesEndRound(Ctx& c, const PassParams& P, const std::vector<JobDev>& jobs, const std::vector<SliceDev>& slices,
+                    const std::vector<void*>& guides, kdn_pass_stats* stats) {
+    GuideRound& R = *(GuideRound*)c.guide_round;
+    for (uint32_t j = 0; j < (uint32_t)jobs.size(); ++j) {
+        if (!guides[j] || jobs[j].train_stride == 0) continue;
+        uint32_t n = 0;
+        for (const SliceDev& sl : slices)
+            if (sl.job == j) n = slice_paths(sl, jobs[j].plane);
+        if (n == 0) continue;
+        const uint32_t stride = jobs[j].train_stride;
+        const uint32_t nRec = ((n + stride - 1) / stride) * kGuideDepths;
+        const float* recs = (const float*)R.recs.ptr + (size_t)jobs[j].rec_first * kGuideDepths * kRecFloats;
+        if (!trainOnRecords(c, *(GuideState*)guides[j], recs, nRec, jobs[j].path_first, stride, (const float*)P.s.rad, P.s.capacity,
+                            R.flags, R.offsets, R.samples, R.scan_tmp, stats)) {
+            return false;
+        }
     }
     return true;
 }
@@ -560,14 +718,15 @@ int kdn_guide_gradients(kdn_ctx* ctx, const kdn_guide_sample* samples, uint32_t 
     float* s = (float*)c.scratch[6].ptr;
     float host[kScalarParts];
     if (hipMemcpyAsync(s, samples, (size_t)n * sizeof(kdn_guide_sample), hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemsetAsync(G->scalars, 0, kScalarWords * sizeof(float), st) != hipSuccess ||
-        hipMemsetAsync(G->gridG(), 0, kGuideGridParams * sizeof(float), st) != hipSuccess) {
+        hipMemsetAsync(G->scalars, 0, kScalarWords * sizeof(float), st) != hipSuccess) {
         return guideFail(ctx, "kdn_guide_gradients: device call failed", KDN_E_DEVICE);
     }
     hipLaunchKernelGGL(k_guide_set_scale, dim3(1), dim3(64), 0, st, G->scalars, 1.0f);
-    if (!launchGuideTrain(G->view, s, n, 0, n, G->scalars, (float*)G->partial.ptr, G->gridG(), G->scalars + 1, st)) {
+    if (!launchGuideTrain(G->view, s, n, 0, n, G->scalars, (float*)G->partial.ptr, G->grid_fixed, G->scalars + 1,
+                          (uint32_t*)(G->scalars + kScalarFlags), st)) {
         return guideFail(ctx, "kdn_guide_gradients: launch failed", KDN_E_DEVICE);
     }
+    hipLaunchKernelGGL(k_guide_unfix, dim3(4096), dim3(256), 0, st, G->grid_fixed, G->gridG(), kGuideGridParams);
     hipLaunchKernelGGL(k_guide_reduce, dim3(kGuideMlpParams / 256), dim3(256), 0, st, (const float*)G->partial.ptr, tiles, G->mlpG(),
                        (uint32_t*)(G->scalars + kScalarFlags));
     if (hipGetLastError() != hipSuccess ||

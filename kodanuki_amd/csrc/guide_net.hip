@@ -94,14 +94,16 @@ __device__ __forceinline__ void store_f32(const f32x4 acc[16], float* O) {
         for (int j = 0; j < 4; ++j) O[d_row(t, j) * kOutLd + d_col(t)] = acc[t][j];
 }
 
-// Adds (v0, v1) to base[2 idx], base[2 idx + 1] for the active lanes of
-// a wavefront (all 64 lanes call it). Lanes with the same entry are summed
-// first, a leader adding the sum, while the first remaining lane's entry
-// is shared by another; the rest add their own. The global atomics are
-// executed far from the CU: the nearby samples of a tile share the coarse
-// levels' entries, and one atomic a group instead of one a lane is what
-// keeps training from waiting on them.
-__device__ __forceinline__ void wave_scatter_add(float* base, uint32_t idx, float v0, float v1, bool active) {
+// Adds (v0, v1) to base[2 idx], base[2 idx + 1] (fixed point, fixed_of)
+// for the active lanes of a wavefront (all 64 lanes call it). Lanes with
+// the same entry are summed first (a butterfly in a fixed order), a leader
+// adding the sum, while the first remaining lane's entry is shared by
+// another; the rest add their own. The global atomics are executed far
+// from the CU: the nearby samples of a tile share the coarse levels'
+// entries, and one atomic a group instead of one a lane is what keeps
+// training from waiting on them. A wild contribution raises *flag.
+__device__ __forceinline__ void wave_scatter_add(uint64_t* base, uint32_t idx, float v0, float v1, bool active, uint32_t* flag) {
+    bool wild = false;
     const int lane = threadIdx.x & 63;
     uint64_t pending = __ballot(active);
     while (pending) {
@@ -116,15 +118,16 @@ __device__ __forceinline__ void wave_scatter_add(float* base, uint32_t idx, floa
             s1 += __shfl_xor(s1, off);
         }
         if (lane == leader) {
-            atomicAdd(&base[(size_t)a * 2], s0);
-            atomicAdd(&base[(size_t)a * 2 + 1], s1);
+            atomicAdd((unsigned long long*)&base[(size_t)a * 2], (unsigned long long)fixed_of(s0, &wild));
+            atomicAdd((unsigned long long*)&base[(size_t)a * 2 + 1], (unsigned long long)fixed_of(s1, &wild));
         }
         pending &= ~same;
     }
     if ((pending >> lane) & 1) {
-        atomicAdd(&base[(size_t)idx * 2], v0);
-        atomicAdd(&base[(size_t)idx * 2 + 1], v1);
+        atomicAdd((unsigned long long*)&base[(size_t)idx * 2], (unsigned long long)fixed_of(v0, &wild));
+        atomicAdd((unsigned long long*)&base[(size_t)idx * 2 + 1], (unsigned long long)fixed_of(v1, &wild));
     }
+    if (wild) atomicOr(flag, 1u);
 }
 
 // The layers' BF16 weights into LDS, eight at a time: Wt[layer][out][in]
@@ -228,9 +231,10 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideView g, Q
 // samples in turn (forward, the loss gradient of each sample, its weight
 // times *scale, backward) and sums its tiles' weight gradients into
 // partial[block][layer][in][out]; grid gradients are added to grid_grad
-// and each tile's loss to *loss.
+// in fixed point (a wild one raises *flag) and each tile's loss to *loss.
 __global__ __launch_bounds__(64) void k_guide_train(GuideView g, const float* samples, uint32_t n, uint32_t first, uint32_t count,
-                                                    const float* scale, float* partial, float* grid_grad, float* loss) {
+                                                    const float* scale, float* partial, uint64_t* grid_grad, float* loss,
+                                                    uint32_t* flag) {
     __shared__ __attribute__((aligned(16))) uint16_t Wt[kGuideLayers][64 * kLd];  // [out][in]
     __shared__ __attribute__((aligned(16))) uint16_t Wn[kGuideLayers][64 * kLd];  // [in][out]
     __shared__ __attribute__((aligned(16))) uint16_t H[kGuideLayers][64 * kLd];   // inputs, h1, h2, h3
@@ -304,8 +308,8 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideView g, const float* sa
                 grid_corners(g, level, p, idx, wt);
                 const float d0 = O[lane * kOutLd + kInGrid + 2 * level], d1 = O[lane * kOutLd + kInGrid + 2 * level + 1];
                 const bool active = live && (d0 != 0 || d1 != 0);
-                float* table = grid_grad + (size_t)level * kGuideTable * kGuideFeatures;
-                for (int c = 0; c < 8; ++c) wave_scatter_add(table, idx[c], wt[c] * d0, wt[c] * d1, active);
+                uint64_t* table = grid_grad + (size_t)level * kGuideTable * kGuideFeatures;
+                for (int c = 0; c < 8; ++c) wave_scatter_add(table, idx[c], wt[c] * d0, wt[c] * d1, active, flag);
             }
         }
         __syncthreads();
@@ -333,10 +337,10 @@ bool launchGuideInferPaths(const GuideView& g, const PassParams& P, uint32_t que
 uint32_t guideTrainPartials(uint32_t count) { return std::min<uint32_t>((count + 63) / 64, kTrainBlocks); }
 
 bool launchGuideTrain(const GuideView& g, const float* samples, uint32_t n, uint32_t first, uint32_t count, const float* scale,
-                      float* partial, float* gridGrad, float* loss, hipStream_t st) {
+                      float* partial, uint64_t* gridGrad, float* loss, uint32_t* flag, hipStream_t st) {
     if (count == 0) return true;
     hipLaunchKernelGGL(k_guide_train, dim3(guideTrainPartials(count)), dim3(64), 0, st, g, samples, n, first, count, scale, partial,
-                       gridGrad, loss);
+                       gridGrad, loss, flag);
     return hipGetLastError() == hipSuccess;
 }
 

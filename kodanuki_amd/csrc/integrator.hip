@@ -102,6 +102,15 @@ __global__ void k_job_keys(PassParams P, const uint32_t* queue, uint32_t count) 
     P.s.keys[q] = P.slices[P.s.slice[queue[q]]].job;
 }
 
+// where each job's stretch of a queue grouped by job begins (starts[job]
+// untouched for a job with no path in it)
+__global__ void k_job_starts(PassParams P, const uint32_t* queue, uint32_t count, uint32_t* starts) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    const uint32_t job = P.slices[P.s.slice[queue[q]]].job;
+    if (q == 0 || P.slices[P.s.slice[queue[q - 1]]].job != job) starts[job] = q;
+}
+
 // estimates[pixel] folded with the pixel's spp colours, in sample order
 __global__ void k_fold_samples(const double* colours, uint32_t pixels, uint32_t spp, uint32_t batch, kdn_estimate* estimates) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -291,7 +300,8 @@ static uint32_t residentTraceBlocks(const Ctx& c) {
     return (uint32_t)units * 4 * kTraceWaves * 64 / kTraceBlock;
 }
 
-bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDev>& slices, uint32_t flags, kdn_pass_stats* stats) {
+bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDev>& slices, uint32_t flags, kdn_pass_stats* stats,
+              const std::vector<void*>* guides) {
     if (!ensureColour(c)) return false;
     hipStream_t st = c.stream;
     const SliceDev& lastSlice = slices.back();
@@ -307,8 +317,6 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
         c.round_bytes = roundBytes;
     }
     char* table = (char*)c.round_dev;
-    KDN_HIP(hipMemcpyAsync(table, jobs.data(), jobBytes, hipMemcpyHostToDevice, st));
-    KDN_HIP(hipMemcpyAsync(table + ((jobBytes + 255) & ~size_t(255)), slices.data(), sliceBytes, hipMemcpyHostToDevice, st));
     PassParams P{};
     P.jobs = (const JobDev*)table;
     P.slices = (const SliceDev*)(table + ((jobBytes + 255) & ~size_t(255)));
@@ -316,10 +324,46 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
     P.paths = total;
     P.colour = (const ColourTables*)c.colour;
     P.s = *c.paths;
-    P.job0 = jobs[0];
     PathState* s = c.paths;
 
     const bool timing = (flags & 2) != 0, counting = (flags & 4) != 0, finishing = (flags & 8) == 0, guiding = (flags & 16) != 0;
+    // the learned guide: the context's own (a render pass, flags bit 4) or
+    // each job's own (serving), with the jobs' fields that place its lobes
+    // and training records
+    std::vector<JobDev> J(jobs);
+    bool ownGuides = false;
+    if (guides) {
+        if (guides->size() != jobs.size()) {
+            c.error = "a round's guides are one for each job";
+            return false;
+        }
+        for (void* g : *guides) ownGuides = ownGuides || g != nullptr;
+    }
+    if (guiding && ownGuides) {
+        c.error = "a round's learned guides are the context's or the jobs' own, not both";
+        return false;
+    }
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        if (jobs[j].hg && (guiding || (ownGuides && (*guides)[j]))) {
+            c.error = "a round guides its paths by the learned guide or by the plain guide, not both";
+            return false;
+        }
+    }
+    if (guiding) {
+        if (!guideReady(c) && !guideReset(c, 1, c.tree.lower, c.tree.size)) return false;
+        if (!guideBeginRound(c, P, total)) return false;
+        for (JobDev& j : J) {
+            j.learned = P.lobes != nullptr ? 1u : 0u;
+            j.train_stride = P.train_stride;
+            j.path_first = 0;
+            j.rec_first = 0;
+        }
+    }
+    if (ownGuides && !guidesBeginRound(c, P, J, slices, *guides)) return false;
+    const bool learnedRound = guiding || ownGuides;
+    KDN_HIP(hipMemcpyAsync(table, J.data(), jobBytes, hipMemcpyHostToDevice, st));
+    KDN_HIP(hipMemcpyAsync(table + ((jobBytes + 255) & ~size_t(255)), slices.data(), sliceBytes, hipMemcpyHostToDevice, st));
+    P.job0 = J[0];
     // kernel timing: an event pair per launch, read after each bounce's sync
     // (destroyed on every way out of the round)
     struct EventPair {
@@ -353,18 +397,10 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
         KDN_HIP(hipMemsetAsync(work, 0, 32, st));
     }
 
-    if (guiding) {
-        for (const JobDev& J : jobs) {
-            if (J.hg) {
-                c.error = "a round guides its paths by the learned guide or by the plain guide, not both";
-                return false;
-            }
-        }
-    }
     // a round with no guide of either kind and no training records runs the
     // finishing launch with the guided sampling compiled out
-    bool plain = !guiding;
-    for (const JobDev& J : jobs) plain = plain && J.hg == nullptr && J.hg_deep == nullptr;
+    bool plain = !learnedRound;
+    for (const JobDev& j : jobs) plain = plain && j.hg == nullptr && j.hg_deep == nullptr;
     // a queue's rays ordered by direction octant and origin cell, so that a
     // wave's lanes walk nearby cells alike (each path's result does not
     // depend on where it stands in a queue)
@@ -381,29 +417,74 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
         if (!ok && c.error.empty()) c.error = "sorting a ray queue failed";
         return ok;
     };
+    // a queue grouped by job (a stable sort: each job's paths keep their
+    // order): each job's first entry and count
+    auto groupByJob = [&](uint32_t qi, uint32_t count, std::vector<uint32_t>& first, std::vector<uint32_t>& cnt) -> bool {
+        const uint32_t nJobs = (uint32_t)jobs.size();
+        first.assign(nJobs, 0);
+        cnt.assign(nJobs, 0);
+        int bits = 1;
+        while ((1u << bits) < nJobs) ++bits;
+        uint32_t* queue = s->queue[qi];
+        hipLaunchKernelGGL(k_job_keys, dim3((count + 255) / 256), dim3(256), 0, st, P, (const uint32_t*)queue, count);
+        uint32_t* sorted = nullptr;
+        if (!sortPairsU32(s->keys, s->keys_alt, queue, s->vals_alt, count, bits, c.sort_tmp, st, c.error, &sorted)) return false;
+        if (sorted != queue) KDN_HIP(hipMemcpyAsync(queue, sorted, (size_t)count * 4, hipMemcpyDeviceToDevice, st));
+        if (!ensure(c.job_counters, (size_t)nJobs * 68 + 64, c.error, false)) return false;
+        uint32_t* starts = (uint32_t*)c.job_counters.ptr + (size_t)nJobs * 16;
+        KDN_HIP(hipMemsetAsync(starts, 0xFF, (size_t)nJobs * 4, st));
+        hipLaunchKernelGGL(k_job_starts, dim3((count + 255) / 256), dim3(256), 0, st, P, (const uint32_t*)queue, count, starts);
+        KDN_HIP(hipGetLastError());
+        std::vector<uint32_t> at(nJobs);
+        KDN_HIP(hipMemcpyAsync(at.data(), starts, (size_t)nJobs * 4, hipMemcpyDeviceToHost, st));
+        KDN_HIP(hipStreamSynchronize(st));
+        uint32_t end = count;
+        for (uint32_t j = nJobs; j-- > 0;) {
+            if (at[j] == 0xFFFFFFFFu) continue;
+            if (at[j] > end) {
+                c.error = "a queue grouped by job is out of order";
+                return false;
+            }
+            first[j] = at[j];
+            cnt[j] = end - at[j];
+            end = at[j];
+        }
+        return true;
+    };
     // A round of several jobs finishes each job's paths in the launch
     // specialised for one job (its fields from the kernel's arguments, the
     // same for every lane), the jobs' launches side by side on the job
     // streams: the queue, sorted, is grouped by job with a stable sort (so
     // each job's paths keep their order), each job gets its stretch of it
-    // and counters of its own. Every live path of a plain slice is queued
-    // (spp x width x height of them), so the stretches are known here.
+    // and counters of its own. At the camera rays every path of a plain
+    // slice is queued (spp x width x height of them), so the stretches are
+    // known here; deeper (a learned round finishes after its guided depths)
+    // they are read from the grouped queue.
     bool everySlicePlain = true;
     for (const SliceDev& sl : slices) everySlicePlain = everySlicePlain && sl.offsets == nullptr;
-    auto finishJobsApart = [&](uint32_t qi, uint32_t count) -> bool {
+    auto finishJobsApart = [&](uint32_t qi, uint32_t count, uint32_t bounce) -> bool {
         const uint32_t nJobs = (uint32_t)jobs.size();
         std::vector<uint64_t> live(nJobs, 0);
-        for (const SliceDev& sl : slices) live[sl.job] += (uint64_t)sl.spp * jobs[sl.job].width * jobs[sl.job].height;
+        if (bounce > 0) {
+            std::vector<uint32_t> first, cnt;
+            bool grouped = true;
+            timed(&stats->other_ms, [&] { grouped = groupByJob(qi, count, first, cnt); });
+            if (!grouped) return false;
+            for (uint32_t j = 0; j < nJobs; ++j) live[j] = cnt[j];
+        } else {
+            for (const SliceDev& sl : slices) live[sl.job] += (uint64_t)sl.spp * jobs[sl.job].width * jobs[sl.job].height;
+        }
         uint64_t sum = 0;
         for (uint64_t n : live) sum += n;
         if (sum != count) {
-            c.error = "a round's queue holds " + std::to_string(count) + " paths, its jobs' slices " + std::to_string(sum);
+            c.error = "a round's queue holds " + std::to_string(count) + " paths, its jobs' stretches " + std::to_string(sum);
             return false;
         }
         int bits = 1;
         while ((1u << bits) < nJobs) ++bits;
         bool ok = true;
-        timed(&stats->other_ms, [&] {
+        // (deeper, groupByJob has grouped the queue already)
+        if (bounce == 0) timed(&stats->other_ms, [&] {
             uint32_t* queue = s->queue[qi];
             hipLaunchKernelGGL(k_job_keys, dim3((count + 255) / 256), dim3(256), 0, st, P, (const uint32_t*)queue, count);
             uint32_t* sorted = nullptr;
@@ -432,7 +513,7 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
                 PassParams Pj = P;
                 Pj.s.queue[qi] = s->queue[qi] + first;
                 Pj.s.counters = jc + (size_t)j * 16;
-                Pj.job0 = jobs[j];
+                Pj.job0 = J[j];
                 first += live[j];
                 if (live[j] == 0) continue;
                 hipStream_t js = c.job_streams[j % Ctx::kJobStreams];
@@ -464,10 +545,6 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
         stats->finish_launches += nJobs;
         return true;
     };
-    if (guiding) {
-        if (!guideReady(c) && !guideReset(c, 1, c.tree.lower, c.tree.size)) return false;
-        if (!guideBeginRound(c, P, total)) return false;
-    }
     KDN_HIP(hipMemsetAsync(s->counters, 0, 64, st));
     timed(&stats->other_ms, [&] { hipLaunchKernelGGL(k_generate, dim3((total + 255) / 256), dim3(256), 0, st, P); });
     uint32_t in = 0, out = 1;
@@ -480,13 +557,13 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
         if (count == 0) break;
         // (a guided round finishes paths only past the guided depths: the
         // finishing launch shades without the network's lobes)
-        if (finishing && (!guiding || bounce >= (uint32_t)kGuideDepths) && (uint64_t)count * kFinishShare <= total) {
+        if (finishing && (!learnedRound || bounce >= (uint32_t)kGuideDepths) && (uint64_t)count * kFinishShare <= total) {
             // the remaining paths to their ends in one launch
             // (camera rays too: grouped by direction octant, the finishing
             // lanes ran 1.05x faster at 1080p than in the 8x8 tiles' order)
             if (!sortQueue(in, count, 0)) return false;
             if (jobs.size() > 1 && !counting && everySlicePlain) {
-                if (!finishJobsApart(in, count)) return false;
+                if (!finishJobsApart(in, count, bounce)) return false;
                 break;
             }
             KDN_HIP(hipMemsetAsync(&s->counters[4], 0, 12, st));
@@ -530,10 +607,28 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
                 }
             }
         });
-        if (guiding && bounce < (uint32_t)kGuideDepths) {
+        if (learnedRound && bounce < (uint32_t)kGuideDepths) {
             bool ok = true;
             double ms = 0;
-            timed(&ms, [&] { ok = guideInferQueue(c, P, in, count); });
+            if (guiding) {
+                timed(&ms, [&] { ok = guideInferQueue(c, P, in, count); });
+            } else if (P.lobes) {
+                // each learned job's stretch of the queue by its own guide
+                std::vector<uint32_t> first(jobs.size(), 0), cnt(jobs.size(), 0);
+                if (jobs.size() == 1) {
+                    cnt[0] = count;
+                } else {
+                    double sortMs = 0;
+                    timed(&sortMs, [&] { ok = groupByJob(in, count, first, cnt); });
+                    stats->other_ms += sortMs;
+                    if (!ok) return false;
+                }
+                timed(&ms, [&] {
+                    for (size_t j = 0; j < jobs.size() && ok; ++j) {
+                        if ((*guides)[j] && J[j].learned) ok = guidesInferQueue(c, P, in, first[j], cnt[j], (*guides)[j]);
+                    }
+                });
+            }
             if (!ok) return false;
             stats->guide_ms += ms;
             stats->guide_infer_ms += ms;
@@ -571,9 +666,9 @@ bool runRound(Ctx& c, const std::vector<JobDev>& jobs, const std::vector<SliceDe
         const uint32_t plane = jobs[slices[k].job].plane;
         timed(&stats->other_ms, [&] { hipLaunchKernelGGL(k_fold, dim3((plane + 255) / 256), dim3(256), 0, st, P, k); });
     }
-    if (guiding) {
+    if (learnedRound) {
         bool ok = true;
-        timed(&stats->guide_ms, [&] { ok = guideEndRound(c, P, total, stats); });
+        timed(&stats->guide_ms, [&] { ok = guiding ? guideEndRound(c, P, total, stats) : guidesEndRound(c, P, J, slices, *guides, stats); });
         if (!ok) return false;
     }
     KDN_HIP(hipStreamSynchronize(st));

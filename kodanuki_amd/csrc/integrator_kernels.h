@@ -58,6 +58,12 @@ struct JobDev {
     // and the survey's highlights filed by cell, steering deeper vertices
     const float* hg;
     const HgDeepHeader* hg_deep;
+    // the learned guide: its lobes steer this job's guided vertices
+    // (learned 0: BSDF sampling), and every train_stride-th of the job's
+    // paths from path_first on keeps training records, the path
+    // path_first + k * train_stride in record block rec_first + k
+    // (train_stride 0: no records)
+    uint32_t learned, train_stride, path_first, rec_first;
 };
 
 // A slice of a round: samples [first_sample, first_sample + spp) of every
@@ -93,9 +99,11 @@ struct PassParams {
     const ColourTables* colour;
     PathState s;
     // the learned guide: lobes and selection probability of the guided
-    // vertices (lane-major, kGuideOutputs per path; null: BSDF sampling
-    // only) and training records (kGuideDepths for each path of
-    // train_stride; null: none kept)
+    // vertices (lane-major, kGuideOutputs per path, read for the paths of
+    // jobs with learned set; null: BSDF sampling only) and training records
+    // (blocks of kGuideDepths, placed by each job's train_stride,
+    // path_first and rec_first; null: none kept); train_stride: the one
+    // job's stride of a render pass's guided round
     const float* lobes;
     float* recs;
     uint32_t train_stride;
@@ -252,7 +260,7 @@ KDN_D bool guide_vertex_at(const PassParams& P, uint32_t i, float x[3], float ou
 }
 
 KDN_D bool guided_vertex(const PassParams& P, uint32_t i, float x[3], float out[3], float normal[3], float* nm) {
-    return P.lobes != nullptr && guide_vertex_at(P, i, x, out, normal, nm);
+    return P.lobes != nullptr && job_of(P, i).learned != 0 && guide_vertex_at(P, i, x, out, normal, nm);
 }
 
 KDN_D void store_lobes(float* lobes, size_t N, uint32_t i, const GuideLobes& L) {
@@ -453,7 +461,9 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
     const float tHit = hit.kind == KDN_HIT_MISSED ? kInf : hit.distance;
     const bool exactBefore = (meta & FlagExact) != 0;
     const float pdfBefore = s.pdf[i];
-    const bool training = !Plain && P.recs != nullptr && i % P.train_stride == 0;
+    const bool training = !Plain && P.recs != nullptr && J.train_stride != 0 && i >= J.path_first &&
+                          (i - J.path_first) % J.train_stride == 0;
+    const size_t recBlock = training ? (size_t)J.rec_first + (i - J.path_first) / J.train_stride : 0;
     // the frame record's first surface of the pixel's last sample
     if (depth == 0 && J.firsts && sample == J.record_sample && s.pixel[i] != 0xFFFFFFFFu)
         J.firsts[s.pixel[i]] = surface_met(hit, ray, J.leaf_material, J.materials, J.n_materials);
@@ -490,7 +500,7 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
     // guided vertex, so the guide learns the light that aiming misses)
     // closes the record of the vertex before
     if (training && depth >= 1 && depth - 1 < kGuideDepths) {
-        float* rec = P.recs + ((size_t)(i / P.train_stride) * kGuideDepths + depth - 1) * kRecFloats;
+        float* rec = P.recs + (recBlock * kGuideDepths + depth - 1) * kRecFloats;
         if (rec[kRecState] == 1.0f) {
             for (int l = 0; l < kLanes; ++l) rec[kRecSnap + l] = rad[l];
             rec[kRecState] = 3.0f;
@@ -519,7 +529,7 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
         const bool exact = m.kind == KDN_GLASS && m.exact;
         // the guide's lobes at this vertex (the same test as its inference)
         const bool guidable = depth < kGuideDepths && m.kind != KDN_GLASS;
-        const bool guided = !Plain && P.lobes != nullptr && guidable;
+        const bool guided = !Plain && P.lobes != nullptr && J.learned != 0 && guidable;
         GuideLobes lobes;
         if (guided) load_lobes(P.lobes, N, i, lobes);
         const float share = guided ? lobes.select : 0.0f;
@@ -680,7 +690,7 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
                         meta = (meta & ~(0xFFu | FlagExact)) | (uint32_t)(depth + 1) | (sc.exact ? FlagExact : 0u);
                         alive = true;
                         if (training && guidable) {
-                            float* rec = P.recs + ((size_t)(i / P.train_stride) * kGuideDepths + depth) * kRecFloats;
+                            float* rec = P.recs + (recBlock * kGuideDepths + depth) * kRecFloats;
                             const float q[14] = {x.x, x.y, x.z, out.x, out.y, out.z, n.x, n.y, n.z, w.nm[0], sc.in.x, sc.in.y, sc.in.z, sc.density};
                             for (int k = 0; k < 14; ++k) rec[k] = q[k];
                             for (int l = 0; l < kLanes; ++l) {

@@ -57,6 +57,7 @@ struct Job {
     uint32_t* accumN = nullptr;
     DevBuf hg;             // the plain guide's pixel guides and deep filing (flags bit 0)
     size_t hgDeep = 0;     // the filing's byte offset in hg (0: none)
+    void* guide = nullptr; // the job's own learned guide (flags bit 1)
     uint64_t bytes = 0;
 };
 
@@ -162,11 +163,12 @@ int tooBig(kdn_server& s, uint64_t need) {
 // may pass the budget while it runs).
 int admit(kdn_server& s, kdn::Job& j) {
     const uint64_t px = (uint64_t)j.desc.width * j.desc.height;
-    const bool guided = (j.desc.flags & 1) != 0;
+    const bool guided = (j.desc.flags & 1) != 0, learned = (j.desc.flags & 2) != 0;
     // (a guided job's pixel guides, and its survey points' filing: about
-    // 1 KB a point, one point to 64 pixels, and the 32^3 cell table)
+    // 1 KB a point, one point to 64 pixels, and the 32^3 cell table; a
+    // learned job's network, its Adam moments and gradients)
     const uint64_t bytes = px * (6 * sizeof(double) + sizeof(uint32_t) + (guided ? kdn::kHgFloats * sizeof(float) + 16 : 0)) +
-                           (guided ? (1u << 18) : 0);
+                           (guided ? (1u << 18) : 0) + (learned ? kdn::guideBytes() : 0);
     bool built = false;
     if (!s.scenes.count(j.key)) {
         auto known = s.sceneBytes.find(j.key);
@@ -220,6 +222,17 @@ int admit(kdn_server& s, kdn::Job& j) {
             return failed(&s, message, KDN_E_DEVICE);
         }
     }
+    if (learned) {
+        // the job's own guide over its scene's region, from the job's seed
+        const kdn::Ctx& sc = res.ctx->c;
+        j.guide = kdn::guideOpen(c, j.desc.seed, sc.tree.lower, sc.tree.size);
+        if (!j.guide) {
+            const std::string message = "serving: the learned guide could not be made: " + c.error;
+            releaseJob(s, j);
+            res.users -= 1;
+            return failed(&s, message, KDN_E_MEMORY);
+        }
+    }
     const uint32_t plane = kdn::planeOf(j.desc.width, j.desc.height);
     // fixed by the job and the server alone, never by the other jobs
     const uint64_t slicePaths = std::min<uint64_t>(kdn::kSlicePaths, s.maxPaths);
@@ -238,6 +251,8 @@ void releaseJob(kdn_server& s, kdn::Job& j) {
     if (j.hg.ptr) hipFree(j.hg.ptr);
     j.hg = kdn::DevBuf{};
     j.hgDeep = 0;
+    if (j.guide) kdn::guideClose(j.guide);
+    j.guide = nullptr;
     j.bytes = 0;
 }
 
@@ -282,9 +297,12 @@ int kdn_serve_open(kdn_ctx* ctx, uint64_t budget_bytes, uint32_t max_paths, kdn_
 int kdn_serve_submit(kdn_server* s, const kdn_job_desc* job, uint64_t* id) {
     if (!s || !job || !job->scene || !id) return failed(s, "kdn_serve_submit: null argument", KDN_E_ARGUMENT);
     if (job->width <= 0 || job->height <= 0 || job->width > 16384 || job->height > 16384 || job->spp == 0 || job->surfaces == 0 ||
-        job->depth < 1 || job->depth > kdn::kMaxDepth || (job->flags & ~1u) != 0) {
+        job->depth < 1 || job->depth > kdn::kMaxDepth || (job->flags & ~3u) != 0) {
         return failed(s, "kdn_serve_submit: image size out of 1..16384, no samples, no surfaces, depth cap out of 1..11 or "
                          "unknown flags", KDN_E_ARGUMENT);
+    }
+    if ((job->flags & 3u) == 3u) {
+        return failed(s, "kdn_serve_submit: a job is guided by the plain guide or by the learned guide, not both", KDN_E_ARGUMENT);
     }
     if ((uint64_t)kdn::planeOf(job->width, job->height) > s->maxPaths) {
         return failed(s, "kdn_serve_submit: one sample of the image needs more paths than a round holds", KDN_E_ARGUMENT);
@@ -327,6 +345,7 @@ int kdn_serve_step(kdn_server* s, kdn_serve_stats* stats) {
         std::vector<kdn::JobDev> jobs;
         std::vector<kdn::SliceDev> slices;
         std::vector<uint64_t> owners;   // job id of each JobDev
+        std::vector<void*> guides;      // each JobDev's own learned guide (null: none)
         std::vector<uint32_t> nextOf(s->running.size());
         for (size_t r = 0; r < s->running.size(); ++r) nextOf[r] = s->jobs[s->running[r]].next;
         std::vector<int> jobIndex(s->running.size(), -1);
@@ -340,6 +359,10 @@ int kdn_serve_step(kdn_server* s, kdn_serve_stats* stats) {
                 const size_t r = (first + turn) % n;
                 kdn::Job& j = s->jobs[s->running[r]];
                 if (nextOf[r] >= j.desc.spp) continue;
+                // a learned job's slices one a round: each slice is steered
+                // by the guide as trained on the slices before it, alone or
+                // in a mix
+                if (j.guide && jobIndex[r] >= 0) continue;
                 const uint32_t plane = kdn::planeOf(j.desc.width, j.desc.height);
                 const uint32_t spp = std::min(j.sliceSpp, j.desc.spp - nextOf[r]);
                 if (paths + (uint64_t)spp * plane > s->maxPaths) continue;
@@ -350,6 +373,7 @@ int kdn_serve_step(kdn_server* s, kdn_serve_stats* stats) {
                     jobs.back().hg = (const float*)j.hg.ptr;
                     jobs.back().hg_deep = j.hgDeep ? (const kdn::HgDeepHeader*)((const char*)j.hg.ptr + j.hgDeep) : nullptr;
                     owners.push_back(j.id);
+                    guides.push_back(j.guide);
                 }
                 slices.push_back(kdn::SliceDev{(uint32_t)jobIndex[r], nextOf[r], spp, (uint32_t)paths, j.accum, j.accumSq, j.accumN, nullptr, 0});
                 paths += (uint64_t)spp * plane;
@@ -358,7 +382,7 @@ int kdn_serve_step(kdn_server* s, kdn_serve_stats* stats) {
             }
         }
         if (!slices.empty()) {
-            if (!kdn::runRound(c, jobs, slices, 0, &local.pass)) return failed(s, "serving: " + c.error, KDN_E_DEVICE);
+            if (!kdn::runRound(c, jobs, slices, 0, &local.pass, &guides)) return failed(s, "serving: " + c.error, KDN_E_DEVICE);
             ++s->rounds;
             local.jobs_in_round = (uint32_t)jobs.size();
             local.slices_in_round = (uint32_t)slices.size();

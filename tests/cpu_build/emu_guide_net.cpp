@@ -77,7 +77,7 @@ bool launchGuideInferPaths(const GuideView& g, const PassParams& P, uint32_t que
 uint32_t guideTrainPartials(uint32_t count) { return (count + 63) / 64; }  // one partial sum a tile here
 
 bool launchGuideTrain(const GuideView& g, const float* samples, uint32_t n, uint32_t first, uint32_t count, const float* scale,
-                      float* partial, float* gridGrad, float* loss, hipStream_t) {
+                      float* partial, uint64_t* gridGrad, float* loss, uint32_t* flag, hipStream_t) {
     const uint32_t tiles = (count + 63) / 64;
     for (uint32_t tile = 0; tile < tiles; ++tile) {
         float* P = partial + (size_t)tile * kGuideMlpParams;
@@ -121,12 +121,14 @@ bool launchGuideTrain(const GuideView& g, const float* samples, uint32_t n, uint
                 uint32_t idx[8];
                 float wt[8];
                 grid_corners(g, level, p, idx, wt);
-                float* table = gridGrad + (size_t)level * kGuideTable * kGuideFeatures;
+                uint64_t* table = gridGrad + (size_t)level * kGuideTable * kGuideFeatures;
+                bool wild = false;
                 for (int c = 0; c < 8; ++c) {
                     KDN_CHECK(idx[c] < kGuideTable);
-                    table[(size_t)idx[c] * 2] += wt[c] * dIn[kInGrid + 2 * level];
-                    table[(size_t)idx[c] * 2 + 1] += wt[c] * dIn[kInGrid + 2 * level + 1];
+                    table[(size_t)idx[c] * 2] += fixed_of(wt[c] * dIn[kInGrid + 2 * level], &wild);
+                    table[(size_t)idx[c] * 2 + 1] += fixed_of(wt[c] * dIn[kInGrid + 2 * level + 1], &wild);
                 }
+                if (wild) *flag |= 1u;
             }
         }
     }

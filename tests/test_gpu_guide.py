@@ -195,6 +195,17 @@ def test_guided_passes_train_and_sample(device):
     assert abs(guided.mean() / plain.mean() - 1) < 0.25
 
 
+def test_guided_passes_are_reproducible(device):
+    """Guided passes from the same reset give the same image to the bit:
+    the grid gradients are summed in fixed point (any order of the
+    training's atomics gives the same sums), the layers' in a fixed order."""
+    text = scenes.window_room(24, 16)
+    first, sq1, _, st = _render(device, text, 6, 16, 3, k.PASS_GUIDE, reset=5)
+    again, sq2, _, _ = _render(device, text, 6, 16, 3, k.PASS_GUIDE, reset=5)
+    assert sum(x.guide_steps for x in st) > 0
+    assert first.tobytes() == again.tobytes() and sq1.tobytes() == sq2.tobytes()
+
+
 @pytest.mark.skipif(bool(EMULATION), reason="thousands of samples per pixel: device only")
 def test_guided_render_is_unbiased_and_gains(device):
     text = scenes.window_room(256, 192)
