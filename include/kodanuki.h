@@ -220,6 +220,12 @@ int kdn_scene_tape(const kdn_scene* scene, kdn_step* steps, kdn_object* objects)
 /* The same tape with its constants in double, for FP64 checks. */
 int kdn_scene_tape_values(const kdn_scene* scene, double* values);
 int kdn_scene_leaves(const kdn_scene* scene, kdn_leaf* leaves);
+/* One leaf alone over world space: leafProgram (source/sdf/scene.cpp:
+ * 284-288), the program the survey's chains walk. Writes at most cap steps
+ * (their constants in double in values) and the step count and result
+ * register; KDN_E_ARGUMENT for a leaf out of range or cap too small. */
+int kdn_scene_leaf_program(const kdn_scene* scene, uint32_t leaf, kdn_step* steps, double* values, uint32_t cap, uint32_t* count,
+                           uint32_t* result);
 int kdn_scene_nodes(const kdn_scene* scene, kdn_node* nodes, uint32_t* top_nodes, uint32_t* children);
 int kdn_scene_materials(const kdn_scene* scene, kdn_material* materials);
 int kdn_scene_lights(const kdn_scene* scene, kdn_light* lights);
@@ -812,6 +818,41 @@ int kdn_highlight_guide_at(kdn_ctx* ctx, int32_t x, int32_t y, float* packed);
 int kdn_highlight_density(kdn_ctx* ctx, const float* packed, const float place[3], const float normal[3], const float out[3],
                           const float* dirs, uint32_t count, float* density);
 int kdn_highlight_draw(kdn_ctx* ctx, const float* packed, const float place[3], const float* numbers, uint32_t count, float* drawn);
+
+/* A chain of the survey: from a start (with two directions across its
+ * surface, along which the highlight's motion is measured), by one bounce
+ * off a shiny leaf of facet width `width`, to a lamp, walked from a seed
+ * point on the leaf (BounceChain, plugins/sampler/highlight_guide/
+ * chain.h:15-39). */
+typedef struct kdn_chain_query {
+    double from[3];
+    double sideways[2][3];
+    double seed[3];
+    double width;
+    uint32_t leaf;
+    uint32_t light;          /* index into the scene's lights */
+} kdn_chain_query;
+
+/* Where the bounce settled and how the highlight spreads and moves there
+ * (SettledChain, chain.h:44-74): found 0 leaves the rest zero; spread is
+ * the upper triangle (xx, xy, yy) on the plane one unit along the axis,
+ * across/up its basis; motion[k] is the point's move per unit the start
+ * moves along sideways[k]. */
+typedef struct kdn_chain {
+    uint32_t found;
+    uint32_t pad;
+    double point[3];
+    double axis[3];
+    double across[3];
+    double up[3];
+    double spread[3];
+    double motion[2][3];
+} kdn_chain;
+
+/* Walk n chains over the uploaded scene's leaves on the device, in FP64:
+ * walkedChain (chain.h:76-97, chain.cpp:507-519), the walk the survey
+ * places its highlights with. */
+int kdn_highlight_chains(kdn_ctx* ctx, const kdn_chain_query* queries, uint32_t n, kdn_chain* chains);
 
 #ifdef __cplusplus
 }

@@ -284,13 +284,19 @@ NODE_DTYPE = np.dtype([("level", "<u4"), ("x", "<u4"), ("y", "<u4"), ("z", "<u4"
                        ("entry_first", "<u4"), ("entry_count", "<u4")])
 ENTRY_DTYPE = np.dtype([("object", "<u4"), ("program", "<u4")])
 OBJECT_DTYPE = np.dtype([("first", "<u4"), ("count", "<u4"), ("result", "<u4"), ("material", "<u4")])
+# kdn_chain_query and kdn_chain
+CHAIN_QUERY_DTYPE = np.dtype([("from", "<f8", 3), ("sideways", "<f8", (2, 3)), ("seed", "<f8", 3), ("width", "<f8"),
+                              ("leaf", "<u4"), ("light", "<u4")])
+CHAIN_DTYPE = np.dtype([("found", "<u4"), ("pad", "<u4"), ("point", "<f8", 3), ("axis", "<f8", 3), ("across", "<f8", 3),
+                        ("up", "<f8", 3), ("spread", "<f8", 3), ("motion", "<f8", (2, 3))])
 assert STEP_DTYPE.itemsize == 16 and RAY_DTYPE.itemsize == 36 and HIT_DTYPE.itemsize == 48
+assert CHAIN_QUERY_DTYPE.itemsize == 112 and CHAIN_DTYPE.itemsize == 176
 
 # Every symbol include/kodanuki.h declares.
 EXPORTS = [
     "kdn_scene_load", "kdn_scene_free", "kdn_scene_describe", "kdn_scene_tape", "kdn_scene_tape_values",
-    "kdn_scene_leaves", "kdn_scene_nodes", "kdn_scene_materials", "kdn_scene_lights", "kdn_scene_settings",
-    "kdn_scene_region", "kdn_open", "kdn_close", "kdn_last_error", "kdn_global_error", "kdn_upload_scene",
+    "kdn_scene_leaves", "kdn_scene_leaf_program", "kdn_scene_nodes", "kdn_scene_materials", "kdn_scene_lights",
+    "kdn_scene_settings", "kdn_scene_region", "kdn_open", "kdn_close", "kdn_last_error", "kdn_global_error", "kdn_upload_scene",
     "kdn_build_octree", "kdn_octree_sizes", "kdn_octree_download", "kdn_extend", "kdn_extend_device",
     "kdn_render_pass", "kdn_read_accumulation", "kdn_read_image", "kdn_read_estimates", "kdn_fold_samples",
     "kdn_stopping_effort", "kdn_stopping_trial", "kdn_stopping_allot", "kdn_read_stopping", "kdn_sequence_numbers",
@@ -301,7 +307,7 @@ EXPORTS = [
     "kdn_guide_gradients", "kdn_guide_train",
     "kdn_serve_open", "kdn_serve_submit", "kdn_serve_step", "kdn_serve_status", "kdn_serve_result",
     "kdn_serve_release", "kdn_serve_error", "kdn_serve_job_error", "kdn_serve_close",
-    "kdn_highlight_guides", "kdn_highlight_guide_at", "kdn_highlight_density", "kdn_highlight_draw",
+    "kdn_highlight_guides", "kdn_highlight_guide_at", "kdn_highlight_density", "kdn_highlight_draw", "kdn_highlight_chains",
 ]
 
 _lib = None
@@ -332,6 +338,7 @@ def _bind(lib):
         "kdn_scene_tape": (C.c_int, [P, P, P]),
         "kdn_scene_tape_values": (C.c_int, [P, P]),
         "kdn_scene_leaves": (C.c_int, [P, P]),
+        "kdn_scene_leaf_program": (C.c_int, [P, C.c_uint32, P, P, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
         "kdn_scene_nodes": (C.c_int, [P, P, P, P]),
         "kdn_scene_materials": (C.c_int, [P, P]),
         "kdn_scene_lights": (C.c_int, [P, P]),
@@ -372,6 +379,7 @@ def _bind(lib):
         "kdn_highlight_guide_at": (C.c_int, [P, C.c_int32, C.c_int32, P]),
         "kdn_highlight_density": (C.c_int, [P, P, P, P, P, P, C.c_uint32, P]),
         "kdn_highlight_draw": (C.c_int, [P, P, P, P, C.c_uint32, P]),
+        "kdn_highlight_chains": (C.c_int, [P, P, C.c_uint32, P]),
         "kdn_guide_reset": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "kdn_guide_describe": (C.c_int, [P, C.POINTER(GuideInfo)]),
         "kdn_guide_configure": (C.c_int, [P, C.POINTER(GuideConfig)]),
@@ -457,6 +465,17 @@ class Scene:
         arr = (Leaf * max(1, self.info.leaves))()
         _check(library().kdn_scene_leaves(self._h, C.cast(arr, C.c_void_p)))
         return list(arr)[: self.info.leaves]
+
+    def leaf_program(self, leaf):
+        """One leaf alone over world space (the survey's chains walk it):
+        its steps (STEP_DTYPE), their constants in double, its result."""
+        count, result = C.c_uint32(), C.c_uint32()
+        _check(library().kdn_scene_leaf_program(self._h, leaf, None, None, 0, C.byref(count), C.byref(result)))
+        steps = np.zeros(max(1, count.value), STEP_DTYPE)
+        values = np.zeros(max(1, count.value), np.float64)
+        _check(library().kdn_scene_leaf_program(self._h, leaf, _ptr(steps), _ptr(values), count.value, C.byref(count),
+                                                C.byref(result)))
+        return steps[: count.value], values[: count.value], result.value
 
     def nodes(self):
         nodes = (Node * max(1, self.info.nodes))()
@@ -737,6 +756,14 @@ class Device:
         x = np.ascontiguousarray(place, np.float32)
         r = np.zeros((len(q), 3), np.float32)
         _check(library().kdn_highlight_draw(self._h, _ptr(g), _ptr(x), _ptr(q), len(q), _ptr(r)), self._h)
+        return r
+
+    def highlight_chains(self, queries):
+        """Chains (CHAIN_DTYPE) walked on the device for queries
+        (CHAIN_QUERY_DTYPE) over the uploaded scene's leaves."""
+        q = np.ascontiguousarray(queries, CHAIN_QUERY_DTYPE)
+        r = np.zeros(len(q), CHAIN_DTYPE)
+        _check(library().kdn_highlight_chains(self._h, _ptr(q), len(q), _ptr(r)), self._h)
         return r
 
     # ---- the learned guide ----

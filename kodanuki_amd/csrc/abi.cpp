@@ -238,6 +238,10 @@ void releaseAll(Ctx& c) {
     freeIntegrator(c);
     freeGuide(c);
     freeBuf(c.hguides);
+    if (c.leaf_steps_dev) hipFree(c.leaf_steps_dev);
+    if (c.leaf_progs_dev) hipFree(c.leaf_progs_dev);
+    c.leaf_steps_dev = c.leaf_progs_dev = nullptr;
+    c.n_leaf_progs = 0;
     freeBuf(c.sort_tmp);
     for (DevBuf* b : {&c.stop_wanted, &c.stop_spreads, &c.stop_allotted, &c.stop_counts, &c.stop_offsets, &c.stop_tmp, &c.tm_history[0],
                       &c.tm_history[1], &c.tm_gbuffer})
@@ -326,6 +330,23 @@ int kdn_scene_leaves(const kdn_scene* s, kdn_leaf* leaves) {
             for (int c = 0; c < 3; ++c) o.linear[3 * r + c] = l.linear[r][c];
         std::memcpy(o.origin, l.origin, sizeof(o.origin));
         o.stretch = l.stretch;
+    }
+    return KDN_OK;
+}
+
+int kdn_scene_leaf_program(const kdn_scene* s, uint32_t leaf, kdn_step* steps, double* values, uint32_t cap, uint32_t* count,
+                           uint32_t* result) {
+    if (!s || !count || !result) return fail("kdn_scene_leaf_program: null argument", KDN_E_ARGUMENT);
+    if (leaf >= s->scene.leaves.size()) return fail("kdn_scene_leaf_program: no such leaf", KDN_E_ARGUMENT);
+    const kdn::Program prog = kdn::leafProgram(s->scene, leaf);
+    *count = (uint32_t)prog.steps.size();
+    *result = prog.result;
+    if (!steps && !values) return KDN_OK;
+    if (prog.steps.size() > cap) return fail("kdn_scene_leaf_program: more steps than cap", KDN_E_ARGUMENT);
+    for (size_t i = 0; i < prog.steps.size(); ++i) {
+        const kdn::Step& st = prog.steps[i];
+        if (steps) steps[i] = kdn_step{st.op, 0, (uint16_t)st.a, (uint16_t)st.b, (uint16_t)st.c, (float)st.value, st.leaf};
+        if (values) values[i] = st.value;
     }
     return KDN_OK;
 }
@@ -503,6 +524,25 @@ int kdn_upload_scene(kdn_ctx* ctx, const kdn_scene* s) {
         !up(&c.lamps_dev, lamps.data(), lamps.size() * sizeof(kdn::LampDev))) {
         c.error = "kdn_upload_scene: device allocation or copy failed";
         return ctxFail(ctx, KDN_E_MEMORY);
+    }
+    // every leaf's own program, the survey's chains walk them (FP64)
+    {
+        std::vector<kdn::LeafStepDev> lsteps;
+        std::vector<kdn::LeafProgDev> lprogs;
+        for (uint32_t leaf = 0; leaf < (uint32_t)s->scene.leaves.size(); ++leaf) {
+            const kdn::Program prog = kdn::leafProgram(s->scene, leaf);
+            lprogs.push_back(kdn::LeafProgDev{(uint32_t)lsteps.size(), (uint32_t)prog.steps.size(), prog.result, 0});
+            for (const kdn::Step& st : prog.steps) lsteps.push_back(kdn::LeafStepDev{st.op, st.a, st.b, st.c, st.value});
+        }
+        if (c.leaf_steps_dev) hipFree(c.leaf_steps_dev);
+        if (c.leaf_progs_dev) hipFree(c.leaf_progs_dev);
+        c.leaf_steps_dev = c.leaf_progs_dev = nullptr;
+        if (!up(&c.leaf_steps_dev, lsteps.data(), lsteps.size() * sizeof(kdn::LeafStepDev)) ||
+            !up(&c.leaf_progs_dev, lprogs.data(), lprogs.size() * sizeof(kdn::LeafProgDev))) {
+            c.error = "kdn_upload_scene: device allocation or copy failed";
+            return ctxFail(ctx, KDN_E_MEMORY);
+        }
+        c.n_leaf_progs = (uint32_t)lprogs.size();
     }
     c.scene_has_glass = false;
     for (uint32_t m : leafMat) c.scene_has_glass = c.scene_has_glass || (m < mats.size() && mats[m].kind == KDN_GLASS);

@@ -85,6 +85,9 @@ struct Ctx {
     bool scene_has_glass = false;   // a leaf is painted with a glass
     void* guide = nullptr;          // the learned guide (GuideState), made on first use
     void* guide_round = nullptr;    // lobes and records of rounds whose jobs own their guides (serving)
+    void* leaf_steps_dev = nullptr;  // every leaf's own program (LeafStepDev), the survey's chains walk them
+    void* leaf_progs_dev = nullptr;  // LeafProgDev per leaf
+    uint32_t n_leaf_progs = 0;
     DevBuf sort_tmp;                // the ray queues' sort's temporary storage
     DevBuf hguides;                 // the plain guide: a packed guide per pixel
     int hguide_width = 0, hguide_height = 0;  // the image they were made for (0: none)

@@ -6,17 +6,15 @@
 // Follows the reference's structure (plugins/sampler/highlight_guide/
 // survey.h:27-70, survey.cpp): survey points every eighth pixel from the
 // fourth cast their middle ray and draw 128 directions through the
-// material (counter-based numbers keyed apart from any sample); a draw
-// that meets a sharp surface (metal of facet width at most 0.125) starts a
-// highlight. Where the reference walks a specular chain of up to several
-// mirrors from each start and lamp to the point that mirrors the lamp
-// exactly (chain.cpp), this survey walks a chain of one mirror in one step
-// (the start's tangent plane: the point where the stand's line to the
-// lamp's mirror image crosses it), checks with a ray that the surface is
-// there and mirrors the lamp within its outline widened by the facets'
-// lobe, and gives the lump the spread a flat glossy mirror of that width
-// would: (2 width + the lamp's angular radius) d / (D + d) radians across,
-// d from the point to the lamp, D from the viewer. A pixel's guide takes the highlights of its
+// material (counter-based numbers keyed apart from any sample); the draws
+// that meet a sharp surface (metal of facet width at most 0.125) are the
+// point's seeds, at most three to a leaf, 0.05 radians apart, sixteen in
+// all (surveySeeds). Each seed and lamp walks the reference's chain (a
+// damped Newton walk over the leaf's own program in FP64, chain_dev.h) to
+// the point whose normal halves the directions to the stand and the lamp;
+// the most important settled one (glare times the spread's area) is kept,
+// and a ray checks that the surface is there and mirrors the lamp within
+// its outline widened by the facets' lobe. A pixel's guide takes the highlights of its
 // four nearest survey points on its own material, each walked again from
 // the pixel's own stand (one mirror, one step) and one lump per highlight,
 // weighs each by nearness, the lamp's glare, the lump's size and the
@@ -25,14 +23,18 @@
 // drops those under a hundredth of the heaviest, and shares its draws by
 // the light the lumps bring against the direct light, within [0.02, 0.5]
 // (guideShare, survey.cpp:818-828). The guide's lumps are the reference's
-// and so is their use in shading (hguide.h); the survey's lumps are this
-// build's and are not the reference's bits.
+// and so is their use in shading (hguide.h), and so are the seeds and the
+// chains; the pixel guides' walk (one mirror in one step from the pixel's
+// stand) and the lumps' round spread (the flat glossy mirror's, (2 width +
+// the lamp's angular radius) d / (D + d) radians across, d from the point
+// to the lamp, D from the viewer) are this build's.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <vector>
 
+#include "chain_dev.h"
 #include "ctx.h"
 #include "hguide.h"
 #include "integrator_kernels.h"
@@ -58,6 +60,9 @@ struct HgScene {
     uint32_t nMaterials;
     const LampDev* lamps;
     uint32_t nLamps;
+    const LeafStepDev* leafSteps;  // every leaf's own program (the chains walk them)
+    const LeafProgDev* leafProgs;
+    uint32_t nLeaves;
 };
 
 KDN_D V3 lens_dir(const HgLens& L, double x, double y) {
@@ -158,82 +163,128 @@ __global__ void k_hg_draws(HgScene S, const float* stands, uint32_t nPoints, kdn
     rays[at] = r;
 }
 
-// A draw that meets a sharp metal starts a highlight: for each lamp in
-// front of the surface, the point of the start's tangent plane that
-// mirrors the lamp's centre toward the stand (the stand joined to the
-// centre's mirror image across the plane); the most important lamp's is
-// kept, and a ray from the stand toward it checks that the surface is
-// there (a chain of one mirror, walked in one step: exact for planes).
-__global__ void k_hg_starts(HgScene S, const float* stands, const kdn_ray* rays, const kdn_hit* hits, uint32_t n, kdn_ray* aims,
-                            float* cand) {
-    const uint32_t at = blockIdx.x * blockDim.x + threadIdx.x;
-    if (at >= n) return;
-    float* c = cand + (size_t)at * kSpotFloats;
-    c[0] = 0;  // no candidate
-    kdn_ray a;
-    a.far_ = -1;  // no aim: an empty span
-    a.inside = 0;
-    a.within = KDN_NO_MATERIAL;
+// Each survey point's seeds among its draws, in draw order (surveySeeds,
+// survey.cpp:487-529): a draw that meets a sharp metal is a seed unless
+// three seeds of its leaf are kept already or one of them lies within
+// 0.05 radians of its direction; sixteen at most, listed in
+// seeds[point * kMostSeeds ...] (the rest of its slots kNoSeed). Every
+// draw's candidate and aim start empty.
+constexpr int kMostSeeds = 16;
+constexpr uint32_t kNoSeed = 0xFFFFFFFFu;
+__global__ void k_hg_seeds(HgScene S, const kdn_ray* rays, const kdn_hit* hits, uint32_t nPoints, uint32_t* seeds, kdn_ray* aims,
+                           float* cand) {
+    const uint32_t point = blockIdx.x * blockDim.x + threadIdx.x;
+    if (point >= nPoints) return;
+    constexpr int kSeedsPerLeaf = 3;
+    const float close = 0.99875026f;  // cos 0.05
+    uint32_t leafOf[kMostSeeds];
+    V3 dirOf[kMostSeeds];
+    int kept = 0;
+    kdn_ray none;
+    none.far_ = -1;  // no aim: an empty span
+    none.inside = 0;
+    none.within = KDN_NO_MATERIAL;
     for (int k = 0; k < 3; ++k) {
-        a.origin[k] = 0;
-        a.direction[k] = k == 2 ? 1.0f : 0.0f;
+        none.origin[k] = 0;
+        none.direction[k] = k == 2 ? 1.0f : 0.0f;
     }
-    const kdn_hit h = hits[at];
-    if (h.kind == KDN_HIT_SURFACE && rays[at].far_ > 0) {
+    for (int draw = 0; draw < kSurveyDraws; ++draw) {
+        const size_t at = (size_t)point * kSurveyDraws + draw;
+        cand[at * kSpotFloats] = 0;
+        aims[at] = none;
+        const kdn_ray r = rays[at];
+        if (!(r.far_ > 0) || kept == kMostSeeds) continue;
+        const kdn_hit h = hits[at];
+        if (h.kind != KDN_HIT_SURFACE) continue;
+        const V3 d = v3(r.direction[0], r.direction[1], r.direction[2]);
+        int same = 0;
+        bool near = false;
+        for (int k = 0; k < kept; ++k) {
+            if (leafOf[k] != h.leaf) continue;
+            ++same;
+            near = near || dot(dirOf[k], d) > close;
+        }
         uint32_t mi;
         const MaterialDev& m = mat_of(S, h.leaf, &mi);
-        if (m.kind == KDN_METAL && m.width <= kShinyWidth) {
-            const float* s = stands + (size_t)(at / kSurveyDraws) * kStandFloats;
-            const V3 stand = v3(s[1], s[2], s[3]), sn = v3(s[4], s[5], s[6]);
-            const V3 x = v3(h.point[0], h.point[1], h.point[2]);
-            V3 nv = v3(h.normal[0], h.normal[1], h.normal[2]);
-            if (dot(nv, x - stand) > 0) nv = -nv;
-            float best = 0;
-            for (uint32_t j = 0; j < S.nLamps; ++j) {
-                const LampDev& L = S.lamps[j];
-                if (!hg_mirrored(L)) continue;
-                const V3 centre = hg_lamp_centre(L);
-                const float above = dot(centre - x, nv);
-                if (!(above > 0)) continue;
-                const V3 image = centre - nv * (2 * above);
-                const float denom = dot(image - stand, nv), t = dot(x - stand, nv) / denom;
-                if (!(t > 0 && t < 1)) continue;
-                const V3 point = stand + (image - stand) * t;
-                const V3 toward = centre - point;
-                const float dl = length(toward), D = length(point - stand);
-                if (!(dl > 1e-4f && D > 1e-4f)) continue;
-                if (!hg_lamp_faces(L, toward * (1.0f / dl))) continue;
-                const float rho = hg_lamp_reach(L, dl);
-                const float sigma = fmax_((2 * m.width + rho) * dl / (D + dl), 1e-3f);
-                const float importance = hg_lamp_glare(L) * sigma * sigma;
-                if (importance > best) {
-                    best = importance;
-                    const V3 o = stand + sn * lift_at(stand);
-                    const V3 d = point - o;
-                    const float reach = length(d);
-                    a.origin[0] = o.x;
-                    a.origin[1] = o.y;
-                    a.origin[2] = o.z;
-                    a.direction[0] = d.x / reach;
-                    a.direction[1] = d.y / reach;
-                    a.direction[2] = d.z / reach;
-                    a.far_ = reach * 1.05f + 0.01f;
-                    c[0] = 1;
-                    c[1] = (float)j;
-                    c[6] = m.width;
-                    c[8] = s[10];
-                    c[10] = __builtin_bit_cast(float, h.leaf);
-                    c[11] = (float)mi;
-                }
-            }
+        if (same < kSeedsPerLeaf && !near && m.kind == KDN_METAL && m.width <= kShinyWidth) {
+            leafOf[kept] = h.leaf;
+            dirOf[kept] = d;
+            seeds[(size_t)point * kMostSeeds + kept] = (uint32_t)at;
+            ++kept;
         }
     }
-    aims[at] = a;
+    for (int k = kept; k < kMostSeeds; ++k) seeds[(size_t)point * kMostSeeds + k] = kNoSeed;
+}
+
+// A seed starts a highlight: for each lamp, the chain from the stand (its
+// surface point lifted along its normal, as standAt, survey.cpp:138-156)
+// by the metal's leaf to the lamp is walked from the seed's hit
+// (walkedChain, chain_dev.h; walkedFrom, survey.cpp:312-326); of the
+// settled ones with the lamp in front of the point, the most important
+// (the lamp's glare times the square root of the spread's determinant,
+// spotImportance, survey.cpp:582-585) is kept, and a ray from the stand
+// toward its point checks that the surface is there. One lane a seed slot.
+__global__ void k_hg_walks(HgScene S, const float* stands, const kdn_hit* hits, const uint32_t* seeds, uint32_t nSlots, kdn_ray* aims,
+                           float* cand) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= nSlots) return;
+    const uint32_t at = seeds[slot];
+    if (at == kNoSeed) return;
+    float* c = cand + (size_t)at * kSpotFloats;
+    const kdn_hit h = hits[at];
+    uint32_t mi;
+    const MaterialDev& m = mat_of(S, h.leaf, &mi);
+    const float* s = stands + (size_t)(at / kSurveyDraws) * kStandFloats;
+    const V3 stand = v3(s[1], s[2], s[3]), sn = v3(s[4], s[5], s[6]);
+    const V3 x = v3(h.point[0], h.point[1], h.point[2]);
+    const V3 from = stand + sn * lift_at(stand);
+    D3 side0, side1;
+    basis_of(d3(sn.x, sn.y, sn.z), &side0, &side1);
+    HD regs[kLeafMostSteps];
+    float best = 0;
+    for (uint32_t j = 0; j < S.nLamps && h.leaf < S.nLeaves; ++j) {
+        const LampDev& L = S.lamps[j];
+        if (!hg_mirrored(L)) continue;
+        const ChainOut walked = walked_chain(S.leafSteps, S.leafProgs[h.leaf], d3(from.x, from.y, from.z), side0, side1, m.width,
+                                             chain_lamp(L), d3(x.x, x.y, x.z), regs);
+        if (!walked.found) continue;
+        const V3 point = v3((float)walked.point.x, (float)walked.point.y, (float)walked.point.z);
+        const V3 toward = hg_lamp_centre(L) - point;
+        const float dl = length(toward), D = length(point - stand);
+        if (!(dl > 1e-4f && D > 1e-4f)) continue;
+        if (!hg_lamp_faces(L, toward * (1.0f / dl))) continue;
+        const double det = walked.spread[0] * walked.spread[2] - walked.spread[1] * walked.spread[1];
+        if (!(det > 0)) continue;
+        const float importance = hg_lamp_glare(L) * (float)__builtin_sqrt(det);
+        if (!(importance > best)) continue;
+        best = importance;
+        const V3 d = point - from;
+        const float reach = length(d);
+        kdn_ray a;
+        a.inside = 0;
+        a.within = KDN_NO_MATERIAL;
+        a.origin[0] = from.x;
+        a.origin[1] = from.y;
+        a.origin[2] = from.z;
+        a.direction[0] = d.x / reach;
+        a.direction[1] = d.y / reach;
+        a.direction[2] = d.z / reach;
+        a.far_ = reach * 1.05f + 0.01f;
+        aims[at] = a;
+        c[0] = 1;
+        c[1] = (float)j;
+        c[5] = (float)__builtin_sqrt(det);  // the spread's area as a round lump's sigma^2
+        c[6] = m.width;
+        c[8] = s[10];
+        c[9] = importance;
+        c[10] = __builtin_bit_cast(float, h.leaf);
+        c[11] = (float)mi;
+    }
 }
 
 // A start whose aim meets the same sharp metal with the lamp within its
 // mirror direction's reach (the lamp's outline widened by the lobe): a
-// candidate highlight there, of the spread a flat glossy mirror gives.
+// candidate highlight there, of its chain's spread and importance.
 __global__ void k_hg_candidates(HgScene S, const kdn_ray* aims, const kdn_hit* hits, uint32_t n, float* cand) {
     const uint32_t at = blockIdx.x * blockDim.x + threadIdx.x;
     if (at >= n) return;
@@ -261,14 +312,11 @@ __global__ void k_hg_candidates(HgScene S, const kdn_ray* aims, const kdn_hit* h
     const float rho = hg_lamp_reach(L, dl);
     const float reach = rho + 2 * m.width;
     if (!(dot(mirror, dir) > f_cos(fmin_(reach, 1.5f)))) return;
-    const float sigma = fmax_((2 * m.width + rho) * dl / (D + dl), 1e-3f);
     c[0] = 1;
     c[2] = x.x;
     c[3] = x.y;
     c[4] = x.z;
-    c[5] = sigma * sigma;
     c[7] = D;
-    c[9] = hg_lamp_glare(L) * sigma * sigma;
     c[10] = __builtin_bit_cast(float, h.leaf);
     c[12] = nv.x;
     c[13] = nv.y;
@@ -491,13 +539,15 @@ bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats, DevBuf&
         L.down[k] = upv[k] * (-2 * reach / height);
         L.eye[k] = cam.eye[k];
     }
-    HgScene S{c.scene.leaf_material, (const MaterialDev*)c.materials_dev, c.n_materials_dev, (const LampDev*)c.lamps_dev, c.n_lamps};
+    HgScene S{c.scene.leaf_material, (const MaterialDev*)c.materials_dev, c.n_materials_dev, (const LampDev*)c.lamps_dev, c.n_lamps,
+              (const LeafStepDev*)c.leaf_steps_dev, (const LeafProgDev*)c.leaf_progs_dev, c.n_leaf_progs};
     const int cols = width > 4 ? (width - 4 + 7) / 8 : 0, rows = height > 4 ? (height - 4 + 7) / 8 : 0;
     const uint32_t nPoints = (uint32_t)(cols * rows), nDraws = nPoints * kSurveyDraws, nPixels = (uint32_t)(width * height);
     // buffers: rays and hits (the largest of draws and pixels), stands, candidates, spots, guides
     const size_t nRays = std::max<size_t>(nDraws, nPixels);
     const size_t bytes = nRays * (sizeof(kdn_ray) + sizeof(kdn_hit)) + (size_t)nDraws * sizeof(kdn_ray) + (size_t)std::max<size_t>(nPoints, nPixels) * kStandFloats * 4 +
-                         (size_t)nDraws * kSpotFloats * 4 + (size_t)nPoints * (1 + kSpots * kSpotFloats) * 4 + 8 * 256;
+                         (size_t)nDraws * kSpotFloats * 4 + (size_t)nPoints * (1 + kSpots * kSpotFloats) * 4 +
+                         (size_t)nPoints * kMostSeeds * 4 + 9 * 256;
     if (!ensure(work, bytes, c.error, false)) return false;
     char* at = (char*)work.ptr;
     auto take = [&](size_t b) {
@@ -511,6 +561,7 @@ bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats, DevBuf&
     float* stands = (float*)take((size_t)std::max<size_t>(nPoints, nPixels) * kStandFloats * 4);
     float* cand = (float*)take((size_t)nDraws * kSpotFloats * 4);
     float* spots = (float*)take((size_t)nPoints * (1 + kSpots * kSpotFloats) * 4);
+    uint32_t* seeds = (uint32_t*)take((size_t)nPoints * kMostSeeds * 4);
     const bool surveyed = nPoints > 0 && c.n_lamps > 0;
     const size_t pixelBytes = ((size_t)nPixels * kHgFloats * 4 + 255) & ~size_t(255);
     const size_t G3 = (size_t)kHgDeepGrid * kHgDeepGrid * kHgDeepGrid;
@@ -532,8 +583,10 @@ bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats, DevBuf&
         hipLaunchKernelGGL(k_hg_stands, grid(nPoints), dim3(kHgBlock), 0, st, S, (const kdn_ray*)rays, (const kdn_hit*)hits, nPoints, stands);
         hipLaunchKernelGGL(k_hg_draws, grid(nDraws), dim3(kHgBlock), 0, st, S, (const float*)stands, nPoints, rays);
         if (!extendRays(c, rays, nDraws, hits, st)) return false;
-        hipLaunchKernelGGL(k_hg_starts, grid(nDraws), dim3(kHgBlock), 0, st, S, (const float*)stands, (const kdn_ray*)rays,
-                           (const kdn_hit*)hits, nDraws, aims, cand);
+        hipLaunchKernelGGL(k_hg_seeds, grid(nPoints), dim3(kHgBlock), 0, st, S, (const kdn_ray*)rays, (const kdn_hit*)hits, nPoints,
+                           seeds, aims, cand);
+        hipLaunchKernelGGL(k_hg_walks, dim3((nPoints * kMostSeeds + 63) / 64), dim3(64), 0, st, S, (const float*)stands,
+                           (const kdn_hit*)hits, (const uint32_t*)seeds, nPoints * kMostSeeds, aims, cand);
         if (!extendRays(c, aims, nDraws, hits, st)) return false;
         hipLaunchKernelGGL(k_hg_candidates, grid(nDraws), dim3(kHgBlock), 0, st, S, (const kdn_ray*)aims, (const kdn_hit*)hits,
                            nDraws, cand);
@@ -605,7 +658,75 @@ bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats, DevBuf&
     return true;
 }
 
+// chains walked for queries (kdn_highlight_chains): one lane a chain, its
+// leaf's hyper-dual registers in the lane's own memory
+__global__ void k_hg_chains(const LeafStepDev* steps, const LeafProgDev* progs, uint32_t nLeaves, const LampDev* lamps, uint32_t nLamps,
+                            const kdn_chain_query* queries, uint32_t n, kdn_chain* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    HD regs[kLeafMostSteps];
+    const kdn_chain_query q = queries[i];
+    kdn_chain r;
+    r.found = 0;
+    r.pad = 0;
+    for (int k = 0; k < 3; ++k) {
+        r.point[k] = r.axis[k] = r.across[k] = r.up[k] = r.spread[k] = 0;
+        r.motion[0][k] = r.motion[1][k] = 0;
+    }
+    if (q.leaf < nLeaves && q.light < nLamps) {
+        const ChainOut o = walked_chain(steps, progs[q.leaf], d3(q.from[0], q.from[1], q.from[2]),
+                                        d3(q.sideways[0][0], q.sideways[0][1], q.sideways[0][2]),
+                                        d3(q.sideways[1][0], q.sideways[1][1], q.sideways[1][2]), q.width, chain_lamp(lamps[q.light]),
+                                        d3(q.seed[0], q.seed[1], q.seed[2]), regs);
+        if (o.found) {
+            r.found = 1;
+            const D3 v[4] = {o.point, o.axis, o.across, o.up};
+            for (int k = 0; k < 3; ++k) {
+                r.point[k] = compd(v[0], k);
+                r.axis[k] = compd(v[1], k);
+                r.across[k] = compd(v[2], k);
+                r.up[k] = compd(v[3], k);
+                r.spread[k] = o.spread[k];
+                r.motion[0][k] = compd(o.motion[0], k);
+                r.motion[1][k] = compd(o.motion[1], k);
+            }
+        }
+    }
+    out[i] = r;
+}
+
 }  // namespace kdn
+
+extern "C" int kdn_highlight_chains(kdn_ctx* ctx, const kdn_chain_query* queries, uint32_t n, kdn_chain* chains) {
+    if (!ctx || (n && (!queries || !chains))) return KDN_E_ARGUMENT;
+    kdn::Ctx& c = ctx->c;
+    if (!c.integrator_ready || !c.leaf_progs_dev) {
+        c.error = "kdn_highlight_chains: no scene on the device";
+        return KDN_E_STATE;
+    }
+    if (n > (1u << 22)) {
+        c.error = "kdn_highlight_chains: at most 2^22 chains a call";
+        return KDN_E_ARGUMENT;
+    }
+    if (n == 0) return KDN_OK;
+    hipSetDevice(c.device);
+    const size_t qb = (size_t)n * sizeof(kdn_chain_query), cb = (size_t)n * sizeof(kdn_chain);
+    if (!kdn::ensure(c.scratch[6], qb + cb, c.error, false)) return KDN_E_MEMORY;
+    kdn_chain_query* q = (kdn_chain_query*)c.scratch[6].ptr;
+    kdn_chain* out = (kdn_chain*)((char*)c.scratch[6].ptr + qb);
+    if (hipMemcpyAsync(q, queries, qb, hipMemcpyHostToDevice, c.stream) != hipSuccess) {
+        c.error = "kdn_highlight_chains: copy failed";
+        return KDN_E_DEVICE;
+    }
+    hipLaunchKernelGGL(kdn::k_hg_chains, dim3((n + 63) / 64), dim3(64), 0, c.stream, (const kdn::LeafStepDev*)c.leaf_steps_dev,
+                       (const kdn::LeafProgDev*)c.leaf_progs_dev, c.n_leaf_progs, (const kdn::LampDev*)c.lamps_dev, c.n_lamps, q, n, out);
+    if (hipGetLastError() != hipSuccess || hipMemcpyAsync(chains, out, cb, hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+        hipStreamSynchronize(c.stream) != hipSuccess) {
+        c.error = "kdn_highlight_chains: device call failed";
+        return KDN_E_DEVICE;
+    }
+    return KDN_OK;
+}
 
 extern "C" int kdn_highlight_guides(kdn_ctx* ctx, int32_t width, int32_t height, kdn_hguide_stats* stats) {
     if (!ctx) return KDN_E_ARGUMENT;

@@ -1137,6 +1137,16 @@ Program nodeProgram(const Scene& scene, uint32_t node) {
     return pruneProgram(b.program);
 }
 
+// One leaf alone over world space (leafProgram, source/sdf/scene.cpp:
+// 284-288): the point's steps, the leaf inlined, pruned.
+Program leafAlone(const Scene& scene, uint32_t leaf) {
+    Builder b;
+    b.leaf = leaf;
+    for (int i = 0; i < 3; ++i) emitStep(b, uint8_t(KDN_POINT_X + i));
+    b.program.result = inlineLeaf(b, scene, leaf);
+    return pruneProgram(b.program);
+}
+
 void unionParts(const Scene& scene, uint32_t node, std::vector<uint32_t>& parts) {
     const Node& n = scene.nodes[node];
     const bool split = n.join == JoinMerge && n.radius == 0 && nodeProgram(scene, node).steps.size() >= kFewestUnionSteps;
@@ -1148,6 +1158,8 @@ void unionParts(const Scene& scene, uint32_t node, std::vector<uint32_t>& parts)
 }
 
 }  // namespace
+
+Program leafProgram(const Scene& scene, uint32_t leaf) { return leafAlone(scene, leaf); }
 
 int operandCount(uint8_t op) {
     switch (op) {

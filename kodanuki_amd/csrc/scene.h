@@ -66,6 +66,10 @@ void compileScene(Scene& scene);
 // Remove steps the result does not read (source/sdf/program.cpp:362-407).
 Program pruneProgram(const Program& program);
 
+// One leaf alone over world space, as the highlight survey walks it
+// (leafProgram, source/sdf/scene.cpp:284-288).
+Program leafProgram(const Scene& scene, uint32_t leaf);
+
 int operandCount(uint8_t op);
 
 }  // namespace kdn

@@ -38,6 +38,16 @@ struct MaterialDev {
 
 // A lamp as the integrator reads it (lampOf, light.cpp:77-94): a point,
 // a directional light, a sphere, a quad or a sun.
+// A leaf program's step (opcodes of kodanuki.h) and a leaf's program in the
+// pool of every leaf's own program (the highlight survey's chains walk them).
+struct LeafStepDev {
+    uint32_t op, a, b, c;
+    double value;
+};
+struct LeafProgDev {
+    uint32_t first, count, result, pad;
+};
+
 struct LampDev {
     float at[3], along[3], across[3];
     float facing[3];

@@ -21,8 +21,10 @@
 #include "math/spectrum.h"
 #include "plugins/reconstruction/atrous/atrous.h"
 #include "plugins/reconstruction/temporal/temporal.h"
+#include "plugins/sampler/highlight_guide/chain.h"
 #include "plugins/sampler/sobol/sobol.h"
 #include "plugins/transport/paths/material/energy.h"
+#include "sdf/program.h"
 #include "shared/image.h"
 
 using namespace kodanuki;
@@ -237,6 +239,61 @@ void kr_denoised_frame(int width, int height, const kr_lens* lens, const double*
 
 
 uint64_t kr_mixed_bits(uint64_t v) { return math::mixedBits(v); }
+
+// walkedChain (chain.h:97) over one leaf program (count steps: opcode,
+// operands a b c, value; its result register) from a start with its two
+// sideways directions, for a lamp of this build's kind numbering (0 point,
+// 1 quad, 2 directional, 3 sphere, 4 sun, 5 moon); out: point, axis,
+// across, up, spread (3), motion (6). Returns found.
+int kr_walked_chain(const uint8_t* ops, const uint32_t* abc, const double* values, uint32_t count, uint32_t result, const double* from,
+                    const double* sideways, double width, int kind, const double* at, const double* direction, const double* along,
+                    const double* across, double radius, double cosine, const double* seed, double* out) {
+    Program leaf;
+    leaf.name = "leaf";
+    leaf.result = result;
+    leaf.arguments = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        Step st{};
+        st.opcode = Opcode(ops[i]);
+        st.a = abc[3 * i];
+        st.b = abc[3 * i + 1];
+        st.c = abc[3 * i + 2];
+        st.value = values[i];
+        st.leaf = 0;
+        leaf.steps.push_back(st);
+    }
+    const LightKind kinds[6] = {LightKind::Point, LightKind::Quad, LightKind::Directional, LightKind::Sphere, LightKind::Sun,
+                                LightKind::Moon};
+    Lamp lamp{};
+    lamp.kind = kinds[kind];
+    lamp.at = {at[0], at[1], at[2]};
+    lamp.direction = {direction[0], direction[1], direction[2]};
+    lamp.along = {along[0], along[1], along[2]};
+    lamp.across = {across[0], across[1], across[2]};
+    lamp.radius = radius;
+    lamp.cosine = cosine;
+    const BounceChain chain = {
+        .from = {from[0], from[1], from[2]},
+        .sideways = {math::Vector{sideways[0], sideways[1], sideways[2]}, math::Vector{sideways[3], sideways[4], sideways[5]}},
+        .leaf = leaf,
+        .width = width,
+        .lamp = lamp,
+    };
+    const SettledChain r = walkedChain(chain, {seed[0], seed[1], seed[2]});
+    const math::Vector v[4] = {r.point, r.axis, r.across, r.up};
+    for (int k = 0; k < 4; ++k) {
+        out[3 * k] = v[k].x;
+        out[3 * k + 1] = v[k].y;
+        out[3 * k + 2] = v[k].z;
+    }
+    for (int k = 0; k < 3; ++k) out[12 + k] = r.spread[k];
+    for (int m = 0; m < 2; ++m) {
+        out[15 + 3 * m] = r.motion[m].x;
+        out[16 + 3 * m] = r.motion[m].y;
+        out[17 + 3 * m] = r.motion[m].z;
+    }
+    return r.found ? 1 : 0;
+}
 
 double kr_hashed_number(uint64_t stream, uint64_t index, uint64_t coordinate) {
     return math::hashedNumber(keyOf(stream, index, coordinate));
