@@ -4,10 +4,11 @@
 // plugins/sampler/highlight_guide/chain.h:76-97, chain.cpp:157-521).
 //
 // The leaf is its own program over world space (leafProgram,
-// sdf/scene.cpp:284-288), evaluated with hyper-dual numbers: the value, the
-// gradient and the six Hessian entries of every register (the rules of
-// sdf/numbers.cpp:593-823: a one-operand step by its first and second
-// derivative, a two-operand step by its six partials; min and max pick an
+// sdf/scene.cpp:284-288), evaluated with dual numbers (value and gradient)
+// where the walk puts a point on the surface and with hyper-dual numbers
+// (the six Hessian entries too) where it reads a vertex (the rules of
+// sdf/numbers.cpp:433-823: a one-operand step by its first and second
+// derivative, a two-operand step by its partials; min and max pick an
 // operand, ties to the first).
 //
 // The walk, as the reference takes it: the seed put on the surface by two
@@ -55,85 +56,106 @@ KDN_D void basis_of(D3 n, D3* t, D3* b) {
     *b = d3(c, side + n.y * n.y * a, -n.y);
 }
 
-// ---- hyper-dual numbers ---------------------------------------------------------
-struct HD {
-    double v, g[3], h[6];  // h: xx, xy, xz, yy, yz, zz
+// ---- dual and hyper-dual numbers --------------------------------------------------
+// Num<true>: value, gradient and the Hessian's six entries (xx, xy, xz, yy,
+// yz, zz); Num<false>: value and gradient (the projections onto the leaf
+// read these alone, as leafSlope does, chain.cpp:140-151).
+template <bool H>
+struct Num {
+    double v, g[3], h[H ? 6 : 1];
 };
+using HD = Num<true>;
+using DL = Num<false>;
+// a lane's registers for one leaf program, read either way
+union LeafRegs {
+    HD hd[kLeafMostSteps];
+    DL dl[kLeafMostSteps];
+};
+
 KDN_D int hd_row(int k) { return k < 3 ? 0 : (k < 5 ? 1 : 2); }
 KDN_D int hd_col(int k) { return k < 3 ? k : (k < 5 ? k - 2 : 2); }
-KDN_D HD hd_const(double v) {
-    HD o;
+template <bool H>
+KDN_D Num<H> num_const(double v) {
+    Num<H> o;
     o.v = v;
     for (int i = 0; i < 3; ++i) o.g[i] = 0;
-    for (int k = 0; k < 6; ++k) o.h[k] = 0;
+    if constexpr (H)
+        for (int k = 0; k < 6; ++k) o.h[k] = 0;
     return o;
 }
-KDN_D HD hd_one(const HD& a, double f, double d1, double d2) {
-    HD o = hd_const(f);
+template <bool H>
+KDN_D Num<H> num_one(const Num<H>& a, double f, double d1, double d2) {
+    Num<H> o = num_const<H>(f);
     for (int i = 0; i < 3; ++i) o.g[i] = d1 * a.g[i];
-    for (int k = 0; k < 6; ++k) o.h[k] = d1 * a.h[k] + d2 * (a.g[hd_row(k)] * a.g[hd_col(k)]);
+    if constexpr (H)
+        for (int k = 0; k < 6; ++k) o.h[k] = d1 * a.h[k] + d2 * (a.g[hd_row(k)] * a.g[hd_col(k)]);
     return o;
 }
-KDN_D HD hd_two(const HD& a, const HD& b, double f, double fa, double fb, double faa, double fab, double fbb) {
-    HD o = hd_const(f);
+template <bool H>
+KDN_D Num<H> num_two(const Num<H>& a, const Num<H>& b, double f, double fa, double fb, double faa, double fab, double fbb) {
+    Num<H> o = num_const<H>(f);
     for (int i = 0; i < 3; ++i) o.g[i] = fa * a.g[i] + fb * b.g[i];
-    for (int k = 0; k < 6; ++k) {
-        const int i = hd_row(k), j = hd_col(k);
-        const double ga = a.g[i] * a.g[j], gab = a.g[i] * b.g[j] + b.g[i] * a.g[j], gb = b.g[i] * b.g[j];
-        o.h[k] = fa * a.h[k] + fb * b.h[k] + faa * ga + fab * gab + fbb * gb;
+    if constexpr (H) {
+        for (int k = 0; k < 6; ++k) {
+            const int i = hd_row(k), j = hd_col(k);
+            const double ga = a.g[i] * a.g[j], gab = a.g[i] * b.g[j] + b.g[i] * a.g[j], gb = b.g[i] * b.g[j];
+            o.h[k] = fa * a.h[k] + fb * b.h[k] + faa * ga + fab * gab + fbb * gb;
+        }
     }
     return o;
 }
 KDN_D double truthd(bool t) { return t ? 1.0 : 0.0; }
 
-// One step of a leaf program on hyper-dual registers (evaluate,
+// One step of a leaf program on (hyper-)dual registers (evaluate,
 // sdf/program.cpp:60-92). Normal inputs are zero, as the walk's reads give.
-KDN_D HD hd_step(const LeafStepDev& s, const HD* r, const double p[3]) {
+template <bool H>
+KDN_D Num<H> num_step(const LeafStepDev& s, const Num<H>* r, const double p[3]) {
+    using N = Num<H>;
     const uint32_t op = s.op;
-    if (op == KDN_CONSTANT) return hd_const(s.value);
+    if (op == KDN_CONSTANT) return num_const<H>(s.value);
     if (op >= KDN_POINT_X && op <= KDN_POINT_Z) {
-        HD o = hd_const(p[op - KDN_POINT_X]);
+        N o = num_const<H>(p[op - KDN_POINT_X]);
         o.g[op - KDN_POINT_X] = 1;
         return o;
     }
-    if (op == KDN_ARGUMENT || (op >= KDN_NORMAL_X && op <= KDN_NORMAL_Z)) return hd_const(0);
-    const HD& a = r[s.a];
+    if (op == KDN_ARGUMENT || (op >= KDN_NORMAL_X && op <= KDN_NORMAL_Z)) return num_const<H>(0);
+    const N& a = r[s.a];
     switch (op) {
-        case KDN_NEGATE: return hd_one(a, -a.v, -1, 0);
+        case KDN_NEGATE: return num_one<H>(a, -a.v, -1, 0);
         case KDN_SQUARE_ROOT: {
             const double f = __builtin_sqrt(a.v);
-            if (f <= 0) return hd_const(f);
-            return hd_one(a, f, 0.5 / f, -0.25 / (a.v * f));
+            if (f <= 0) return num_const<H>(f);
+            return num_one<H>(a, f, 0.5 / f, -0.25 / (a.v * f));
         }
-        case KDN_ABSOLUTE: return a.v < 0 ? hd_one(a, -a.v, -1, 0) : a;
-        case KDN_FLOOR: return hd_const(__builtin_floor(a.v));
-        case KDN_SINE: return hd_one(a, sin(a.v), cos(a.v), -sin(a.v));
-        case KDN_COSINE: return hd_one(a, cos(a.v), -sin(a.v), -cos(a.v));
-        case KDN_SIGN: return hd_const(a.v > 0 ? 1.0 : (a.v < 0 ? -1.0 : 0.0));
-        case KDN_NOT: return hd_const(truthd(a.v == 0));
+        case KDN_ABSOLUTE: return a.v < 0 ? num_one<H>(a, -a.v, -1, 0) : a;
+        case KDN_FLOOR: return num_const<H>(__builtin_floor(a.v));
+        case KDN_SINE: return num_one<H>(a, sin(a.v), cos(a.v), -sin(a.v));
+        case KDN_COSINE: return num_one<H>(a, cos(a.v), -sin(a.v), -cos(a.v));
+        case KDN_SIGN: return num_const<H>(a.v > 0 ? 1.0 : (a.v < 0 ? -1.0 : 0.0));
+        case KDN_NOT: return num_const<H>(truthd(a.v == 0));
         case KDN_POSITIVE_SQUARE: {
             const double q = a.v > 0 ? a.v : 0.0;
-            return hd_one(a, q * q, 2 * q, a.v > 0 ? 2.0 : 0.0);
+            return num_one<H>(a, q * q, 2 * q, a.v > 0 ? 2.0 : 0.0);
         }
         default: break;
     }
-    const HD& b = r[s.b];
+    const N& b = r[s.b];
     switch (op) {
-        case KDN_ADD: return hd_two(a, b, a.v + b.v, 1, 1, 0, 0, 0);
-        case KDN_SUBTRACT: return hd_two(a, b, a.v - b.v, 1, -1, 0, 0, 0);
-        case KDN_MULTIPLY: return hd_two(a, b, a.v * b.v, b.v, a.v, 0, 1, 0);
+        case KDN_ADD: return num_two<H>(a, b, a.v + b.v, 1, 1, 0, 0, 0);
+        case KDN_SUBTRACT: return num_two<H>(a, b, a.v - b.v, 1, -1, 0, 0, 0);
+        case KDN_MULTIPLY: return num_two<H>(a, b, a.v * b.v, b.v, a.v, 0, 1, 0);
         case KDN_DIVIDE: {
             const double v = b.v;
-            return hd_two(a, b, a.v / v, 1 / v, -a.v / (v * v), 0, -1 / (v * v), 2 * a.v / (v * v * v));
+            return num_two<H>(a, b, a.v / v, 1 / v, -a.v / (v * v), 0, -1 / (v * v), 2 * a.v / (v * v * v));
         }
         case KDN_ARC_TANGENT: {
             const double rr = b.v * b.v + a.v * a.v, xy = b.v * a.v;
-            return hd_two(a, b, atan2(a.v, b.v), b.v / rr, -a.v / rr, -2 * xy / (rr * rr), (a.v * a.v - b.v * b.v) / (rr * rr),
-                          2 * xy / (rr * rr));
+            return num_two<H>(a, b, atan2(a.v, b.v), b.v / rr, -a.v / rr, -2 * xy / (rr * rr), (a.v * a.v - b.v * b.v) / (rr * rr),
+                              2 * xy / (rr * rr));
         }
         case KDN_POWER: {
             const double v = a.v, n = b.v, f = pow(v, n), l = v > 0 ? log(v) : 0.0, below = pow(v, n - 1);
-            return hd_two(a, b, f, n * below, f * l, n * (n - 1) * pow(v, n - 2), below * (1 + n * l), f * l * l);
+            return num_two<H>(a, b, f, n * below, f * l, n * (n - 1) * pow(v, n - 2), below * (1 + n * l), f * l * l);
         }
         case KDN_MINIMUM: return a.v <= b.v ? a : b;
         case KDN_MAXIMUM: return a.v >= b.v ? a : b;
@@ -144,23 +166,24 @@ KDN_D HD hd_step(const LeafStepDev& s, const HD* r, const double p[3]) {
             double fa = (rad - d) / (2 * rad);
             fa = fa < 0 ? 0.0 : (fa > 1 ? 1.0 : fa);
             const double k = __builtin_fabs(d) < rad ? 1 / (2 * rad) : 0.0;
-            return hd_two(a, b, f, fa, 1 - fa, -k, k, -k);
+            return num_two<H>(a, b, f, fa, 1 - fa, -k, k, -k);
         }
         case KDN_SELECT: return a.v != 0 ? b : r[s.c];
-        case KDN_LESS: return hd_const(truthd(a.v < b.v));
-        case KDN_LESS_EQUAL: return hd_const(truthd(a.v <= b.v));
-        case KDN_EQUAL: return hd_const(truthd(a.v == b.v));
-        case KDN_NOT_EQUAL: return hd_const(truthd(a.v != b.v));
-        case KDN_AND: return hd_const(truthd(a.v != 0 && b.v != 0));
-        case KDN_OR: return hd_const(truthd(a.v != 0 || b.v != 0));
-        default: return hd_const(0);
+        case KDN_LESS: return num_const<H>(truthd(a.v < b.v));
+        case KDN_LESS_EQUAL: return num_const<H>(truthd(a.v <= b.v));
+        case KDN_EQUAL: return num_const<H>(truthd(a.v == b.v));
+        case KDN_NOT_EQUAL: return num_const<H>(truthd(a.v != b.v));
+        case KDN_AND: return num_const<H>(truthd(a.v != 0 && b.v != 0));
+        case KDN_OR: return num_const<H>(truthd(a.v != 0 || b.v != 0));
+        default: return num_const<H>(0);
     }
 }
 
-// The leaf's value, gradient and Hessian at p (regs: kLeafMostSteps).
-KDN_D HD leaf_read(const LeafStepDev* steps, const LeafProgDev& prog, D3 at, HD* regs) {
+// The leaf's value and gradient (and, with H, its Hessian) at p.
+template <bool H>
+KDN_D Num<H> leaf_read(const LeafStepDev* steps, const LeafProgDev& prog, D3 at, Num<H>* regs) {
     const double p[3] = {at.x, at.y, at.z};
-    for (uint32_t k = 0; k < prog.count; ++k) regs[k] = hd_step(steps[prog.first + k], regs, p);
+    for (uint32_t k = 0; k < prog.count; ++k) regs[k] = num_step<H>(steps[prog.first + k], regs, p);
     return regs[prog.result];
 }
 
@@ -304,16 +327,16 @@ KDN_D void chain_bend(D3 from, const ChainLamp& L, ChainAt& h) {
 
 // chainVertexAt / settledVertex (chain.cpp:157-184): the point put on the
 // surface by two Newton steps, then read there
-KDN_D ChainAt chain_vertex(const LeafStepDev* steps, const LeafProgDev& prog, D3 start, HD* regs) {
+KDN_D ChainAt chain_vertex(const LeafStepDev* steps, const LeafProgDev& prog, D3 start, LeafRegs& regs) {
     D3 p = start;
     for (int k = 0; k < 2; ++k) {
-        const HD r = leaf_read(steps, prog, p, regs);
+        const DL r = leaf_read<false>(steps, prog, p, regs.dl);
         const D3 g = d3(r.g[0], r.g[1], r.g[2]);
         const double steep = dotd(g, g);
         if (!(steep > 0)) break;
         p = p - g * (r.v / steep);
     }
-    const HD r = leaf_read(steps, prog, p, regs);
+    const HD r = leaf_read<true>(steps, prog, p, regs.hd);
     ChainAt h;
     h.point = p;
     h.value = r.v;
@@ -368,7 +391,7 @@ KDN_D ChainOut chain_unsettled() {
 // walkedChain (chain.cpp:507-519): from a start with its two sideways
 // directions, over a leaf of facet width `width`, toward lamp L, from seed
 KDN_D ChainOut walked_chain(const LeafStepDev* steps, const LeafProgDev& prog, D3 from, D3 side0, D3 side1, double width,
-                            const ChainLamp& L, D3 seed, HD* regs) {
+                            const ChainLamp& L, D3 seed, LeafRegs& regs) {
     if (prog.count == 0 || prog.count > (uint32_t)kLeafMostSteps) return chain_unsettled();
     ChainAt now = chain_vertex(steps, prog, seed, regs);
     chain_bend(from, L, now);

@@ -240,7 +240,7 @@ __global__ void k_hg_walks(HgScene S, const float* stands, const kdn_hit* hits, 
     const V3 from = stand + sn * lift_at(stand);
     D3 side0, side1;
     basis_of(d3(sn.x, sn.y, sn.z), &side0, &side1);
-    HD regs[kLeafMostSteps];
+    LeafRegs regs;
     float best = 0;
     for (uint32_t j = 0; j < S.nLamps && h.leaf < S.nLeaves; ++j) {
         const LampDev& L = S.lamps[j];
@@ -664,7 +664,7 @@ __global__ void k_hg_chains(const LeafStepDev* steps, const LeafProgDev* progs, 
                             const kdn_chain_query* queries, uint32_t n, kdn_chain* out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    HD regs[kLeafMostSteps];
+    LeafRegs regs;
     const kdn_chain_query q = queries[i];
     kdn_chain r;
     r.found = 0;
