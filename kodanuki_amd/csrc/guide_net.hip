@@ -94,6 +94,18 @@ __device__ __forceinline__ void store_f32(const f32x4 acc[16], float* O) {
         for (int j = 0; j < 4; ++j) O[d_row(t, j) * kOutLd + d_col(t)] = acc[t][j];
 }
 
+// The lobes' outputs alone (columns below kGuideOutputs), rows kInferOutLd
+// apart: a wave's outputs fit where its activations were.
+constexpr int kInferOutLd = kGuideOutputs;  // odd: rows start in different banks
+static_assert(64 * kInferOutLd * 4 >= 64 * kLd * 2, "a wave's outputs cover its activations' space");
+__device__ __forceinline__ void store_outputs(const f32x4 acc[16], float* O) {
+    for (int t = 0; t < 16; ++t) {
+        const int col = d_col(t);
+        if (col >= kGuideOutputs) continue;
+        for (int j = 0; j < 4; ++j) O[d_row(t, j) * kInferOutLd + col] = acc[t][j];
+    }
+}
+
 // Adds (v0, v1) to base[2 idx], base[2 idx + 1] (fixed point, fixed_of)
 // for the active lanes of a wavefront (all 64 lanes call it). Lanes with
 // the same entry are summed first (a butterfly in a fixed order), a leader
@@ -189,16 +201,18 @@ struct PathQueries {
     }
 };
 
+// Inference: 4 waves a block, persistent over tiles of 64 queries; the
+// weights in LDS, each wave's activations (BF16 rows) and then its outputs
+// in one buffer of its own: 78.8 KB a block, two blocks to a CU.
 template <class Q>
 __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideView g, Q src) {
     __shared__ __attribute__((aligned(16))) uint16_t Wt[kGuideLayers][64 * kLd];
-    __shared__ __attribute__((aligned(16))) uint16_t act[kInferWaves][64 * kLd];
-    __shared__ float fo[kInferWaves][64 * kOutLd];
+    __shared__ __attribute__((aligned(16))) float buf[kInferWaves][64 * kInferOutLd];
     load_weights(g.mlp, Wt, nullptr, 64 * kInferWaves);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    uint16_t* A = act[wave];
-    float* O = fo[wave];
+    uint16_t* A = (uint16_t*)buf[wave];
+    float* O = buf[wave];
     const uint32_t tiles = (src.n + 63) / 64;
     for (uint32_t base = blockIdx.x * kInferWaves; base < tiles; base += gridDim.x * kInferWaves) {
         const uint32_t item = (base + wave) * 64 + lane;
@@ -214,11 +228,12 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideView g, Q
             __syncthreads();
         }
         wave_gemm(A, kLd, Wt[kGuideLayers - 1], kLd, acc);
-        store_f32(acc, O);
+        __syncthreads();  // the last layer has read the activations the outputs replace
+        store_outputs(acc, O);
         __syncthreads();
         if (live) {
             float o[kGuideOutputs];
-            for (int c = 0; c < kGuideOutputs; ++c) o[c] = O[lane * kOutLd + c];
+            for (int c = 0; c < kGuideOutputs; ++c) o[c] = O[lane * kInferOutLd + c];
             GuideLobes L;
             decode_lobes(o, g.share_cap, g.learn_select != 0, L);
             src.write(item, L);
