@@ -57,3 +57,55 @@ def test_chain_queries_out_of_range_do_not_settle(device):
     qs["light"][1] = 99
     got = device.highlight_chains(qs)
     assert got["found"][0] == 0 and got["found"][1] == 0
+
+
+def test_device_chains_match_the_reference_on_the_production_scene(device):
+    """The production scene's own leaves (spheres, rounded boxes and tori
+    under arbitrary turns) toward its four quad lamps: stands and seeds are
+    camera hits (a stand on a matte or plastic surface, a seed on a metal
+    leaf), 600 chains; the same chains settle as in the reference."""
+    from tests import raysets
+    lib = CC.ref_lib()
+    if lib is None:
+        pytest.skip("oracle/_ref is not built")
+    scene = k.Scene(scenes_production())
+    lower, size = scene.region(8)
+    device.upload(scene)
+    device.build_octree(lower, size, 8)
+    rays = raysets.camera_rays(scene, 20000, 11)
+    hits = device.extend(rays)
+    mats = scene.materials()
+    leaves = scene.leaves()
+    surface = hits["kind"] == k.KDN_HIT_SURFACE
+    leaf = hits["leaf"]
+    kind = np.array([mats[leaves[l].material].kind if surface[i] and l < len(leaves) else -1 for i, l in enumerate(leaf)])
+    metal = np.nonzero(kind == k.METAL)[0]
+    stands = np.nonzero((kind == k.MATTE) | (kind == k.PLASTIC))[0]
+    rng = np.random.default_rng(5)
+    qs = np.zeros(600, k.CHAIN_QUERY_DTYPE)
+    for i in range(len(qs)):
+        m, s = metal[rng.integers(len(metal))], stands[rng.integers(len(stands))]
+        n = hits["normal"][s].astype(np.float64)
+        qs[i]["from"] = hits["point"][s] + n * 1e-3
+        t = np.cross(n, [0, 0, 1] if abs(n[2]) < 0.9 else [1, 0, 0])
+        t /= np.linalg.norm(t)
+        qs[i]["sideways"] = [t, np.cross(n, t)]
+        qs[i]["seed"] = hits["point"][m]
+        qs[i]["width"] = 0.05
+        qs[i]["leaf"] = leaf[m]
+        qs[i]["light"] = rng.integers(4)
+    got = device.highlight_chains(qs)
+    want = CC.ref_chains(lib, scene, qs)
+    same = got["found"] == want["found"]
+    both = (got["found"] == 1) & (want["found"] == 1)
+    dp = np.linalg.norm(got["point"][both] - want["point"][both], axis=1)
+    note(f"[chain] production scene: {len(qs)} chains, {int(want['found'].sum())} settle in the reference, "
+         f"{int(got['found'].sum())} on the device, {int((~same).sum())} differ; max |point| diff {dp.max() if len(dp) else 0:.2e}")
+    assert same.mean() >= 0.99
+    assert both.sum() >= 50
+    assert dp.max() < 1e-7
+
+
+def scenes_production():
+    from kodanuki_amd import scenes
+    return scenes.production(0, glass=False)
