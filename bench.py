@@ -62,9 +62,17 @@ def cpu_baseline(scene, lower, size, width, height, seed, surfaces, budget_s=12.
             break
         side = min(512, int(side * (budget_s / max(dt, 1e-3)) ** 0.5))
     samples = side * side
+    # one thread on a window sized to about a third of the budget
+    one_side = max(8, min(side, int((samples / dt * budget_s / 3 / max(threads, 1)) ** 0.5)))
+    window = (cx - one_side // 2, cy - one_side // 2, cx - one_side // 2 + one_side, cy - one_side // 2 + one_side)
+    t0 = time.perf_counter()
+    O.render64(scene, A, lower, size, width, height, 1, seed=seed, surfaces=surfaces, window=window, threads=1)
+    dt1 = time.perf_counter() - t0
     return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
             "sample": f"{side}x{side} centre window of the same 1920x1080 image, 1 spp, FP64 oracle renderer "
-                      f"(FP64 sphere trace over every object program), {threads} threads, {dt:.1f} s"}
+                      f"(FP64 sphere trace over every object program), {threads} threads, {dt:.1f} s",
+            "single_thread": {"value": one_side * one_side / dt1 / 1e6, "unit": "Msamples/s", "cores": 1,
+                              "sample": f"{one_side}x{one_side} centre window, 1 spp, one thread, {dt1:.1f} s"}}
 
 
 def serving_stream(k, scenes, dev, jobs, rate, seed, max_paths, budget, sizes=None, spps=None, pool_texts=None, guided=False,
