@@ -143,6 +143,8 @@ void* guideOpen(Ctx& c, uint32_t seed, const float lower[3], const float size[3]
 void guideClose(void* guide);
 uint64_t guideBytes();
 uint32_t guideSteps(const void* guide);
+// whether the next round trains the guide (a served job's last slice does not)
+void guideTrainNext(void* guide, bool on);
 // the device bytes of a context's rounds' lobes and records (serving)
 uint64_t guideRoundBytes(const Ctx& c);
 

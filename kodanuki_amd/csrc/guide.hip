@@ -51,6 +51,7 @@ struct GuideState {
     float* scalars = nullptr;  // [0] weight scale, [1] loss, [2..] per-step skip flags (uint)
     DevBuf partial, lobes, recs, samples, flags, offsets, scan_tmp;
     uint32_t steps = 0;
+    bool train_next = true;  // the next round keeps training records for it (serving: not after a job's last slice)
     float loss_scale = 128.0f;
     kdn_guide_config config = defaultConfig();
     double last_loss = 0;
@@ -471,6 +472,8 @@ uint64_t guideBytes() {
 
 uint32_t guideSteps(const void* guide) { return ((const GuideState*)guide)->steps; }
 
+void guideTrainNext(void* guide, bool on) { ((GuideState*)guide)->train_next = on; }
+
 uint64_t guideRoundBytes(const Ctx& c) {
     const GuideRound* R = (const GuideRound*)c.guide_round;
     if (!R) return 0;
@@ -501,11 +504,12 @@ bool guidesBeginRound(Ctx& c, PassParams& P, std::vector<JobDev>& jobs, const st
         uint32_t stride = 1;
         while ((uint64_t)stride * G.config.train_paths < n) stride *= 2;
         jobs[j].learned = G.steps > 0 ? 1u : 0u;
+        anyLearned = anyLearned || G.steps > 0;
+        if (!G.train_next) continue;  // no records: nothing trains on this round
         jobs[j].train_stride = stride;
         jobs[j].path_first = one->first_path;
         jobs[j].rec_first = (uint32_t)blocks;
         blocks += (n + stride - 1) / stride;
-        anyLearned = anyLearned || G.steps > 0;
     }
     if (anyLearned && !ensure(R.lobes, N * kGuideOutputs * sizeof(float), c.error, false)) return false;
     const size_t recBytes = (size_t)std::max<uint64_t>(blocks, 1) * kGuideDepths * kRecFloats * sizeof(float);

@@ -163,7 +163,15 @@ int tooBig(kdn_server& s, uint64_t need) {
 // may pass the budget while it runs).
 int admit(kdn_server& s, kdn::Job& j) {
     const uint64_t px = (uint64_t)j.desc.width * j.desc.height;
-    const bool guided = (j.desc.flags & 1) != 0, learned = (j.desc.flags & 2) != 0;
+    const bool guided = (j.desc.flags & 1) != 0;
+    const uint32_t plane = kdn::planeOf(j.desc.width, j.desc.height);
+    // samples a slice, fixed by the job and the server alone, never by the
+    // other jobs
+    const uint64_t slicePaths = std::min<uint64_t>(kdn::kSlicePaths, s.maxPaths);
+    const uint32_t sliceSpp = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(j.desc.spp, slicePaths / plane));
+    // a learned job gets its guide when it has more than one slice (the
+    // guide of a job of one slice would never steer a sample)
+    const bool learned = (j.desc.flags & 2) != 0 && sliceSpp < j.desc.spp;
     // (a guided job's pixel guides, and its survey points' filing: about
     // 1 KB a point, one point to 64 pixels, and the 32^3 cell table; a
     // learned job's network, its Adam moments and gradients)
@@ -222,6 +230,7 @@ int admit(kdn_server& s, kdn::Job& j) {
             return failed(&s, message, KDN_E_DEVICE);
         }
     }
+    j.sliceSpp = sliceSpp;
     if (learned) {
         // the job's own guide over its scene's region, from the job's seed
         const kdn::Ctx& sc = res.ctx->c;
@@ -233,10 +242,6 @@ int admit(kdn_server& s, kdn::Job& j) {
             return failed(&s, message, KDN_E_MEMORY);
         }
     }
-    const uint32_t plane = kdn::planeOf(j.desc.width, j.desc.height);
-    // fixed by the job and the server alone, never by the other jobs
-    const uint64_t slicePaths = std::min<uint64_t>(kdn::kSlicePaths, s.maxPaths);
-    j.sliceSpp = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(j.desc.spp, slicePaths / plane));
     j.state = KDN_JOB_RUNNING;
     j.admitted = s.now();
     return KDN_OK;
@@ -378,6 +383,8 @@ int kdn_serve_step(kdn_server* s, kdn_serve_stats* stats) {
                 slices.push_back(kdn::SliceDev{(uint32_t)jobIndex[r], nextOf[r], spp, (uint32_t)paths, j.accum, j.accumSq, j.accumN, nullptr, 0});
                 paths += (uint64_t)spp * plane;
                 nextOf[r] += spp;
+                // a learned job's last slice: its guide is not trained after it
+                if (j.guide) kdn::guideTrainNext(j.guide, nextOf[r] < j.desc.spp);
                 added = true;
             }
         }

@@ -24,12 +24,12 @@ def note(msg):
 
 
 # (kind, width, height, spp, seed): at 65536 paths a round, 9 samples a
-# slice at 96x72 and 12 at 80x64, so those learned jobs take 2 to 3 slices
-# (a slice fills a round of 65536); the first job's one slice shares its
-# rounds, so a round with a learned job finishes several jobs' paths apart
-# past the guided depths
+# slice at 96x72, 12 at 80x64 and 256 at 16x16, so the learned jobs take 2
+# to 3 slices (a whole slice fills a round of 65536); the first job's last
+# slice (8 samples) shares its round, so a round with a learned job
+# finishes several jobs' paths apart past the guided depths
 JOBS = [
-    ("learned", 32, 32, 4, 6),
+    ("learned", 16, 16, 264, 6),
     ("plain", 64, 48, 6, 11),
     ("learned", 96, 72, 18 if EMULATION else 24, 3),
     ("learned", 80, 64, 24, 4),
@@ -78,8 +78,9 @@ def _served_alone(device, scene_set, job, learned=None, max_paths=1 << 16):
 
 def test_learned_job_trains_between_its_slices(device, scene_set):
     """Alone, a learned job runs one slice a round; the guide trains after
-    each, and its later slices sample with it: the image differs from the
-    same job without the guide, stays finite and near its mean."""
+    each but the last, and its later slices sample with it: the image
+    differs from the same job without the guide, stays finite and near its
+    mean."""
     job = JOBS[2]
     learned, rounds = _served_alone(device, scene_set, job)
     plain, _ = _served_alone(device, scene_set, job, learned=False)
@@ -88,7 +89,8 @@ def test_learned_job_trains_between_its_slices(device, scene_set):
     samples = [r.pass_.guide_samples for r in rounds if r.slices_in_round]
     note(f"[serve-learned] {len(steps)} rounds, guide steps {steps}, samples {samples}")
     assert len(steps) == -(-spp // 9) and all(r.slices_in_round <= 1 for r in rounds)
-    assert all(n > 0 for n in steps) and all(n > 0 for n in samples)
+    assert all(n > 0 for n in steps[:-1]) and all(n > 0 for n in samples[:-1])
+    assert steps[-1] == 0 and samples[-1] == 0  # nothing samples with a guide trained after the last slice
     assert np.all(learned[2] == spp) and np.all(np.isfinite(learned[0]))
     assert not _same(learned, plain)  # the later slices sampled with the guide
     mean_l = learned[0].sum() / learned[2].sum()
@@ -116,6 +118,16 @@ def test_learned_jobs_in_a_mix_match_jobs_alone(device, scene_set, mix):
             assert _same(server.result(j), alone[i]), f"job {i} {job} differs in the mix"
     note(f"[serve-learned] {mix}: {len(jobs)} jobs, up to {most} a round, each equal to its render alone")
     assert most >= (2 if mix == "small" else 3)
+
+
+def test_learned_job_of_one_slice_is_the_plain_job(device, scene_set):
+    """A learned job whose samples fit one slice never samples with a
+    guide: it gets none, and equals the same job without the flag."""
+    job = ("learned", 32, 24, 4, 9)
+    learned, rounds = _served_alone(device, scene_set, job)
+    plain, _ = _served_alone(device, scene_set, job, learned=False)
+    assert _same(learned, plain)
+    assert all(r.pass_.guide_steps == 0 for r in rounds)
 
 
 def test_learned_and_plain_guide_together_are_refused(device, scene_set):
