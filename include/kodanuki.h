@@ -729,10 +729,12 @@ typedef struct kdn_job_desc {
                                 image at admission; its guides are the job's
                                 memory, 736 B a pixel); bit 1: the learned
                                 guide (the job's own network over its scene's
-                                region from its seed, trained after each round
-                                on the job's own paths; one slice of the job a
-                                round; about 420 MB of the job's memory); not
-                                both */
+                                region from its seed, trained after each of its
+                                slices but the last on the job's own paths; one
+                                slice of the job a round, up to half a round on
+                                servers of 2^23 paths or more; none for a job of
+                                one slice; about 420 MB of the job's memory);
+                                not both */
     uint32_t pad;
 } kdn_job_desc;
 

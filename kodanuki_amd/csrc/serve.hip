@@ -167,7 +167,12 @@ int admit(kdn_server& s, kdn::Job& j) {
     const uint32_t plane = kdn::planeOf(j.desc.width, j.desc.height);
     // samples a slice, fixed by the job and the server alone, never by the
     // other jobs
-    const uint64_t slicePaths = std::min<uint64_t>(kdn::kSlicePaths, s.maxPaths);
+    uint64_t slicePaths = std::min<uint64_t>(kdn::kSlicePaths, s.maxPaths);
+    // a learned job's slices are up to half a round where a round holds four
+    // plain slices or more: it takes one slice a round, and fewer, larger
+    // slices (each trained on before the next) take it through fewer rounds
+    if ((j.desc.flags & 2) != 0 && s.maxPaths >= 4 * kdn::kSlicePaths)
+        slicePaths = std::min<uint64_t>(2 * kdn::kSlicePaths, s.maxPaths / 2);
     const uint32_t sliceSpp = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(j.desc.spp, slicePaths / plane));
     // a learned job gets its guide when it has more than one slice (the
     // guide of a job of one slice would never steer a sample)

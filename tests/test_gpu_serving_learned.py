@@ -35,12 +35,14 @@ JOBS = [
     ("learned", 80, 64, 24, 4),
     ("guided", 64, 48, 8, 5),
 ]
-# rounds of 2^23 paths hold several 2^21-path slices: learned jobs of 2
-# slices share rounds with each other and with plain and plain-guided jobs
+# rounds of 2^23 paths hold several 2^21-path slices, and a learned job's
+# slices are up to 2^22 paths there (606 samples at 96x72, 819 at 80x64):
+# learned jobs of 2 slices share rounds with each other and with plain and
+# plain-guided jobs
 SHARED = [
-    ("learned", 96, 72, 320, 3),
+    ("learned", 96, 72, 700, 3),
     ("plain", 64, 48, 6, 11),
-    ("learned", 80, 64, 420, 4),
+    ("learned", 80, 64, 900, 4),
     ("guided", 64, 48, 8, 5),
 ]
 
