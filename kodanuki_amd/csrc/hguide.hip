@@ -166,13 +166,13 @@ __global__ void k_hg_draws(HgScene S, const float* stands, uint32_t nPoints, kdn
 // Each survey point's seeds among its draws, in draw order (surveySeeds,
 // survey.cpp:487-529): a draw that meets a sharp metal is a seed unless
 // three seeds of its leaf are kept already or one of them lies within
-// 0.05 radians of its direction; sixteen at most, listed in
-// seeds[point * kMostSeeds ...] (the rest of its slots kNoSeed). Every
-// draw's candidate and aim start empty.
+// 0.05 radians of its direction; sixteen at most. The seeds' draws are
+// appended to seeds (*seedCount of them, in no fixed order: each walk
+// writes its own draw's candidate and aim alone). Every draw's candidate and
+// aim start empty.
 constexpr int kMostSeeds = 16;
-constexpr uint32_t kNoSeed = 0xFFFFFFFFu;
-__global__ void k_hg_seeds(HgScene S, const kdn_ray* rays, const kdn_hit* hits, uint32_t nPoints, uint32_t* seeds, kdn_ray* aims,
-                           float* cand) {
+__global__ void k_hg_seeds(HgScene S, const kdn_ray* rays, const kdn_hit* hits, uint32_t nPoints, uint32_t* seeds,
+                           uint32_t* seedCount, kdn_ray* aims, float* cand) {
     const uint32_t point = blockIdx.x * blockDim.x + threadIdx.x;
     if (point >= nPoints) return;
     constexpr int kSeedsPerLeaf = 3;
@@ -209,11 +209,10 @@ __global__ void k_hg_seeds(HgScene S, const kdn_ray* rays, const kdn_hit* hits, 
         if (same < kSeedsPerLeaf && !near && m.kind == KDN_METAL && m.width <= kShinyWidth) {
             leafOf[kept] = h.leaf;
             dirOf[kept] = d;
-            seeds[(size_t)point * kMostSeeds + kept] = (uint32_t)at;
+            seeds[atomicAdd(seedCount, 1u)] = (uint32_t)at;
             ++kept;
         }
     }
-    for (int k = kept; k < kMostSeeds; ++k) seeds[(size_t)point * kMostSeeds + k] = kNoSeed;
 }
 
 // A seed starts a highlight: for each lamp, the chain from the stand (its
@@ -223,13 +222,13 @@ __global__ void k_hg_seeds(HgScene S, const kdn_ray* rays, const kdn_hit* hits, 
 // settled ones with the lamp in front of the point, the most important
 // (the lamp's glare times the square root of the spread's determinant,
 // spotImportance, survey.cpp:582-585) is kept, and a ray from the stand
-// toward its point checks that the surface is there. One lane a seed slot.
-__global__ void k_hg_walks(HgScene S, const float* stands, const kdn_hit* hits, const uint32_t* seeds, uint32_t nSlots, kdn_ray* aims,
-                           float* cand) {
+// toward its point checks that the surface is there. One lane a seed (a
+// launch sized for the most seeds; the lanes past *seedCount end at once).
+__global__ void k_hg_walks(HgScene S, const float* stands, const kdn_hit* hits, const uint32_t* seeds, const uint32_t* seedCount,
+                           kdn_ray* aims, float* cand) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= nSlots) return;
+    if (slot >= *seedCount) return;
     const uint32_t at = seeds[slot];
-    if (at == kNoSeed) return;
     float* c = cand + (size_t)at * kSpotFloats;
     const kdn_hit h = hits[at];
     uint32_t mi;
@@ -547,7 +546,7 @@ bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats, DevBuf&
     const size_t nRays = std::max<size_t>(nDraws, nPixels);
     const size_t bytes = nRays * (sizeof(kdn_ray) + sizeof(kdn_hit)) + (size_t)nDraws * sizeof(kdn_ray) + (size_t)std::max<size_t>(nPoints, nPixels) * kStandFloats * 4 +
                          (size_t)nDraws * kSpotFloats * 4 + (size_t)nPoints * (1 + kSpots * kSpotFloats) * 4 +
-                         (size_t)nPoints * kMostSeeds * 4 + 9 * 256;
+                         (size_t)nPoints * kMostSeeds * 4 + 10 * 256;
     if (!ensure(work, bytes, c.error, false)) return false;
     char* at = (char*)work.ptr;
     auto take = [&](size_t b) {
@@ -562,6 +561,7 @@ bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats, DevBuf&
     float* cand = (float*)take((size_t)nDraws * kSpotFloats * 4);
     float* spots = (float*)take((size_t)nPoints * (1 + kSpots * kSpotFloats) * 4);
     uint32_t* seeds = (uint32_t*)take((size_t)nPoints * kMostSeeds * 4);
+    uint32_t* seedCount = (uint32_t*)take(4);
     const bool surveyed = nPoints > 0 && c.n_lamps > 0;
     const size_t pixelBytes = ((size_t)nPixels * kHgFloats * 4 + 255) & ~size_t(255);
     const size_t G3 = (size_t)kHgDeepGrid * kHgDeepGrid * kHgDeepGrid;
@@ -583,10 +583,11 @@ bool hguideBuild(Ctx& c, int width, int height, kdn_hguide_stats* stats, DevBuf&
         hipLaunchKernelGGL(k_hg_stands, grid(nPoints), dim3(kHgBlock), 0, st, S, (const kdn_ray*)rays, (const kdn_hit*)hits, nPoints, stands);
         hipLaunchKernelGGL(k_hg_draws, grid(nDraws), dim3(kHgBlock), 0, st, S, (const float*)stands, nPoints, rays);
         if (!extendRays(c, rays, nDraws, hits, st)) return false;
+        KDN_HIP(hipMemsetAsync(seedCount, 0, 4, st));
         hipLaunchKernelGGL(k_hg_seeds, grid(nPoints), dim3(kHgBlock), 0, st, S, (const kdn_ray*)rays, (const kdn_hit*)hits, nPoints,
-                           seeds, aims, cand);
+                           seeds, seedCount, aims, cand);
         hipLaunchKernelGGL(k_hg_walks, dim3((nPoints * kMostSeeds + 63) / 64), dim3(64), 0, st, S, (const float*)stands,
-                           (const kdn_hit*)hits, (const uint32_t*)seeds, nPoints * kMostSeeds, aims, cand);
+                           (const kdn_hit*)hits, (const uint32_t*)seeds, (const uint32_t*)seedCount, aims, cand);
         if (!extendRays(c, aims, nDraws, hits, st)) return false;
         hipLaunchKernelGGL(k_hg_candidates, grid(nDraws), dim3(kHgBlock), 0, st, S, (const kdn_ray*)aims, (const kdn_hit*)hits,
                            nDraws, cand);
