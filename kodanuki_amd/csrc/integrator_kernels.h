@@ -435,7 +435,10 @@ KDN_D uint32_t ray_key(const TreeView& tv, const kdn_ray& r) {
 // (Plain: the round has no guide of either kind and keeps no training
 // records, so the guided sampling is compiled out: the kernel keeps fewer
 // values live and spills fewer registers.)
-template <bool One = false, bool Plain = false>
+// (Lobes false: the caller shades no vertex the learned guide steers, as
+// the finishing launch, which runs paths past the guided depths only; the
+// lobes' sampling is compiled out.)
+template <bool One = false, bool Plain = false, bool Lobes = true>
 KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
     PathState& s = P.s;
     const size_t N = s.capacity;
@@ -529,7 +532,7 @@ KDN_D bool shade_path(PassParams& P, uint32_t i, bool* shadow) {
         const bool exact = m.kind == KDN_GLASS && m.exact;
         // the guide's lobes at this vertex (the same test as its inference)
         const bool guidable = depth < kGuideDepths && m.kind != KDN_GLASS;
-        const bool guided = !Plain && P.lobes != nullptr && J.learned != 0 && guidable;
+        const bool guided = !Plain && Lobes && P.lobes != nullptr && J.learned != 0 && guidable;
         GuideLobes lobes;
         if (guided) load_lobes(P.lobes, N, i, lobes);
         const float share = guided ? lobes.select : 0.0f;
@@ -799,7 +802,7 @@ KDN_D void finish_lane(PassParams& P, uint32_t inQueue, uint32_t count, const La
             s.hits[i] = h;
             ++segments;
             bool shadow;
-            const bool alive = shade_path<One, Plain>(P, i, &shadow);
+            const bool alive = shade_path<One, Plain, false>(P, i, &shadow);
             if (shadow) {
                 ++shadows;
                 shadowTurn = true;
